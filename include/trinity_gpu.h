@@ -1,0 +1,217 @@
+/*
+ * trinity_gpu.h -- C ABI of the MI355X (gfx950) k-mer hot path of Trinity.
+ *
+ * The reference boundary is process-level: the Perl driver runs the tools
+ * below with system("bash","-c",cmd) (Trinity:2910-2926) and reads their
+ * files.  This ABI is what those tools' inner loops become; the CLI shims in
+ * trinityrnaseq_amd/csrc/tools/ keep the reference command lines and file
+ * formats and call only these entry points.
+ *
+ *   reference tool / call site                             -> entry points
+ *   jellyfish count -m 25 -s S [--canonical] reads.fa
+ *        Trinity:2612-2619, insilico_read_normalization.pl:617-622
+ *                                                -> tk_reads_from_host, tk_estimate_distinct,
+ *                                                   tk_table_create, tk_count
+ *   jellyfish dump -L m db   (Trinity:2625, :641) -> tk_table_dump_size, tk_table_dump
+ *   jellyfish histo db       (Trinity:2630, :636) -> tk_table_histo
+ *   inchworm --kmers dump --DS --min_any_entropy E [--minKmerCount m]
+ *        Trinity:2654,2669-2711,1050             -> tk_table_load, tk_table_prune
+ *   fastaToKmerCoverageStats --reads --kmers --kmer_size --DS
+ *        insilico_read_normalization.pl:837-845  -> tk_read_cov_stats
+ *   nbkc_merge_left_right_stats.pl + nbkc_normalize.pl
+ *        nbkc_normalize.pl:67-127, nbkc_merge_left_right_stats.pl:80-145
+ *                                                -> tk_select_pairs
+ *   ReadsToTranscripts -i reads -f bundles -p P [-strand]
+ *        Trinity:2238-2250                       -> tk_ctable_build, tk_r2t_vote
+ *
+ * Conventions
+ *   - Every function returns TK_OK (0) or a negative tk_status; the message
+ *     of the last failure on the calling thread is tk_last_error().
+ *   - The caller owns every host buffer.  The library owns device objects
+ *     (tk_reads, tk_table, tk_ctable) until their *_free call.
+ *   - A tk_ctx is one device + one HIP stream; it is not re-entrant.  Use one
+ *     context per host thread; objects may be shared by contexts of the
+ *     same device once the producing call has returned.
+ *   - Functions with a _d suffix take/return DEVICE pointers and are
+ *     asynchronous on the context's stream (call tk_sync before reading).
+ *   - k-mer keys are 2-bit packed, first base most significant, A=0 C=1
+ *     G=2 T=3 (so numeric order = lexicographic order).  Canonical key =
+ *     min(kmer, reverse complement).  1 <= k <= 31.
+ */
+#ifndef TRINITY_GPU_H
+#define TRINITY_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TK_ABI_VERSION 1
+
+typedef enum {
+    TK_OK = 0,
+    TK_ERR_ARG = -1,    /* invalid argument or unsupported size            */
+    TK_ERR_HIP = -2,    /* HIP runtime error                               */
+    TK_ERR_OOM = -3,    /* device allocation failed                        */
+    TK_ERR_FULL = -4,   /* hash table capacity exhausted (grow and retry)  */
+    TK_ERR_STATE = -5,  /* object in the wrong state for this call         */
+    TK_ERR_CAP = -6     /* caller's output buffer too small                */
+} tk_status;
+
+typedef struct tk_ctx tk_ctx;
+typedef struct tk_reads tk_reads;
+typedef struct tk_table tk_table;
+typedef struct tk_ctable tk_ctable;
+
+/* ---------------- context ---------------- */
+const char* tk_last_error(void);
+int tk_abi_version(void);
+int tk_init(int device, tk_ctx** out);
+int tk_ctx_free(tk_ctx* ctx);
+int tk_sync(tk_ctx* ctx);
+/* Per-kernel HIP-event timing on the context's stream. */
+int tk_timing_enable(tk_ctx* ctx, int on);
+int tk_timing_reset(tk_ctx* ctx);
+/* Sum of event-measured durations and launch count of one kernel name. */
+int tk_timing_get(tk_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
+/* Device memory currently held by objects of this process (bytes). */
+int tk_device_bytes_in_use(uint64_t* bytes);
+
+/* ---------------- reads (device-resident, 2-bit packed) ---------------- */
+/*
+ * seq: concatenated read sequences (ASCII, no separators); offsets[n+1]:
+ * byte offset of each read in seq.  Bases A/C/G/T in either case are
+ * valid; any other byte (N, X, ...) is invalid and no window spans it.
+ * Replaces the per-read parse loops of the tools above (reads FASTA as
+ * produced by Trinity:2746-2880 / Fasta_reader.pm:45-73).
+ */
+int tk_reads_from_host(tk_ctx* ctx, const char* seq, const uint64_t* offsets,
+                       uint64_t n_reads, tk_reads** out);
+/* Same from device-resident buffers (d_seq needs >= offsets[n] bytes). */
+int tk_reads_from_device_d(tk_ctx* ctx, const char* d_seq, const uint64_t* d_offsets,
+                           uint64_t n_reads, tk_reads** out);
+int tk_reads_info(const tk_reads* r, uint64_t* n_reads, uint64_t* n_bases,
+                  uint32_t* max_len);
+/* Windows of length k over all reads: total (len-k+1 summed). */
+int tk_reads_windows(const tk_reads* r, int k, uint64_t* total);
+int tk_reads_free(tk_reads* r);
+
+/* ---------------- k-mer -> count table ---------------- */
+/*
+ * HyperLogLog estimate of the distinct (canonical) k-mers of a read set,
+ * used to size tables (jellyfish's -s hint, Trinity:2598-2606) and to admit
+ * jobs by device footprint.
+ */
+int tk_estimate_distinct(tk_ctx* ctx, const tk_reads* reads, int k, int canonical,
+                         double* estimate);
+/*
+ * Create an empty table for k-mers of length k that holds at least
+ * expected_distinct keys at load factor <= max_load (0 < max_load <= 0.9).
+ */
+int tk_table_create(tk_ctx* ctx, int k, int canonical, uint64_t expected_distinct,
+                    double max_load, tk_table** out);
+int tk_table_info(const tk_table* t, int* k, int* canonical, uint64_t* capacity,
+                  uint64_t* device_bytes);
+/* Count every N-free window of reads (jellyfish count).  Accumulates. */
+int tk_count(tk_ctx* ctx, tk_table* t, const tk_reads* reads);
+/* Number of keys whose count lies in [min_count, max_count]. */
+int tk_table_dump_size(tk_ctx* ctx, tk_table* t, uint32_t min_count, uint32_t max_count,
+                       uint64_t* n);
+/* Keys with count in [min_count, max_count], ascending key order (jellyfish dump -L -U). */
+int tk_table_dump(tk_ctx* ctx, tk_table* t, uint32_t min_count, uint32_t max_count,
+                  uint64_t* keys, uint32_t* counts, uint64_t cap, uint64_t* n_out);
+/*
+ * jellyfish histo -l low -h high -i inc: bins[i] counts keys with count in
+ * [base+i*inc, base+(i+1)*inc), base = low>1 ? (inc>=low ? 1 : low-inc) : 1;
+ * bin 0 also takes counts < base, the last bin counts >= high+inc.
+ * nbins must equal (high + 2*inc - base) / inc.
+ */
+int tk_table_histo(tk_ctx* ctx, tk_table* t, uint64_t low, uint64_t high, uint64_t inc,
+                   uint64_t* bins, uint64_t nbins);
+/* Insert (key,count) pairs, e.g. a parsed jellyfish dump (inchworm --kmers).
+ * Keys must already be canonical for a canonical table; counts add up. */
+int tk_table_load(tk_ctx* ctx, tk_table* t, const uint64_t* keys, const uint32_t* counts,
+                  uint64_t n);
+/*
+ * Inchworm prune: remove keys with count < min_count or whose base
+ * composition is rejected by keep_table ((k+1)^3 bytes, index
+ * (nA*(k+1)+nC)*(k+1)+nG, nonzero = keep).  NULL keep_table keeps all.
+ */
+int tk_table_prune(tk_ctx* ctx, tk_table* t, uint32_t min_count, const uint8_t* keep_table,
+                   uint64_t* n_kept);
+/* Look up keys (already canonical for a canonical table); 0 = absent. */
+int tk_table_lookup(tk_ctx* ctx, tk_table* t, const uint64_t* keys, uint64_t n,
+                    uint32_t* counts);
+int tk_table_free(tk_table* t);
+
+/* ---------------- per-read k-mer coverage (fastaToKmerCoverageStats) ---------------- */
+enum { TK_MEDIAN_UPPER = 0, TK_MEDIAN_LOWER = 1, TK_MEDIAN_MEAN_FLOOR = 2 };
+enum { TK_NWIN_AS_MISSING = 0, TK_NWIN_SKIP = 1 };
+
+typedef struct {
+    uint32_t min_count;      /* dictionary entries below this count as missing (dump -L)  */
+    uint32_t missing_value;  /* value of a missing k-mer (1: Changelog.txt:537)           */
+    int32_t nwin_policy;     /* TK_NWIN_*: windows holding a non-ACGT base                */
+    int32_t median_policy;   /* TK_MEDIAN_*: median of an even number of values           */
+} tk_stats_opts;
+
+typedef struct {
+    uint32_t median;
+    uint32_t nwin;           /* values that entered the statistics */
+    uint64_t sum;
+    uint64_t sumsq_lo;       /* 128-bit sum of squares */
+    uint64_t sumsq_hi;
+} tk_read_stat;
+
+/* One tk_read_stat per read, in read order. */
+int tk_read_cov_stats(tk_ctx* ctx, tk_table* t, const tk_reads* reads,
+                      const tk_stats_opts* opts, tk_read_stat* out);
+int tk_read_cov_stats_d(tk_ctx* ctx, tk_table* t, const tk_reads* reads,
+                        const tk_stats_opts* opts, tk_read_stat* d_out);
+
+/* ---------------- pair merge + normalization selection ---------------- */
+/*
+ * nbkc_merge_left_right_stats.pl --sorted followed by nbkc_normalize.pl
+ * (srand(12345); keep pair when rand(1) <= max_cov/median) for pairs that
+ * are already in sorted-accession order with left i paired to right i.
+ * Requires stats computed with missing_value >= 1 and
+ * max_cv >= 1.2*sqrt(max nwin) + 1 (then the coefficient-of-variation and
+ * zero-mean tests depend on nwin only and the selection is exact).
+ * d_selected[i] = 1 when pair i is kept.
+ */
+int tk_select_pairs_d(tk_ctx* ctx, const tk_read_stat* d_left, const tk_read_stat* d_right,
+                      uint64_t n_pairs, int max_cov, int min_cov, int max_cv,
+                      uint32_t max_nwin, uint8_t* d_selected, uint64_t* n_selected);
+int tk_select_pairs(tk_ctx* ctx, const tk_read_stat* left, const tk_read_stat* right,
+                    uint64_t n_pairs, int max_cov, int min_cov, int max_cv,
+                    uint8_t* selected, uint64_t* n_selected);
+
+/* ---------------- ReadsToTranscripts ---------------- */
+/*
+ * k-mer -> component table from bundle sequences (">s_<comp>" records of
+ * iworm contigs joined by 'X'; add_LR_reads_to_iworm_bundle.pl:26,34-42).
+ * comp_ids[i] is the component of bundle i.  A k-mer present in several
+ * components maps to the smallest id.
+ */
+int tk_ctable_build(tk_ctx* ctx, int k, int canonical, const tk_reads* bundles,
+                    const uint32_t* comp_ids, tk_ctable** out);
+int tk_ctable_info(const tk_ctable* c, uint64_t* n_keys, uint64_t* capacity);
+
+typedef struct {
+    uint32_t comp;   /* most-voted component (ties: smallest id); 0xFFFFFFFF = none */
+    uint32_t hits;   /* windows voting for comp */
+    uint32_t nwin;   /* all windows of the read, len-k+1 */
+} tk_r2t_vote;
+
+int tk_r2t_vote_reads(tk_ctx* ctx, const tk_ctable* c, const tk_reads* reads,
+                      tk_r2t_vote* out);
+int tk_r2t_vote_reads_d(tk_ctx* ctx, const tk_ctable* c, const tk_reads* reads,
+                        tk_r2t_vote* d_out);
+int tk_ctable_free(tk_ctable* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRINITY_GPU_H */

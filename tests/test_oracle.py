@@ -1,0 +1,200 @@
+"""Pins of the C oracle (CPU): brute force, closed forms and Perl's own rand()."""
+from __future__ import annotations
+
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import brute
+from data import concat, edge_reads, random_reads, repetitive_reads
+from oracle import oracle as O
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _as_dict(keys, counts, k):
+    return {brute.kmer_of(int(a), k): int(b) for a, b in zip(keys, counts)}
+
+
+@pytest.mark.parametrize("k", [1, 5, 13, 25, 31])
+def test_revcomp_matches_strings(k):
+    rng = np.random.default_rng(k)
+    for _ in range(200):
+        v = int(rng.integers(0, 1 << (2 * k), dtype=np.uint64))
+        s = brute.kmer_of(v, k)
+        assert brute.kmer_of(O.revcomp(v, k), k) == brute.rc(s)
+        assert O.revcomp(O.revcomp(v, k), k) == v
+
+
+@pytest.mark.parametrize("canonical", [True, False])
+@pytest.mark.parametrize("k", [5, 25])
+def test_count_matches_brute_force(k, canonical):
+    seqs = random_reads(7 + k, 300) + edge_reads() + repetitive_reads(3, 200)
+    seq, off = concat(seqs)
+    keys, counts = O.count(seq, off, k, canonical)
+    assert np.all(np.diff(keys.astype(np.int64)) > 0) if len(keys) > 1 else True
+    assert _as_dict(keys, counts, k) == dict(brute.count(seqs, k, canonical))
+
+
+def test_count_closed_forms():
+    k = 25
+    seqs = repetitive_reads(11, 500) + random_reads(12, 200)
+    seq, off = concat(seqs)
+    keys, counts = O.count(seq, off, k, True)
+    total, valid = O.window_totals(seq, off, k)
+    assert int(counts.sum()) == valid
+    assert total == sum(max(0, len(s) - k + 1) for s in seqs)
+    # canonical counting is invariant under reverse-complementing every read
+    seq2, off2 = concat([brute.rc(s) for s in seqs])
+    k2, c2 = O.count(seq2, off2, k, True)
+    assert np.array_equal(keys, k2) and np.array_equal(counts, c2)
+    # every canonical key is <= its reverse complement
+    for v in keys[:2000]:
+        assert int(v) <= O.revcomp(int(v), k)
+    h = O.histo(counts)
+    assert int(h.sum()) == len(keys)
+
+
+@pytest.mark.parametrize("low,high,inc", [(1, 10000, 1), (2, 50, 1), (1, 20, 3), (5, 40, 2), (3, 9, 5)])
+def test_histo_matches_brute_force(low, high, inc):
+    rng = np.random.default_rng(low * 100 + high)
+    counts = rng.integers(1, 80, size=3000).astype(np.uint32)
+    counts[:5] = [1, 10000, 10001, 123456, 2]
+    assert O.histo(counts, low, high, inc).tolist() == brute.histo(counts.tolist(), low, high, inc)
+
+
+def test_entropy_closed_forms():
+    k = 25
+    assert O.entropy_counts(25, 0, 0, 0, k) == 0.0
+    assert math.isclose(O.entropy_counts(12, 13, 0, 0, k),
+                        -(12 / 25) * math.log2(12 / 25) - (13 / 25) * math.log2(13 / 25))
+    # entropy of a composition is symmetric in the bases (DS-mode invariance)
+    assert O.entropy_counts(3, 5, 7, 10, k) == O.entropy_counts(10, 7, 5, 3, k)
+    for w in ["A" * 25, "ACGT" * 6 + "A", "GATTACA" * 3 + "GATT"]:
+        a, c, g, t = (w.count(b) for b in "ACGT")
+        assert math.isclose(O.entropy_counts(a, c, g, t, k), brute.entropy(w), rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("thr", [0.0, 1.0, 1.5, 1.9])
+def test_entropy_keep_table_is_robust(thr):
+    """No 25-mer composition lies within 1e-6 of the thresholds Trinity uses, so
+    float vs double entropy (an open detail of the reference) cannot flip a decision."""
+    k = 25
+    keep = O.entropy_keep_table(k, thr)
+    K1 = k + 1
+    for a in range(K1):
+        for c in range(K1 - a):
+            for g in range(K1 - a - c):
+                t = k - a - c - g
+                h = O.entropy_counts(a, c, g, t, k)
+                if thr > 0:
+                    assert abs(h - thr) > 1e-6
+                assert keep[(a * K1 + c) * K1 + g] == (h >= thr)
+    # float32 evaluation agrees everywhere
+    for a in range(K1):
+        for c in range(K1 - a):
+            for g in range(K1 - a - c):
+                t = k - a - c - g
+                hf = np.float32(0)
+                for x in (a, c, g, t):
+                    if x:
+                        p = np.float32(x) / np.float32(k)
+                        hf -= p * np.float32(np.log2(p))
+                assert (float(hf) >= thr) == bool(keep[(a * K1 + c) * K1 + g])
+
+
+def test_prune_matches_filter():
+    k = 25
+    seqs = repetitive_reads(5, 400, genome_len=300) + ["A" * 40, "AC" * 30, "AAAAAAAAAAAACCCCCCCCCCCCCC"]
+    seq, off = concat(seqs)
+    keys, counts = O.count(seq, off, k)
+    for mc, me in [(1, 0.0), (2, 1.0), (3, 1.5), (1, 1.9)]:
+        pk, pc = O.prune(keys, counts, k, mc, me)
+        exp = [(int(a), int(b)) for a, b in zip(keys, counts)
+               if b >= mc and brute.entropy(brute.kmer_of(int(a), k)) >= me]
+        assert list(zip(pk.tolist(), pc.tolist())) == exp
+
+
+@pytest.mark.parametrize("median_policy", [0, 1, 2])
+@pytest.mark.parametrize("nwin_policy", [0, 1])
+@pytest.mark.parametrize("min_count", [1, 2])
+def test_read_stats_matches_brute_force(median_policy, nwin_policy, min_count):
+    k = 25
+    seqs = repetitive_reads(21, 300) + edge_reads() + random_reads(22, 40, n_rate=0.05)
+    seq, off = concat(seqs)
+    keys, counts = O.count(seq, off, k)
+    d = _as_dict(keys, counts, k)
+    got = O.read_stats(seq, off, keys, counts, k, True, min_count, 1, nwin_policy, median_policy)
+    for s, g in zip(seqs, got):
+        e = brute.read_stats(s, d, k, True, min_count, 1, nwin_policy, median_policy)
+        assert (int(g["median"]), int(g["nwin"]), int(g["sum"])) == (e["median"], e["nwin"], e["sum"])
+        assert int(g["sumsq_lo"]) + (int(g["sumsq_hi"]) << 64) == e["sumsq"]
+
+
+def test_stats_row_format():
+    st = np.zeros(1, O.READ_STAT_DTYPE)[0]
+    st["median"], st["nwin"], st["sum"], st["sumsq_lo"] = 4, 3, 12, 4 * 4 + 3 * 3 + 5 * 5
+    row = O.format_stats_row("r1/1", st)
+    mean = 4.0
+    sd = math.sqrt((3 * 50 - 144)) / 3
+    assert row == "r1/1\t4\t%g\t%g\t%g\n" % (mean, sd, sd / mean * 100)
+
+
+def test_drand48_matches_perl_rand():
+    want = [float(x) for x in (GOLDEN / "perl_rand_12345.txt").read_text().split()]
+    got = O.drand48_stream(12345, len(want))
+    assert got == want
+    py = brute.Drand48(12345)
+    assert [py() for _ in range(len(want))] == want
+
+
+def test_perl_avg1_rounding():
+    assert O.perl_avg1(3, 4) == 3.5
+    assert O.perl_avg1(0.25, 0.0) == float("%.1f" % 0.125)
+    assert O.perl_avg1(12.3456, 7.1) == float("%.1f" % ((12.3456 + 7.1) / 2))
+
+
+def test_select_matches_perl_semantics():
+    rng = np.random.default_rng(5)
+    n = 5000
+    lm = rng.integers(0, 300, n)
+    rm = rng.integers(0, 300, n)
+    lu = np.round(rng.random(n) * 200, 4)
+    ru = np.round(rng.random(n) * 200, 4)
+    lu[:20] = 0
+    ru[:20] = 0
+    ls = np.round(rng.random(n) * 50, 4)
+    rs = np.round(rng.random(n) * 50, 4)
+    rows = [(str(a), repr(float(b)), repr(float(c)), str(d), repr(float(e)), repr(float(f)))
+            for a, b, c, d, e, f in zip(lm, lu, ls, rm, ru, rs)]
+    for max_cov, max_cv in [(50, 10000), (200, 10000), (30, 1)]:
+        want = brute.perl_select(rows, max_cov, 1, max_cv)
+        got = O.select_pairs(lm, rm, lu, ru, ls, rs, max_cov, 1, max_cv)
+        assert np.flatnonzero(got).tolist() == want
+
+
+@pytest.mark.parametrize("canonical", [True, False])
+def test_r2t_matches_brute_force(canonical):
+    k = 25
+    bundles = ["X".join(random_reads(40 + i, 3, 60, 300, n_rate=0.0, lower_rate=0.0)) for i in range(30)]
+    comps = list(range(100, 130))
+    # shared k-mers between components, and a duplicated bundle to exercise the min rule
+    bundles[5] = bundles[5] + "X" + bundles[9][:80]
+    bundles.append(bundles[3])
+    comps.append(7)
+    reads = []
+    rnd = np.random.default_rng(3)
+    for i in range(400):
+        b = bundles[int(rnd.integers(0, len(bundles)))]
+        s = int(rnd.integers(0, max(1, len(b) - 76)))
+        r = b[s:s + 76].replace("X", "N")
+        reads.append(brute.rc(r) if (canonical and rnd.random() < 0.5) else r)
+    reads += edge_reads()
+    bseq, boff = concat(bundles)
+    keys, cv = O.r2t_build(bseq, boff, comps, k, canonical)
+    rseq, roff = concat(reads)
+    got = O.r2t_vote(rseq, roff, keys, cv, k, canonical)
+    want = brute.r2t(bundles, comps, reads, k, canonical)
+    assert [(int(g["comp"]), int(g["hits"]), int(g["nwin"])) for g in got] == want

@@ -1,0 +1,333 @@
+"""ctypes binding of libtrinity_gpu.so (include/trinity_gpu.h) for tests and the bench.
+
+The command-line tools in trinityrnaseq_amd/bin are the drop-in replacements
+for the reference binaries; this module exposes the same entry points to
+Python.  Loading fails loudly when the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libtrinity_gpu.so"
+
+READ_STAT_DTYPE = np.dtype([("median", "<u4"), ("nwin", "<u4"), ("sum", "<u8"),
+                            ("sumsq_lo", "<u8"), ("sumsq_hi", "<u8")])
+R2T_VOTE_DTYPE = np.dtype([("comp", "<u4"), ("hits", "<u4"), ("nwin", "<u4")])
+
+MEDIAN_UPPER, MEDIAN_LOWER, MEDIAN_MEAN_FLOOR = 0, 1, 2
+NWIN_AS_MISSING, NWIN_SKIP = 0, 1
+
+STATUS = {0: "TK_OK", -1: "TK_ERR_ARG", -2: "TK_ERR_HIP", -3: "TK_ERR_OOM",
+          -4: "TK_ERR_FULL", -5: "TK_ERR_STATE", -6: "TK_ERR_CAP"}
+
+# name -> (restype, argtypes); the full exported surface of trinity_gpu.h
+_P = C.c_void_p
+_U64P = C.POINTER(C.c_uint64)
+_U32P = C.POINTER(C.c_uint32)
+SIGNATURES = {
+    "tk_last_error": (C.c_char_p, []),
+    "tk_abi_version": (C.c_int, []),
+    "tk_init": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "tk_ctx_free": (C.c_int, [_P]),
+    "tk_sync": (C.c_int, [_P]),
+    "tk_timing_enable": (C.c_int, [_P, C.c_int]),
+    "tk_timing_reset": (C.c_int, [_P]),
+    "tk_timing_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), _U64P]),
+    "tk_device_bytes_in_use": (C.c_int, [_U64P]),
+    "tk_reads_from_host": (C.c_int, [_P, C.c_char_p, _P, C.c_uint64, C.POINTER(_P)]),
+    "tk_reads_from_device_d": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(_P)]),
+    "tk_reads_info": (C.c_int, [_P, _U64P, _U64P, _U32P]),
+    "tk_reads_windows": (C.c_int, [_P, C.c_int, _U64P]),
+    "tk_reads_free": (C.c_int, [_P]),
+    "tk_estimate_distinct": (C.c_int, [_P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "tk_table_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_uint64, C.c_double, C.POINTER(_P)]),
+    "tk_table_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), _U64P, _U64P]),
+    "tk_count": (C.c_int, [_P, _P, _P]),
+    "tk_table_dump_size": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _U64P]),
+    "tk_table_dump": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint64, _U64P]),
+    "tk_table_histo": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64, C.c_uint64, _P, C.c_uint64]),
+    "tk_table_load": (C.c_int, [_P, _P, _P, _P, C.c_uint64]),
+    "tk_table_prune": (C.c_int, [_P, _P, C.c_uint32, _P, _U64P]),
+    "tk_table_lookup": (C.c_int, [_P, _P, _P, C.c_uint64, _P]),
+    "tk_table_free": (C.c_int, [_P]),
+    "tk_read_cov_stats": (C.c_int, [_P, _P, _P, _P, _P]),
+    "tk_read_cov_stats_d": (C.c_int, [_P, _P, _P, _P, _P]),
+    "tk_select_pairs_d": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                    C.c_uint32, _P, _U64P]),
+    "tk_select_pairs": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_int, C.c_int, C.c_int, _P, _U64P]),
+    "tk_ctable_build": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.POINTER(_P)]),
+    "tk_ctable_info": (C.c_int, [_P, _U64P, _U64P]),
+    "tk_r2t_vote_reads": (C.c_int, [_P, _P, _P, _P]),
+    "tk_r2t_vote_reads_d": (C.c_int, [_P, _P, _P, _P]),
+    "tk_ctable_free": (C.c_int, [_P]),
+}
+
+
+class StatsOpts(C.Structure):
+    _fields_ = [("min_count", C.c_uint32), ("missing_value", C.c_uint32),
+                ("nwin_policy", C.c_int32), ("median_policy", C.c_int32)]
+
+
+class TkError(RuntimeError):
+    def __init__(self, code: int, fn: str, msg: str):
+        super().__init__(f"{fn} -> {STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        l = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _check(rc: int, fn: str) -> None:
+    if rc != 0:
+        raise TkError(rc, fn, lib().tk_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data) if a.size else None
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        _check(lib().tk_init(device, C.byref(self.h)), "tk_init")
+
+    def close(self):
+        if self.h:
+            lib().tk_ctx_free(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        _check(lib().tk_sync(self.h), "tk_sync")
+
+    def timing(self, on: bool = True):
+        _check(lib().tk_timing_enable(self.h, int(on)), "tk_timing_enable")
+
+    def timing_reset(self):
+        _check(lib().tk_timing_reset(self.h), "tk_timing_reset")
+
+    def kernel_time(self, name: str) -> tuple[float, int]:
+        ms = C.c_double()
+        n = C.c_uint64()
+        _check(lib().tk_timing_get(self.h, name.encode(), C.byref(ms), C.byref(n)), "tk_timing_get")
+        return ms.value, n.value
+
+
+def pack_sequences(seqs) -> tuple[bytes, np.ndarray]:
+    """List of str/bytes -> (concatenated bytes, uint64 offsets[n+1])."""
+    bs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+    off = np.zeros(len(bs) + 1, dtype=np.uint64)
+    if bs:
+        off[1:] = np.cumsum([len(b) for b in bs], dtype=np.uint64)
+    return b"".join(bs), off
+
+
+class Reads:
+    def __init__(self, ctx: Context, seq: bytes, offsets: np.ndarray):
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.h = C.c_void_p()
+        self._ctx = ctx
+        _check(lib().tk_reads_from_host(ctx.h, seq, _ptr(offsets), len(offsets) - 1, C.byref(self.h)),
+               "tk_reads_from_host")
+
+    @classmethod
+    def from_device(cls, ctx: Context, d_seq: int, d_offsets: int, n: int) -> "Reads":
+        self = cls.__new__(cls)
+        self.h = C.c_void_p()
+        self._ctx = ctx
+        _check(lib().tk_reads_from_device_d(ctx.h, C.c_void_p(d_seq), C.c_void_p(d_offsets), n,
+                                            C.byref(self.h)), "tk_reads_from_device_d")
+        return self
+
+    @classmethod
+    def from_list(cls, ctx: Context, seqs) -> "Reads":
+        s, o = pack_sequences(seqs)
+        return cls(ctx, s, o)
+
+    def info(self):
+        n = C.c_uint64(); b = C.c_uint64(); m = C.c_uint32()
+        _check(lib().tk_reads_info(self.h, C.byref(n), C.byref(b), C.byref(m)), "tk_reads_info")
+        return n.value, b.value, m.value
+
+    def windows(self, k: int) -> int:
+        t = C.c_uint64()
+        _check(lib().tk_reads_windows(self.h, k, C.byref(t)), "tk_reads_windows")
+        return t.value
+
+    def free(self):
+        if self.h:
+            lib().tk_reads_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def estimate_distinct(ctx: Context, reads: Reads, k: int, canonical: bool = True) -> float:
+    e = C.c_double()
+    _check(lib().tk_estimate_distinct(ctx.h, reads.h, k, int(canonical), C.byref(e)),
+           "tk_estimate_distinct")
+    return e.value
+
+
+class Table:
+    def __init__(self, ctx: Context, k: int = 25, canonical: bool = True, expected: int = 1 << 16,
+                 max_load: float = 0.5):
+        self.ctx = ctx
+        self.k = k
+        self.h = C.c_void_p()
+        _check(lib().tk_table_create(ctx.h, k, int(canonical), int(expected), float(max_load),
+                                     C.byref(self.h)), "tk_table_create")
+
+    def info(self):
+        k = C.c_int(); can = C.c_int(); cap = C.c_uint64(); b = C.c_uint64()
+        _check(lib().tk_table_info(self.h, C.byref(k), C.byref(can), C.byref(cap), C.byref(b)),
+               "tk_table_info")
+        return {"k": k.value, "canonical": can.value, "capacity": cap.value, "bytes": b.value}
+
+    def count(self, reads: Reads):
+        _check(lib().tk_count(self.ctx.h, self.h, reads.h), "tk_count")
+
+    def dump_size(self, lo: int = 1, hi: int = 0xFFFFFFFF) -> int:
+        n = C.c_uint64()
+        _check(lib().tk_table_dump_size(self.ctx.h, self.h, lo, hi, C.byref(n)), "tk_table_dump_size")
+        return n.value
+
+    def dump(self, lo: int = 1, hi: int = 0xFFFFFFFF):
+        n = self.dump_size(lo, hi)
+        keys = np.empty(n, dtype=np.uint64)
+        counts = np.empty(n, dtype=np.uint32)
+        got = C.c_uint64()
+        _check(lib().tk_table_dump(self.ctx.h, self.h, lo, hi, _ptr(keys), _ptr(counts), n,
+                                   C.byref(got)), "tk_table_dump")
+        return keys[: got.value], counts[: got.value]
+
+    def histo(self, low: int = 1, high: int = 10000, inc: int = 1) -> np.ndarray:
+        base = (1 if inc >= low else low - inc) if low > 1 else 1
+        nb = (high + 2 * inc - base) // inc
+        bins = np.zeros(nb, dtype=np.uint64)
+        _check(lib().tk_table_histo(self.ctx.h, self.h, low, high, inc, _ptr(bins), nb),
+               "tk_table_histo")
+        return bins
+
+    def load(self, keys: np.ndarray, counts: np.ndarray):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        assert keys.shape == counts.shape
+        _check(lib().tk_table_load(self.ctx.h, self.h, _ptr(keys), _ptr(counts), len(keys)),
+               "tk_table_load")
+
+    def prune(self, min_count: int = 1, keep_table: np.ndarray | None = None) -> int:
+        kept = C.c_uint64()
+        kt = None
+        if keep_table is not None:
+            keep_table = np.ascontiguousarray(keep_table, dtype=np.uint8)
+            assert keep_table.size == (self.k + 1) ** 3
+            kt = _ptr(keep_table)
+        _check(lib().tk_table_prune(self.ctx.h, self.h, min_count, kt, C.byref(kept)), "tk_table_prune")
+        return kept.value
+
+    def lookup(self, keys: np.ndarray) -> np.ndarray:
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.zeros(len(keys), dtype=np.uint32)
+        _check(lib().tk_table_lookup(self.ctx.h, self.h, _ptr(keys), len(keys), _ptr(out)),
+               "tk_table_lookup")
+        return out
+
+    def free(self):
+        if self.h:
+            lib().tk_table_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def read_cov_stats(ctx: Context, table: Table, reads: Reads, min_count: int = 1,
+                   missing_value: int = 1, nwin_policy: int = NWIN_AS_MISSING,
+                   median_policy: int = MEDIAN_UPPER) -> np.ndarray:
+    n = reads.info()[0]
+    out = np.zeros(n, dtype=READ_STAT_DTYPE)
+    o = StatsOpts(min_count, missing_value, nwin_policy, median_policy)
+    _check(lib().tk_read_cov_stats(ctx.h, table.h, reads.h, C.byref(o), _ptr(out)),
+           "tk_read_cov_stats")
+    return out
+
+
+def select_pairs(ctx: Context, left: np.ndarray, right: np.ndarray, max_cov: int, min_cov: int = 1,
+                 max_cv: int = 10000) -> tuple[np.ndarray, int]:
+    left = np.ascontiguousarray(left, dtype=READ_STAT_DTYPE)
+    right = np.ascontiguousarray(right, dtype=READ_STAT_DTYPE)
+    assert left.shape == right.shape
+    sel = np.zeros(len(left), dtype=np.uint8)
+    ns = C.c_uint64()
+    _check(lib().tk_select_pairs(ctx.h, _ptr(left), _ptr(right), len(left), max_cov, min_cov, max_cv,
+                                 _ptr(sel), C.byref(ns)), "tk_select_pairs")
+    return sel, ns.value
+
+
+class CTable:
+    def __init__(self, ctx: Context, bundles: Reads, comp_ids: np.ndarray, k: int = 25,
+                 canonical: bool = True):
+        self.ctx = ctx
+        comp_ids = np.ascontiguousarray(comp_ids, dtype=np.uint32)
+        self.h = C.c_void_p()
+        _check(lib().tk_ctable_build(ctx.h, k, int(canonical), bundles.h, _ptr(comp_ids),
+                                     C.byref(self.h)), "tk_ctable_build")
+
+    def info(self):
+        n = C.c_uint64(); cap = C.c_uint64()
+        _check(lib().tk_ctable_info(self.h, C.byref(n), C.byref(cap)), "tk_ctable_info")
+        return n.value, cap.value
+
+    def vote(self, reads: Reads) -> np.ndarray:
+        n = reads.info()[0]
+        out = np.zeros(n, dtype=R2T_VOTE_DTYPE)
+        _check(lib().tk_r2t_vote_reads(self.ctx.h, self.h, reads.h, _ptr(out)), "tk_r2t_vote_reads")
+        return out
+
+    def free(self):
+        if self.h:
+            lib().tk_ctable_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_bytes_in_use() -> int:
+    b = C.c_uint64()
+    _check(lib().tk_device_bytes_in_use(C.byref(b)), "tk_device_bytes_in_use")
+    return b.value

@@ -1,0 +1,104 @@
+"""Build the gfx950 library, the CLI tools and the synthetic-data helper.
+
+Outputs (git-ignored, shipped with the source tree):
+  trinityrnaseq_amd/lib/libtrinity_gpu.so   HIP kernels + C ABI (include/trinity_gpu.h)
+  trinityrnaseq_amd/bin/*                   CLI tools keeping the reference command lines
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "lib"
+BIN = PKG / "bin"
+OBJ = PKG / "_build"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("TK_OFFLOAD_ARCH", "gfx950")
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # no packed-FP32 instructions anywhere in device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+    f"-I{ROOT / 'include'}",
+]
+CXX = os.environ.get("CXX", "g++")
+CXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-pthread", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+HIP_SOURCES = ["tk_lib.hip", "tk_sort.hip"]
+TOOLS = {
+    "jellyfish": ["tools/jellyfish_main.cpp"],
+    "fastaToKmerCoverageStats": ["tools/fastaToKmerCoverageStats_main.cpp"],
+    "inchworm": ["tools/inchworm_main.cpp"],
+    "ReadsToTranscripts": ["tools/ReadsToTranscripts_main.cpp"],
+    "nbkc_normalize": ["tools/nbkc_normalize_main.cpp"],
+}
+TOOL_COMMON = ["tools/seq_io.cpp", "tools/tool_util.cpp"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def _newer(out: Path, deps: list[Path]) -> bool:
+    return out.exists() and all(out.stat().st_mtime >= d.stat().st_mtime for d in deps)
+
+
+def build_library(force: bool = True) -> Path:
+    OBJ.mkdir(exist_ok=True)
+    LIB.mkdir(exist_ok=True)
+    headers = list(CSRC.glob("*.h")) + [ROOT / "include" / "trinity_gpu.h"]
+    objs = []
+    jobs = []
+    for src in HIP_SOURCES:
+        s = CSRC / src
+        o = OBJ / (s.stem + ".o")
+        objs.append(o)
+        if force or not _newer(o, [s] + headers):
+            jobs.append([HIPCC, *HIP_FLAGS, "-c", str(s), "-o", str(o)])
+    with ThreadPoolExecutor(max_workers=max(1, len(jobs))) as ex:
+        list(ex.map(_run, jobs))
+    out = LIB / "libtrinity_gpu.so"
+    tmp = LIB / "libtrinity_gpu.so.tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, out)
+    return out
+
+
+def build_tools(force: bool = True) -> list[Path]:
+    BIN.mkdir(exist_ok=True)
+    common = [CSRC / c for c in TOOL_COMMON]
+    outs = []
+    jobs = []
+    for name, srcs in TOOLS.items():
+        paths = [CSRC / s for s in srcs] + common
+        if not all(p.exists() for p in paths):
+            continue
+        out = BIN / name
+        outs.append(out)
+        deps = paths + list((CSRC / "tools").glob("*.h")) + [ROOT / "include" / "trinity_gpu.h"]
+        if force or not _newer(out, deps):
+            jobs.append([CXX, *CXX_FLAGS, *map(str, paths), "-o", str(out),
+                         f"-L{LIB}", "-ltrinity_gpu", "-Wl,-rpath,$ORIGIN/../lib"])
+    with ThreadPoolExecutor(max_workers=max(1, min(8, len(jobs)))) as ex:
+        list(ex.map(_run, jobs))
+    return outs
+
+
+def build_all(force: bool = True) -> None:
+    build_library(force)
+    build_tools(force)
+
+
+if __name__ == "__main__":
+    build_all(force="--incremental" not in sys.argv)
+    print("built", LIB, BIN)

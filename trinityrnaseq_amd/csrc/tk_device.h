@@ -1,0 +1,231 @@
+// tk_device.h -- device-side building blocks shared by the gfx950 kernels.
+//
+// Read layout in HBM (tk_reads):
+//   codes[w]  : 64-bit word, 32 bases, 2 bits each, first base in bits 63..62
+//   nmask[w]  : 32-bit word, bit 31-i set when base i of codes[w] is not ACGT
+//   word_off[r]: first word of read r (each read starts on a fresh word)
+//   len[r]    : bases in read r
+// One zero padding word follows the last read, so a window extractor may
+// always read words w and w+1.
+//
+// Count table (tk_table): open addressing over 64-byte buckets of eight
+// 64-bit slots.  A slot packs
+//     [ count field : 63-2k bits | occupied : 1 bit | key : 2k bits ]
+// so one 64-bit atomicAdd of (1 << (2k+1)) increments a k-mer and any carry
+// leaves through bit 63 without touching the key.  A carry (field wrap) is
+// recorded as 2^(63-2k) in a small side table keyed by k-mer ("ovf").
+// Finalisation folds the side table back: the main field of every key with
+// a side entry becomes all-ones and the side entry holds the full count.
+// A field of 0 on an occupied, finalised slot is a pruned key (tombstone).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tk {
+
+constexpr int kWave = 64;
+constexpr int kBucketSlots = 8;
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+// Reverse complement of a k-mer held in the low 2k bits.
+__device__ __forceinline__ uint64_t revcomp(uint64_t v, int k) {
+  uint64_t x = ~v;
+  x = ((x >> 2) & 0x3333333333333333ULL) | ((x & 0x3333333333333333ULL) << 2);
+  x = ((x >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((x & 0x0F0F0F0F0F0F0F0FULL) << 4);
+  x = __builtin_bswap64(x);
+  return x >> (64 - 2 * k);
+}
+
+struct ReadsView {
+  const uint64_t* codes;
+  const uint32_t* nmask;
+  const uint64_t* word_off;
+  const uint32_t* len;
+  uint64_t n_reads;
+};
+
+// Forward k-mer starting at base `pos` of the read whose first word is wbase.
+__device__ __forceinline__ void window_at(const ReadsView& rv, uint64_t wbase,
+                                          uint32_t pos, int k, uint64_t& fwd,
+                                          bool& valid) {
+  const uint64_t w = wbase + (pos >> 5);
+  const uint32_t o = pos & 31u;
+  const uint64_t hi = rv.codes[w];
+  const uint64_t lo = rv.codes[w + 1];
+  const uint32_t mhi = rv.nmask[w];
+  const uint32_t mlo = rv.nmask[w + 1];
+  const uint64_t x = o ? ((hi << (2 * o)) | (lo >> (64 - 2 * o))) : hi;
+  const uint32_t m = o ? ((mhi << o) | (mlo >> (32 - o))) : mhi;
+  fwd = x >> (64 - 2 * k);
+  valid = (m >> (32 - k)) == 0u;
+}
+
+__device__ __forceinline__ uint64_t canon_key(uint64_t fwd, int k, int canonical) {
+  if (!canonical) return fwd;
+  const uint64_t rc = revcomp(fwd, k);
+  return rc < fwd ? rc : fwd;
+}
+
+struct TableView {
+  uint64_t* slots;    // nbuckets * 8
+  uint64_t bmask;     // nbuckets - 1
+  uint64_t occ;       // occupied bit (1 << 2k)
+  uint64_t kmo;       // key + occupied bits
+  uint64_t cmax;      // all-ones count field value
+  uint32_t cshift;    // 2k + 1
+  uint32_t cbits;     // 63 - 2k
+  uint64_t* ovf;      // side table: pairs (tagged key, value)
+  uint64_t omask;     // side capacity - 1
+  uint32_t max_probe; // buckets probed before giving up
+};
+
+// ---- side table (field carries) ----
+__device__ __forceinline__ uint64_t ovf_slot0(const TableView& t, uint64_t tagged) {
+  return mix64(tagged ^ 0x9E3779B97F4A7C15ULL) & t.omask;
+}
+
+__device__ inline void ovf_add(const TableView& t, uint64_t tagged, uint64_t units,
+                               uint32_t* err) {
+  uint64_t i = ovf_slot0(t, tagged);
+  for (uint64_t p = 0; p <= t.omask; p++) {
+    unsigned long long* kp = (unsigned long long*)&t.ovf[2 * i];
+    uint64_t cur = *kp;
+    if (cur == 0) {
+      cur = atomicCAS(kp, 0ULL, (unsigned long long)tagged);
+      if (cur == 0) cur = tagged;
+    }
+    if (cur == tagged) {
+      atomicAdd((unsigned long long*)&t.ovf[2 * i + 1], (unsigned long long)units);
+      return;
+    }
+    i = (i + 1) & t.omask;
+  }
+  atomicOr(err, 2u);
+}
+
+__device__ inline uint64_t* ovf_find(const TableView& t, uint64_t tagged) {
+  uint64_t i = ovf_slot0(t, tagged);
+  for (uint64_t p = 0; p <= t.omask; p++) {
+    uint64_t cur = t.ovf[2 * i];
+    if (cur == tagged) return &t.ovf[2 * i + 1];
+    if (cur == 0) return nullptr;
+    i = (i + 1) & t.omask;
+  }
+  return nullptr;
+}
+
+// Add r (< 2^cbits) to the count field of slot `s` and q (a multiple of
+// 2^cbits) plus any carry to the side table.
+__device__ __forceinline__ void field_add(const TableView& t, uint64_t* s, uint64_t tagged,
+                                          uint64_t r, uint64_t q, uint32_t* err) {
+  if (r) {
+    const uint64_t old = atomicAdd((unsigned long long*)s, (unsigned long long)(r << t.cshift));
+    if ((old >> t.cshift) + r > t.cmax) q += t.cmax + 1;
+  }
+  if (q) ovf_add(t, tagged, q, err);
+}
+
+// Insert-or-add c >= 1 occurrences of key.  Returns 1 when a new slot was
+// claimed.  Sets err bit 0 when the probe limit is reached.
+__device__ inline uint32_t table_add(const TableView& t, uint64_t key, uint64_t c,
+                                     uint32_t* err) {
+  const uint64_t tagged = key | t.occ;
+  const uint64_t r = c & t.cmax;
+  const uint64_t q = c - r;
+  uint64_t b = mix64(key) & t.bmask;
+  for (uint32_t p = 0; p < t.max_probe; p++) {
+    uint64_t* line = t.slots + (b << 3);
+    const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(line);
+    ulonglong2 v0 = l2[0], v1 = l2[1], v2 = l2[2], v3 = l2[3];
+    uint64_t sv[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      uint64_t s = sv[j];
+      if (s == 0) {
+        s = atomicCAS((unsigned long long*)&line[j], 0ULL,
+                      (unsigned long long)(tagged | (r << t.cshift)));
+        if (s == 0) {
+          if (q) ovf_add(t, tagged, q, err);
+          return 1u;
+        }
+      }
+      if ((s & t.kmo) == tagged) {
+        field_add(t, &line[j], tagged, r, q, err);
+        return 0u;
+      }
+    }
+    b = (b + 1) & t.bmask;
+  }
+  atomicOr(err, 1u);
+  return 0u;
+}
+
+// Count of key in a finalised table; 0 when absent or pruned.
+__device__ __forceinline__ uint64_t table_get(const TableView& t, uint64_t key) {
+  const uint64_t tagged = key | t.occ;
+  uint64_t b = mix64(key) & t.bmask;
+  for (uint32_t p = 0; p < t.max_probe; p++) {
+    const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(t.slots + (b << 3));
+    ulonglong2 v0 = l2[0], v1 = l2[1], v2 = l2[2], v3 = l2[3];
+    uint64_t sv[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const uint64_t s = sv[j];
+      if ((s & t.kmo) == tagged) {
+        const uint64_t f = s >> t.cshift;
+        if (f != t.cmax) return f;
+        const uint64_t* o = ovf_find(t, tagged);
+        return o ? *o : f;
+      }
+      if (s == 0) return 0;
+    }
+    b = (b + 1) & t.bmask;
+  }
+  return 0;
+}
+
+// Count of the key held by an occupied slot value (finalised table).
+__device__ __forceinline__ uint64_t slot_count(const TableView& t, uint64_t s) {
+  const uint64_t f = s >> t.cshift;
+  if (f != t.cmax) return f;
+  const uint64_t* o = ovf_find(t, s & t.kmo);
+  return o ? *o : f;
+}
+
+// ---- wave helpers ----
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = __shfl_xor(v, o);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+}  // namespace tk

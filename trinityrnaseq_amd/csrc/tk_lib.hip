@@ -1,0 +1,1724 @@
+// tk_lib.hip -- gfx950 implementation of the C ABI in include/trinity_gpu.h.
+//
+// Kernels (all integer work; no MFMA):
+//   k_read_words / k_pack   ASCII reads -> 2-bit codes + non-ACGT mask
+//   k_hll                   HyperLogLog registers of canonical k-mers (table sizing)
+//   k_count                 jellyfish count: one thread per window, packed-slot table
+//   k_stats                 fastaToKmerCoverageStats: one wavefront per read,
+//                           LDS-staged window values, radix-select median
+//   k_select_*              nbkc merge + normalize with a jump-ahead drand48
+//   k_ctable_build / k_vote ReadsToTranscripts
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "tk_internal.h"
+
+namespace tk {
+
+static thread_local std::string g_err;
+std::atomic<uint64_t> g_dev_bytes{0};
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? TK_ERR_OOM : TK_ERR_HIP, "%s: %s", what,
+              hipGetErrorString(e));
+}
+
+int dev_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(e == hipErrorOutOfMemory ? TK_ERR_OOM : TK_ERR_HIP,
+                "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  }
+  g_dev_bytes += bytes;
+  return TK_OK;
+}
+
+void dev_free(void* p, size_t bytes) {
+  if (!p) return;
+  if (bytes == 0) bytes = 256;
+  (void)hipFree(p);
+  g_dev_bytes -= bytes;
+}
+
+int ensure_scratch(tk_ctx* ctx, int slot, size_t bytes, void** p) {
+  Scratch& s = ctx->scratch[slot];
+  if (s.bytes < bytes) {
+    if (s.p) {
+      (void)hipStreamSynchronize(ctx->stream);
+      dev_free(s.p, s.bytes);
+      s.p = nullptr;
+      s.bytes = 0;
+    }
+    size_t nb = std::max<size_t>(bytes, 1 << 20);
+    int rc = dev_alloc(&s.p, nb);
+    if (rc) return rc;
+    s.bytes = nb;
+  }
+  *p = s.p;
+  return TK_OK;
+}
+
+void timing_begin(tk_ctx* ctx, hipEvent_t* a) {
+  *a = nullptr;
+  if (!ctx->timing) return;
+  if (ctx->event_pool.empty()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    ctx->event_pool.push_back(e);
+  }
+  *a = ctx->event_pool.back();
+  ctx->event_pool.pop_back();
+  (void)hipEventRecord(*a, ctx->stream);
+}
+
+void timing_end(tk_ctx* ctx, const char* name, hipEvent_t a) {
+  if (!ctx->timing || !a) return;
+  hipEvent_t b;
+  if (ctx->event_pool.empty()) {
+    if (hipEventCreate(&b) != hipSuccess) return;
+  } else {
+    b = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+  }
+  (void)hipEventRecord(b, ctx->stream);
+  ctx->pending.push_back(PendingEvent{name, a, b});
+}
+
+int collect_timing(tk_ctx* ctx) {
+  if (ctx->pending.empty()) return TK_OK;
+  TK_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& p : ctx->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto& slot = ctx->times[p.name];
+      slot.first += ms;
+      slot.second += 1;
+    }
+    ctx->event_pool.push_back(p.a);
+    ctx->event_pool.push_back(p.b);
+  }
+  ctx->pending.clear();
+  return TK_OK;
+}
+
+#define TK_LAUNCH(ctx, name, grid, block, shmem, kernel, ...)            \
+  do {                                                                   \
+    hipEvent_t _a;                                                       \
+    ::tk::timing_begin((ctx), &_a);                                      \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), (shmem), (ctx)->stream, \
+                       __VA_ARGS__);                                     \
+    ::tk::timing_end((ctx), (name), _a);                                 \
+    hipError_t _le = hipGetLastError();                                  \
+    if (_le != hipSuccess) return ::tk::hip_fail(_le, "launch " name);   \
+  } while (0)
+
+static inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+static inline uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+static int set_device(int dev) {
+  TK_HIP(hipSetDevice(dev));
+  return TK_OK;
+}
+
+static ReadsView reads_view(const tk_reads* r) {
+  ReadsView v;
+  v.codes = r->d_codes;
+  v.nmask = r->d_nmask;
+  v.word_off = r->d_word_off;
+  v.len = r->d_len;
+  v.n_reads = r->n_reads;
+  return v;
+}
+
+TableView table_view(const tk_table* t) {
+  TableView v;
+  v.slots = t->d_slots;
+  v.bmask = t->nbuckets - 1;
+  v.occ = 1ull << (2 * t->k);
+  v.kmo = (v.occ << 1) - 1;
+  v.cshift = 2 * t->k + 1;
+  v.cbits = 63 - 2 * t->k;
+  v.cmax = (1ull << v.cbits) - 1;
+  v.ovf = t->d_ovf;
+  v.omask = t->ovf_cap - 1;
+  v.max_probe = (uint32_t)std::min<uint64_t>(t->nbuckets, 4096);
+  return v;
+}
+
+// =====================================================================
+// Read packing
+// =====================================================================
+__global__ void k_read_words(const uint64_t* __restrict__ off, uint64_t n,
+                             uint32_t* __restrict__ len, uint64_t* __restrict__ nwords,
+                             uint32_t* __restrict__ maxlen) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t L = 0;
+  if (i < n) {
+    uint64_t l64 = off[i + 1] - off[i];
+    L = l64 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)l64;
+    len[i] = L;
+    nwords[i] = (l64 + 31) / 32;
+  } else if (i == n) {
+    nwords[i] = 0;
+  }
+  L = wave_max_u32(L);
+  if ((threadIdx.x & 63) == 0 && L) atomicMax(maxlen, L);
+}
+
+__device__ __forceinline__ uint32_t base_code(uint8_t ch, uint32_t& bad) {
+  // A/a=0 C/c=1 G/g=2 T/t=3 ; everything else invalid
+  const uint8_t u = ch & 0xDF;  // upper-case letters
+  uint32_t c = 0;
+  bad = 0;
+  if (u == 'A') c = 0;
+  else if (u == 'C') c = 1;
+  else if (u == 'G') c = 2;
+  else if (u == 'T') c = 3;
+  else bad = 1;
+  if (ch < 'A') bad = 1;  // digits/punctuation whose low bits alias letters
+  return c;
+}
+
+// One thread per output word: 32 bases -> 64-bit code word + 32-bit mask.
+__global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
+                                              const uint64_t* __restrict__ off,
+                                              const uint64_t* __restrict__ word_off,
+                                              const uint32_t* __restrict__ len,
+                                              uint64_t n_reads, uint64_t n_words,
+                                              uint64_t* __restrict__ codes,
+                                              uint32_t* __restrict__ nmask) {
+  __shared__ uint64_t s_range[2];
+  const uint64_t w0 = (uint64_t)blockIdx.x * blockDim.x;
+  if (w0 >= n_words) return;
+  const uint64_t wl = std::min<uint64_t>(w0 + blockDim.x, n_words) - 1;
+  if (threadIdx.x < 2) {
+    const uint64_t target = threadIdx.x == 0 ? w0 : wl;
+    uint64_t lo = 0, hi = n_reads;  // largest r with word_off[r] <= target
+    while (hi - lo > 1) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (word_off[mid] <= target) lo = mid; else hi = mid;
+    }
+    s_range[threadIdx.x] = lo;
+  }
+  __syncthreads();
+  const uint64_t w = w0 + threadIdx.x;
+  if (w >= n_words) return;
+  uint64_t lo = s_range[0], hi = s_range[1] + 1;
+  while (hi - lo > 1) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (word_off[mid] <= w) lo = mid; else hi = mid;
+  }
+  const uint64_t r = lo;
+  const uint64_t j = w - word_off[r];
+  const uint32_t L = len[r];
+  const uint64_t b0 = j * 32;
+  const char* src = seq + off[r] + b0;
+  const uint32_t nb = (uint32_t)std::min<uint64_t>(32, L - b0);
+  uint64_t code = 0;
+  uint32_t mask = 0;
+  for (uint32_t i = 0; i < 32; i++) {
+    uint32_t bad = 1, c = 0;
+    if (i < nb) c = base_code((uint8_t)src[i], bad);
+    code = (code << 2) | c;
+    mask = (mask << 1) | bad;
+  }
+  codes[w] = code;
+  nmask[w] = mask;
+}
+
+__global__ void k_windows(const uint32_t* __restrict__ len, uint64_t n, int k,
+                          unsigned long long* __restrict__ total) {
+  uint64_t s = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t L = len[i];
+    s += L >= (uint32_t)k ? L - k + 1 : 0;
+  }
+  s = wave_sum_u64(s);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(total, (unsigned long long)s);
+}
+
+static int reads_build(tk_ctx* ctx, const char* d_seq, const uint64_t* d_off, uint64_t n,
+                       uint64_t n_bases, tk_reads** out) {
+  tk_reads* r = new tk_reads();
+  r->device = ctx->device;
+  r->n_reads = n;
+  r->n_bases = n_bases;
+  int rc = TK_OK;
+  uint64_t* d_nw = nullptr;
+  size_t nwb = (n + 1) * sizeof(uint64_t);
+  do {
+    if ((rc = dev_alloc((void**)&r->d_len, std::max<uint64_t>(n, 1) * 4))) break;
+    if ((rc = dev_alloc((void**)&r->d_word_off, nwb))) break;
+    if ((rc = dev_alloc((void**)&d_nw, nwb))) break;
+    hipError_t e = hipMemsetAsync(ctx->d_flags, 0, 64 * 4, ctx->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "memset flags"); break; }
+    {
+      uint64_t grid = ceil_div(n + 1, 256);
+      hipEvent_t a;
+      timing_begin(ctx, &a);
+      hipLaunchKernelGGL(k_read_words, dim3(grid), dim3(256), 0, ctx->stream, d_off, n,
+                         r->d_len, d_nw, ctx->d_flags);
+      timing_end(ctx, "read_words", a);
+      if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_read_words"); break; }
+    }
+    if ((rc = exclusive_scan_u64(ctx, d_nw, r->d_word_off, n + 1))) break;
+    uint64_t nwords = 0;
+    e = hipMemcpyAsync(&nwords, r->d_word_off + n, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->h_flags, ctx->d_flags, 64 * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "read words"); break; }
+    r->max_len = ctx->h_flags[0];
+    r->n_words = nwords;
+    if ((rc = dev_alloc((void**)&r->d_codes, (nwords + 1) * 8))) break;
+    if ((rc = dev_alloc((void**)&r->d_nmask, (nwords + 1) * 4))) break;
+    e = hipMemsetAsync(r->d_codes + nwords, 0, 8, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(r->d_nmask + nwords, 0xFF, 4, ctx->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "pad"); break; }
+    if (nwords) {
+      hipEvent_t a;
+      timing_begin(ctx, &a);
+      hipLaunchKernelGGL(k_pack, dim3(ceil_div(nwords, 256)), dim3(256), 0, ctx->stream,
+                         d_seq, d_off, r->d_word_off, r->d_len, n, nwords, r->d_codes,
+                         r->d_nmask);
+      timing_end(ctx, "pack", a);
+      if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_pack"); break; }
+    }
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "pack sync"); break; }
+  } while (0);
+  dev_free(d_nw, nwb);
+  r->bytes = std::max<uint64_t>(n, 1) * 4 + nwb + (r->n_words + 1) * 12;
+  if (rc) {
+    tk_reads_free(r);
+    return rc;
+  }
+  *out = r;
+  return TK_OK;
+}
+
+static int reads_windows(tk_ctx* ctx, const tk_reads* rc_, int k, uint64_t* total) {
+  tk_reads* r = const_cast<tk_reads*>(rc_);
+  auto it = r->windows.find(k);
+  if (it != r->windows.end()) { *total = it->second; return TK_OK; }
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(ctx->d_flags + 8);
+  TK_HIP(hipMemsetAsync(d, 0, 8, ctx->stream));
+  if (r->n_reads) {
+    uint64_t grid = std::min<uint64_t>(ceil_div(r->n_reads, 256), 4096);
+    TK_LAUNCH(ctx, "windows", grid, 256, 0, k_windows, r->d_len, r->n_reads, k, d);
+  }
+  uint64_t h = 0;
+  TK_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, ctx->stream));
+  TK_HIP(hipStreamSynchronize(ctx->stream));
+  r->windows[k] = h;
+  *total = h;
+  return TK_OK;
+}
+
+// =====================================================================
+// HyperLogLog sizing pass
+// =====================================================================
+constexpr int kHllBits = 12;
+constexpr int kHllRegs = 1 << kHllBits;
+
+__global__ __launch_bounds__(256) void k_hll(ReadsView rv, int k, int canonical,
+                                             uint32_t* __restrict__ regs) {
+  __shared__ uint32_t s_reg[kHllRegs];
+  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) s_reg[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint64_t wpb = blockDim.x >> 6;
+  for (uint64_t r = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); r < rv.n_reads;
+       r += (uint64_t)gridDim.x * wpb) {
+    const uint32_t L = rv.len[r];
+    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
+    const uint64_t wb = rv.word_off[r];
+    for (uint32_t p = lane; p < nwin; p += 64) {
+      uint64_t fwd;
+      bool valid;
+      window_at(rv, wb, p, k, fwd, valid);
+      if (!valid) continue;
+      const uint64_t h = mix64(canon_key(fwd, k, canonical) ^ 0x5bd1e9955bd1e995ULL);
+      const uint32_t idx = (uint32_t)(h >> (64 - kHllBits));
+      const uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1));
+      const uint32_t rho = (uint32_t)__clzll(rest) + 1;
+      atomicMax(&s_reg[idx], rho);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x)
+    if (s_reg[i]) atomicMax(&regs[i], s_reg[i]);
+}
+
+// =====================================================================
+// Counting
+// =====================================================================
+constexpr int kReadsPerBlock = 64;
+
+// One thread per k-window of a block of 64 consecutive reads.  flags[0] gets
+// error bits, flags[2..3] (u64) the number of newly claimed slots.
+__global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
+                                               uint64_t r_end, int k, int canonical,
+                                               TableView t, uint32_t* __restrict__ flags) {
+  __shared__ uint32_t s_off[kReadsPerBlock + 1];
+  __shared__ uint64_t s_wb[kReadsPerBlock];
+  __shared__ uint32_t s_new;
+  const uint64_t r0 = r_begin + (uint64_t)blockIdx.x * kReadsPerBlock;
+  if (r0 >= r_end) return;
+  const int nr = (int)std::min<uint64_t>(kReadsPerBlock, r_end - r0);
+  const int tid = threadIdx.x;
+  if (tid < kReadsPerBlock) {
+    uint32_t nw = 0;
+    if (tid < nr) {
+      const uint32_t L = rv.len[r0 + tid];
+      nw = L >= (uint32_t)k ? L - k + 1 : 0;
+      s_wb[tid] = rv.word_off[r0 + tid];
+    }
+    uint32_t x = nw;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o);
+      if (tid >= o) x += y;
+    }
+    s_off[tid + 1] = x;
+    if (tid == 0) { s_off[0] = 0; s_new = 0; }
+  }
+  __syncthreads();
+  const uint32_t total = s_off[nr];
+  uint32_t mynew = 0;
+  for (uint32_t w = tid; w < total; w += blockDim.x) {
+    int lo = 0, hi = nr;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (s_off[mid] <= w) lo = mid; else hi = mid;
+    }
+    uint64_t fwd;
+    bool valid;
+    window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
+    if (valid) mynew += table_add(t, canon_key(fwd, k, canonical), 1, &flags[0]);
+  }
+  mynew = wave_sum_u32(mynew);
+  if ((tid & 63) == 0 && mynew) atomicAdd(&s_new, mynew);
+  __syncthreads();
+  if (tid == 0 && s_new)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)s_new);
+}
+
+// Move every slot of an old table into a larger one (slot values unchanged).
+__global__ void k_rehash(const uint64_t* __restrict__ old, uint64_t n_slots, TableView t,
+                         uint32_t* __restrict__ flags) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = old[i];
+    if (!s) continue;
+    uint64_t b = mix64(s & (t.occ - 1)) & t.bmask;
+    bool done = false;
+    for (uint32_t p = 0; p < t.max_probe && !done; p++) {
+      uint64_t* line = t.slots + (b << 3);
+      for (int j = 0; j < 8; j++) {
+        if (line[j] == 0 &&
+            atomicCAS((unsigned long long*)&line[j], 0ULL, (unsigned long long)s) == 0ULL) {
+          done = true;
+          break;
+        }
+      }
+      b = (b + 1) & t.bmask;
+    }
+    if (!done) atomicOr(&flags[0], 1u);
+  }
+}
+
+__global__ void k_ovf_rehash(const uint64_t* __restrict__ old, uint64_t n_pairs, TableView t,
+                             uint32_t* __restrict__ flags) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = old[2 * i];
+    if (!key) continue;
+    ovf_add(t, key, old[2 * i + 1], &flags[0]);
+  }
+}
+
+__device__ inline uint64_t* table_find_slot(const TableView& t, uint64_t tagged) {
+  uint64_t b = mix64(tagged & (t.occ - 1)) & t.bmask;
+  for (uint32_t p = 0; p < t.max_probe; p++) {
+    uint64_t* line = t.slots + (b << 3);
+    for (int j = 0; j < 8; j++) {
+      const uint64_t s = line[j];
+      if ((s & t.kmo) == tagged) return &line[j];
+      if (s == 0) return nullptr;
+    }
+    b = (b + 1) & t.bmask;
+  }
+  return nullptr;
+}
+
+// Fold side-table carries into full counts (finalise) or back (reopen).
+__global__ void k_ovf_fold(TableView t, uint64_t n_pairs, int finalize,
+                           uint32_t* __restrict__ flags) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t tagged = t.ovf[2 * i];
+    if (!tagged) continue;
+    uint64_t* s = table_find_slot(t, tagged);
+    if (!s) { atomicOr(&flags[0], 4u); continue; }
+    if (finalize) {
+      const uint64_t total = t.ovf[2 * i + 1] + (*s >> t.cshift);
+      t.ovf[2 * i + 1] = total;
+      *s = total ? ((*s & t.kmo) | (t.cmax << t.cshift)) : (*s & t.kmo);
+    } else {
+      const uint64_t total = t.ovf[2 * i + 1];
+      const uint64_t r = total & t.cmax;
+      t.ovf[2 * i + 1] = total - r;
+      *s = (*s & t.kmo) | (r << t.cshift);
+    }
+  }
+}
+
+static int table_alloc_slots(tk_ctx* ctx, uint64_t nbuckets, uint64_t** p) {
+  int rc = dev_alloc((void**)p, nbuckets * 64);
+  if (rc) return rc;
+  TK_HIP(hipMemsetAsync(*p, 0, nbuckets * 64, ctx->stream));
+  return TK_OK;
+}
+
+static int table_ensure_ovf(tk_ctx* ctx, tk_table* t, uint64_t bound) {
+  uint64_t need = next_pow2(std::max<uint64_t>(4096, 2 * bound + 64));
+  if (t->d_ovf && t->ovf_cap >= need) return TK_OK;
+  uint64_t* nov = nullptr;
+  int rc = dev_alloc((void**)&nov, need * 16);
+  if (rc) return rc;
+  TK_HIP(hipMemsetAsync(nov, 0, need * 16, ctx->stream));
+  uint64_t* old = t->d_ovf;
+  uint64_t oldcap = t->ovf_cap;
+  t->d_ovf = nov;
+  t->ovf_cap = need;
+  if (old) {
+    TK_HIP(hipMemsetAsync(ctx->d_flags, 0, 4, ctx->stream));
+    TableView v = table_view(t);
+    TK_LAUNCH(ctx, "ovf_rehash", std::min<uint64_t>(ceil_div(oldcap, 256), 8192), 256, 0,
+              k_ovf_rehash, old, oldcap, v, ctx->d_flags);
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    dev_free(old, oldcap * 16);
+  }
+  t->bytes = t->nbuckets * 64 + t->ovf_cap * 16;
+  return TK_OK;
+}
+
+static int table_grow(tk_ctx* ctx, tk_table* t, uint64_t min_keys) {
+  uint64_t nb = t->nbuckets;
+  while ((double)min_keys > t->max_load * (double)(nb * 8)) nb <<= 1;
+  if (nb == t->nbuckets) return TK_OK;
+  uint64_t* ns = nullptr;
+  int rc = table_alloc_slots(ctx, nb, &ns);
+  if (rc) return rc;
+  uint64_t* old = t->d_slots;
+  uint64_t oldnb = t->nbuckets;
+  t->d_slots = ns;
+  t->nbuckets = nb;
+  TK_HIP(hipMemsetAsync(ctx->d_flags, 0, 4, ctx->stream));
+  TableView v = table_view(t);
+  TK_LAUNCH(ctx, "rehash", std::min<uint64_t>(ceil_div(oldnb * 8, 256), 65536), 256, 0,
+            k_rehash, old, oldnb * 8, v, ctx->d_flags);
+  TK_HIP(hipMemcpyAsync(ctx->h_flags, ctx->d_flags, 4, hipMemcpyDeviceToHost, ctx->stream));
+  TK_HIP(hipStreamSynchronize(ctx->stream));
+  dev_free(old, oldnb * 64);
+  t->bytes = t->nbuckets * 64 + t->ovf_cap * 16;
+  if (ctx->h_flags[0]) return fail(TK_ERR_FULL, "rehash into %llu buckets failed",
+                                   (unsigned long long)nb);
+  return TK_OK;
+}
+
+static int table_set_final(tk_ctx* ctx, tk_table* t, bool final_) {
+  if (t->finalized == final_) return TK_OK;
+  TK_HIP(hipMemsetAsync(ctx->d_flags, 0, 4, ctx->stream));
+  TableView v = table_view(t);
+  TK_LAUNCH(ctx, "ovf_fold", std::min<uint64_t>(ceil_div(t->ovf_cap, 256), 8192), 256, 0,
+            k_ovf_fold, v, t->ovf_cap, final_ ? 1 : 0, ctx->d_flags);
+  TK_HIP(hipMemcpyAsync(ctx->h_flags, ctx->d_flags, 4, hipMemcpyDeviceToHost, ctx->stream));
+  TK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->h_flags[0]) return fail(TK_ERR_STATE, "side-table key missing from main table");
+  t->finalized = final_;
+  return TK_OK;
+}
+
+// Thread per (key,count) pair.
+__global__ void k_load(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ counts,
+                       uint64_t n, TableView t, uint32_t* __restrict__ flags) {
+  uint32_t mynew = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = counts[i];
+    const uint64_t key = keys[i];
+    if (c == 0) continue;
+    if (key & ~(t.occ - 1)) { atomicOr(&flags[0], 8u); continue; }
+    mynew += table_add(t, key, c, &flags[0]);
+  }
+  mynew = wave_sum_u32(mynew);
+  if ((threadIdx.x & 63) == 0 && mynew)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)mynew);
+}
+
+// =====================================================================
+// Dump / histo / prune / lookup over a finalised table
+// =====================================================================
+__global__ void k_dump_count(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
+                             unsigned long long* __restrict__ out) {
+  uint64_t c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = t.slots[i];
+    if (!s || (s >> t.cshift) == 0) continue;
+    const uint64_t v = slot_count(t, s);
+    c += (v >= lo && v <= hi);
+  }
+  c = wave_sum_u64(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+__global__ void k_dump(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
+                       uint64_t* __restrict__ keys, uint32_t* __restrict__ counts,
+                       uint64_t cap, unsigned long long* __restrict__ cursor) {
+  const int lane = threadIdx.x & 63;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_slots;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    bool take = false;
+    uint64_t key = 0, v = 0;
+    if (i < n_slots) {
+      const uint64_t s = t.slots[i];
+      if (s && (s >> t.cshift) != 0) {
+        v = slot_count(t, s);
+        take = v >= lo && v <= hi;
+        key = s & (t.occ - 1);
+      }
+    }
+    const uint64_t m = __ballot(take);
+    if (!m) continue;
+    unsigned long long start = 0;
+    if (lane == 0) start = atomicAdd(cursor, (unsigned long long)__popcll(m));
+    start = __shfl(start, 0);
+    if (take) {
+      const uint64_t pos = start + __popcll(m & ((1ull << lane) - 1));
+      if (pos < cap) {
+        keys[pos] = key;
+        counts[pos] = v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v;
+      }
+    }
+  }
+}
+
+constexpr int kHistoLds = 4096;
+
+__global__ __launch_bounds__(256) void k_histo(TableView t, uint64_t n_slots, uint64_t base,
+                                               uint64_t ceil_, uint64_t inc, uint64_t nb,
+                                               unsigned long long* __restrict__ bins) {
+  __shared__ uint32_t s_bins[kHistoLds];
+  const bool lds = nb <= kHistoLds;
+  if (lds) {
+    for (int i = threadIdx.x; i < kHistoLds; i += blockDim.x) s_bins[i] = 0;
+    __syncthreads();
+  }
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = t.slots[i];
+    if (!s || (s >> t.cshift) == 0) continue;
+    const uint64_t v = slot_count(t, s);
+    uint64_t b;
+    if (v < base) b = 0;
+    else if (v >= ceil_) b = nb - 1;
+    else b = (v - base) / inc;
+    if (lds) atomicAdd(&s_bins[b], 1u);
+    else atomicAdd(&bins[b], 1ull);
+  }
+  if (lds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)nb; i += blockDim.x)
+      if (s_bins[i]) atomicAdd(&bins[i], (unsigned long long)s_bins[i]);
+  }
+}
+
+__device__ __forceinline__ void composition(uint64_t key, int k, int& a, int& c, int& g) {
+  const uint64_t M = (k == 32) ? 0x5555555555555555ULL
+                               : (0x5555555555555555ULL & ((1ull << (2 * k)) - 1));
+  const uint64_t lo = key & M, hi = (key >> 1) & M;
+  const int nt = __popcll(hi & lo);
+  g = __popcll(hi & ~lo & M);
+  c = __popcll(~hi & lo & M);
+  a = k - nt - g - c;
+}
+
+__global__ void k_prune(TableView t, uint64_t n_slots, int k, uint64_t min_count,
+                        const uint8_t* __restrict__ keep, unsigned long long* __restrict__ kept) {
+  uint64_t nk = 0;
+  const int K1 = k + 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = t.slots[i];
+    if (!s) continue;
+    const uint64_t f = s >> t.cshift;
+    if (f == 0) continue;
+    const uint64_t v = slot_count(t, s);
+    bool ok = v >= min_count;
+    if (ok && keep) {
+      int a, c, g;
+      composition(s & (t.occ - 1), k, a, c, g);
+      ok = keep[(a * K1 + c) * K1 + g] != 0;
+    }
+    if (ok) {
+      nk++;
+    } else {
+      if (f == t.cmax) {
+        uint64_t* o = ovf_find(t, s & t.kmo);
+        if (o) *o = 0;
+      }
+      t.slots[i] = s & t.kmo;
+    }
+  }
+  nk = wave_sum_u64(nk);
+  if ((threadIdx.x & 63) == 0 && nk) atomicAdd(kept, (unsigned long long)nk);
+}
+
+__global__ void k_lookup(TableView t, const uint64_t* __restrict__ keys, uint64_t n,
+                         uint32_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = keys[i];
+    uint64_t v = (key & ~(t.occ - 1)) ? 0 : table_get(t, key);
+    out[i] = v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+}
+
+// =====================================================================
+// Per-read coverage statistics: one wavefront per read
+// =====================================================================
+struct StatsParams {
+  uint32_t min_count;
+  uint32_t missing;
+  int32_t nwin_policy;
+  int32_t median_policy;
+};
+
+// kth smallest (0-based) of vals[0..n) by radix select over bits hb..0.
+__device__ uint32_t wave_select(const uint32_t* vals, uint32_t n, uint32_t kth, int hb,
+                                int lane) {
+  uint32_t prefix = 0, pmask = 0;
+  for (int bit = hb; bit >= 0; --bit) {
+    uint32_t c = 0;
+    for (uint32_t i = lane; i < n; i += 64) {
+      const uint32_t v = vals[i];
+      c += (((v & pmask) == prefix) && !((v >> bit) & 1u)) ? 1u : 0u;
+    }
+    c = wave_sum_u32(c);
+    if (kth >= c) {
+      kth -= c;
+      prefix |= 1u << bit;
+    }
+    pmask |= 1u << bit;
+  }
+  return prefix;
+}
+
+__global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonical, TableView t,
+                                               StatsParams sp, tk_read_stat* __restrict__ out,
+                                               uint32_t slab) {
+  extern __shared__ uint32_t s_dyn[];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const uint64_t wpb = blockDim.x >> 6;
+  uint32_t* vals = s_dyn + (uint64_t)wv * slab;
+  for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
+       r += (uint64_t)gridDim.x * wpb) {
+    const uint32_t L = rv.len[r];
+    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
+    const uint64_t wb = rv.word_off[r];
+    uint32_t n = 0, vmax = 0;
+    uint64_t sum = 0, sqlo = 0, sqhi = 0;
+    for (uint32_t base = 0; base < nwin; base += 64) {
+      const uint32_t p = base + lane;
+      bool has = false;
+      uint32_t v = 0;
+      if (p < nwin) {
+        uint64_t fwd;
+        bool valid;
+        window_at(rv, wb, p, k, fwd, valid);
+        if (valid) {
+          const uint64_t c = table_get(t, canon_key(fwd, k, canonical));
+          v = (c != 0 && c >= sp.min_count) ? (c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c)
+                                            : sp.missing;
+          has = true;
+        } else if (sp.nwin_policy == TK_NWIN_AS_MISSING) {
+          v = sp.missing;
+          has = true;
+        }
+      }
+      const uint64_t m = __ballot(has);
+      if (has) {
+        vals[n + __popcll(m & ((1ull << lane) - 1))] = v;
+        sum += v;
+        const uint64_t sq = (uint64_t)v * v;
+        sqlo += sq;
+        sqhi += (sqlo < sq);
+        vmax = v > vmax ? v : vmax;
+      }
+      n += (uint32_t)__popcll(m);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    sum = wave_sum_u64(sum);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t ol = __shfl_xor(sqlo, o);
+      const uint64_t oh = __shfl_xor(sqhi, o);
+      const uint64_t nl = sqlo + ol;
+      sqhi = sqhi + oh + (nl < sqlo);
+      sqlo = nl;
+    }
+    uint32_t med = 0;
+    if (n) {
+      vmax = wave_max_u32(vmax);
+      const int hb = 31 - __clz(vmax | 1u);
+      if (sp.median_policy == TK_MEDIAN_LOWER) {
+        med = wave_select(vals, n, (n - 1) / 2, hb, lane);
+      } else if (sp.median_policy == TK_MEDIAN_MEAN_FLOOR && !(n & 1)) {
+        const uint64_t a = wave_select(vals, n, n / 2 - 1, hb, lane);
+        const uint64_t b = wave_select(vals, n, n / 2, hb, lane);
+        med = (uint32_t)((a + b) / 2);
+      } else {
+        med = wave_select(vals, n, n / 2, hb, lane);
+      }
+    }
+    if (lane == 0) {
+      tk_read_stat st;
+      st.median = med;
+      st.nwin = n;
+      st.sum = sum;
+      st.sumsq_lo = sqlo;
+      st.sumsq_hi = sqhi;
+      out[r] = st;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// =====================================================================
+// Pair selection (nbkc_merge_left_right_stats.pl + nbkc_normalize.pl)
+// =====================================================================
+struct Lcg48Jump {
+  uint64_t a[48];
+  uint64_t c[48];
+};
+
+__global__ void k_select_flags(const tk_read_stat* __restrict__ L, const tk_read_stat* __restrict__ R,
+                               uint64_t n, int min_cov, uint32_t* __restrict__ consumes) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const double med = ((double)L[i].median + (double)R[i].median) / 2.0;
+    bool use = !(med < (double)min_cov);
+    if (use && L[i].nwin == 0 && R[i].nwin == 0) use = false;  // merged mean "0.0" <= 0
+    consumes[i] = use ? 1u : 0u;
+  }
+}
+
+__global__ void k_select(const tk_read_stat* __restrict__ L, const tk_read_stat* __restrict__ R,
+                         uint64_t n, const uint32_t* __restrict__ consumes,
+                         const uint32_t* __restrict__ idx, Lcg48Jump jump, uint64_t state0,
+                         int max_cov, uint8_t* __restrict__ sel,
+                         unsigned long long* __restrict__ nsel) {
+  const uint64_t M = (1ull << 48) - 1;
+  uint64_t cnt = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint8_t s = 0;
+    if (consumes[i]) {
+      uint64_t steps = (uint64_t)idx[i] + 1;
+      uint64_t x = state0;
+      for (int j = 0; steps; j++, steps >>= 1)
+        if (steps & 1) x = (jump.a[j] * x + jump.c[j]) & M;
+      const double r = (double)x * 0x1p-48;
+      const double med = ((double)L[i].median + (double)R[i].median) / 2.0;
+      s = (r <= (double)max_cov / med) ? 1 : 0;
+    }
+    sel[i] = s;
+    cnt += s;
+  }
+  cnt = wave_sum_u64(cnt);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(nsel, (unsigned long long)cnt);
+}
+
+// =====================================================================
+// ReadsToTranscripts
+// =====================================================================
+// ctable: 64-byte buckets of four 16-byte slots (tagged key, ~comp).
+__device__ inline void ctable_add(uint64_t* slots, uint64_t bmask, uint64_t occ, uint64_t key,
+                                  uint32_t comp, uint32_t& mynew, uint32_t* err) {
+  const uint64_t tagged = key | occ;
+  uint64_t b = mix64(key) & bmask;
+  for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
+    uint64_t* line = slots + (b << 3);
+    for (int j = 0; j < 4; j++) {
+      uint64_t s = line[2 * j];
+      if (s == 0) {
+        s = atomicCAS((unsigned long long*)&line[2 * j], 0ULL, (unsigned long long)tagged);
+        if (s == 0) { mynew++; s = tagged; }
+      }
+      if (s == tagged) {
+        atomicMax((unsigned int*)&line[2 * j + 1], ~comp);
+        return;
+      }
+    }
+    b = (b + 1) & bmask;
+  }
+  atomicOr(err, 1u);
+}
+
+__device__ __forceinline__ uint32_t ctable_get(const uint64_t* slots, uint64_t bmask, uint64_t occ,
+                                               uint64_t key) {
+  const uint64_t tagged = key | occ;
+  uint64_t b = mix64(key) & bmask;
+  for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
+    const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(slots + (b << 3));
+    ulonglong2 v[4] = {l2[0], l2[1], l2[2], l2[3]};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (v[j].x == tagged) return ~(uint32_t)v[j].y;
+      if (v[j].x == 0) return 0xFFFFFFFFu;
+    }
+    b = (b + 1) & bmask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(256) void k_ctable_build(ReadsView rv, const uint32_t* __restrict__ comp,
+                                                      int k, int canonical, uint64_t* slots,
+                                                      uint64_t bmask, uint32_t* __restrict__ flags) {
+  __shared__ uint32_t s_off[kReadsPerBlock + 1];
+  __shared__ uint64_t s_wb[kReadsPerBlock];
+  __shared__ uint32_t s_comp[kReadsPerBlock];
+  __shared__ uint32_t s_new;
+  const uint64_t r0 = (uint64_t)blockIdx.x * kReadsPerBlock;
+  if (r0 >= rv.n_reads) return;
+  const int nr = (int)std::min<uint64_t>(kReadsPerBlock, rv.n_reads - r0);
+  const int tid = threadIdx.x;
+  if (tid < kReadsPerBlock) {
+    uint32_t nw = 0;
+    if (tid < nr) {
+      const uint32_t L = rv.len[r0 + tid];
+      nw = L >= (uint32_t)k ? L - k + 1 : 0;
+      s_wb[tid] = rv.word_off[r0 + tid];
+      s_comp[tid] = comp[r0 + tid];
+    }
+    uint32_t x = nw;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o);
+      if (tid >= o) x += y;
+    }
+    s_off[tid + 1] = x;
+    if (tid == 0) { s_off[0] = 0; s_new = 0; }
+  }
+  __syncthreads();
+  const uint64_t occ = 1ull << (2 * k);
+  const uint32_t total = s_off[nr];
+  uint32_t mynew = 0;
+  for (uint32_t w = tid; w < total; w += blockDim.x) {
+    int lo = 0, hi = nr;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (s_off[mid] <= w) lo = mid; else hi = mid;
+    }
+    uint64_t fwd;
+    bool valid;
+    window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
+    if (valid) ctable_add(slots, bmask, occ, canon_key(fwd, k, canonical), s_comp[lo], mynew, &flags[0]);
+  }
+  mynew = wave_sum_u32(mynew);
+  if ((tid & 63) == 0 && mynew) atomicAdd(&s_new, mynew);
+  __syncthreads();
+  if (tid == 0 && s_new)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)s_new);
+}
+
+__global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical,
+                                              const uint64_t* __restrict__ slots, uint64_t bmask,
+                                              tk_r2t_vote* __restrict__ out, uint32_t slab) {
+  extern __shared__ uint32_t s_dyn[];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const uint64_t wpb = blockDim.x >> 6;
+  uint32_t* hits = s_dyn + (uint64_t)wv * slab;
+  const uint64_t occ = 1ull << (2 * k);
+  for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
+       r += (uint64_t)gridDim.x * wpb) {
+    const uint32_t L = rv.len[r];
+    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
+    const uint64_t wb = rv.word_off[r];
+    uint32_t n = 0;
+    for (uint32_t base = 0; base < nwin; base += 64) {
+      const uint32_t p = base + lane;
+      uint32_t c = 0xFFFFFFFFu;
+      if (p < nwin) {
+        uint64_t fwd;
+        bool valid;
+        window_at(rv, wb, p, k, fwd, valid);
+        if (valid) c = ctable_get(slots, bmask, occ, canon_key(fwd, k, canonical));
+      }
+      const bool has = c != 0xFFFFFFFFu;
+      const uint64_t m = __ballot(has);
+      if (has) hits[n + __popcll(m & ((1ull << lane) - 1))] = c;
+      n += (uint32_t)__popcll(m);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Walk the distinct components in ascending order; keep the first with
+    // the strictly largest number of hits (ties -> smallest id).
+    uint32_t best = 0xFFFFFFFFu, bestn = 0;
+    uint64_t prev = 0;  // candidates are > prev-1
+    bool first = true;
+    while (n) {
+      uint32_t cand = 0xFFFFFFFFu;
+      for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t v = hits[i];
+        if ((first || v > (uint32_t)prev) && v < cand) cand = v;
+      }
+      cand = wave_min_u32(cand);
+      if (cand == 0xFFFFFFFFu) break;
+      uint32_t cnt = 0;
+      for (uint32_t i = lane; i < n; i += 64) cnt += hits[i] == cand;
+      cnt = wave_sum_u32(cnt);
+      if (cnt > bestn) { bestn = cnt; best = cand; }
+      prev = cand;
+      first = false;
+    }
+    if (lane == 0) {
+      tk_r2t_vote v;
+      v.comp = best;
+      v.hits = bestn;
+      v.nwin = nwin;
+      out[r] = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+static int grid_for_waves(tk_ctx* ctx, uint64_t n_items, int waves_per_block) {
+  uint64_t want = ceil_div(n_items, (uint64_t)waves_per_block);
+  uint64_t cap = (uint64_t)std::max(ctx->num_cu, 1) * 32;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
+}
+
+// LDS slab per wave for the per-read kernels; returns waves per block.
+static int slab_layout(uint32_t max_nwin, uint32_t* slab, int* wpb) {
+  uint32_t s = std::max<uint32_t>(64, (max_nwin + 63) / 64 * 64);
+  if ((uint64_t)s * 4 > 65536) return -1;
+  int w = 4;
+  while (w > 1 && (uint64_t)s * 4 * w > 65536) w >>= 1;
+  *slab = s;
+  *wpb = w;
+  return 0;
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+// =====================================================================
+// C ABI
+// =====================================================================
+extern "C" {
+
+const char* tk_last_error(void) { return tk::g_err.c_str(); }
+int tk_abi_version(void) { return TK_ABI_VERSION; }
+
+int tk_init(int device, tk_ctx** out) {
+  if (!out) return fail(TK_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(TK_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return fail(TK_ERR_ARG, "device %d out of range [0,%d)", device, n);
+  TK_HIP(hipSetDevice(device));
+  tk_ctx* c = new tk_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
+  int rc = dev_alloc((void**)&c->d_flags, 64 * 4);
+  if (rc) { (void)hipStreamDestroy(c->stream); delete c; return rc; }
+  e = hipHostMalloc((void**)&c->h_flags, 64 * 4, hipHostMallocDefault);
+  if (e != hipSuccess) { dev_free(c->d_flags, 256); (void)hipStreamDestroy(c->stream); delete c; return hip_fail(e, "hipHostMalloc"); }
+  *out = c;
+  return TK_OK;
+}
+
+int tk_ctx_free(tk_ctx* c) {
+  if (!c) return TK_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  for (auto& s : c->scratch) dev_free(s.p, s.bytes);
+  dev_free(c->d_flags, 256);
+  (void)hipHostFree(c->h_flags);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return TK_OK;
+}
+
+int tk_sync(tk_ctx* c) {
+  if (!c) return fail(TK_ERR_ARG, "ctx is NULL");
+  TK_HIP(hipStreamSynchronize(c->stream));
+  return collect_timing(c);
+}
+
+int tk_timing_enable(tk_ctx* c, int on) {
+  if (!c) return fail(TK_ERR_ARG, "ctx is NULL");
+  c->timing = on != 0;
+  return TK_OK;
+}
+
+int tk_timing_reset(tk_ctx* c) {
+  if (!c) return fail(TK_ERR_ARG, "ctx is NULL");
+  int rc = collect_timing(c);
+  c->times.clear();
+  return rc;
+}
+
+int tk_timing_get(tk_ctx* c, const char* kernel, double* total_ms, uint64_t* launches) {
+  if (!c || !kernel) return fail(TK_ERR_ARG, "NULL argument");
+  int rc = collect_timing(c);
+  if (rc) return rc;
+  auto it = c->times.find(kernel);
+  if (total_ms) *total_ms = it == c->times.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == c->times.end() ? 0 : it->second.second;
+  return TK_OK;
+}
+
+int tk_device_bytes_in_use(uint64_t* bytes) {
+  if (!bytes) return fail(TK_ERR_ARG, "NULL argument");
+  *bytes = g_dev_bytes.load();
+  return TK_OK;
+}
+
+// ---------------- reads ----------------
+int tk_reads_from_host(tk_ctx* c, const char* seq, const uint64_t* offsets, uint64_t n,
+                       tk_reads** out) {
+  if (!c || !offsets || !out || (!seq && n && offsets[n] > offsets[0]))
+    return fail(TK_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (offsets[0] != 0) return fail(TK_ERR_ARG, "offsets[0] must be 0");
+  for (uint64_t i = 0; i < n; i++)
+    if (offsets[i + 1] < offsets[i]) return fail(TK_ERR_ARG, "offsets not monotone at %llu", (unsigned long long)i);
+  if (set_device(c->device)) return TK_ERR_HIP;
+  const uint64_t nb = offsets[n];
+  char* d_seq = nullptr;
+  uint64_t* d_off = nullptr;
+  int rc = dev_alloc((void**)&d_seq, nb + 64);
+  if (rc) return rc;
+  rc = dev_alloc((void**)&d_off, (n + 1) * 8);
+  if (rc) { dev_free(d_seq, nb + 64); return rc; }
+  hipError_t e = hipSuccess;
+  if (nb) e = hipMemcpyAsync(d_seq, seq, nb, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_off, offsets, (n + 1) * 8, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) rc = hip_fail(e, "upload reads");
+  if (!rc) rc = reads_build(c, d_seq, d_off, n, nb, out);
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(d_seq, nb + 64);
+  dev_free(d_off, (n + 1) * 8);
+  return rc;
+}
+
+int tk_reads_from_device_d(tk_ctx* c, const char* d_seq, const uint64_t* d_offsets, uint64_t n,
+                           tk_reads** out) {
+  if (!c || !d_offsets || !out) return fail(TK_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (set_device(c->device)) return TK_ERR_HIP;
+  uint64_t nb = 0;
+  TK_HIP(hipMemcpyAsync(&nb, d_offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  return reads_build(c, d_seq, d_offsets, n, nb, out);
+}
+
+int tk_reads_info(const tk_reads* r, uint64_t* n_reads, uint64_t* n_bases, uint32_t* max_len) {
+  if (!r) return fail(TK_ERR_ARG, "reads is NULL");
+  if (n_reads) *n_reads = r->n_reads;
+  if (n_bases) *n_bases = r->n_bases;
+  if (max_len) *max_len = r->max_len;
+  return TK_OK;
+}
+
+int tk_reads_windows(const tk_reads* r, int k, uint64_t* total) {
+  if (!r || !total) return fail(TK_ERR_ARG, "NULL argument");
+  if (k < 1 || k > 31) return fail(TK_ERR_ARG, "k=%d outside [1,31]", k);
+  uint64_t t = 0;
+  // lengths are on the device; a throwaway context is cheap compared to the pass
+  tk_ctx* c = nullptr;
+  int rc = tk_init(r->device, &c);
+  if (rc) return rc;
+  rc = reads_windows(c, r, k, &t);
+  tk_ctx_free(c);
+  *total = t;
+  return rc;
+}
+
+int tk_reads_free(tk_reads* r) {
+  if (!r) return TK_OK;
+  (void)hipSetDevice(r->device);
+  dev_free(r->d_len, std::max<uint64_t>(r->n_reads, 1) * 4);
+  dev_free(r->d_word_off, (r->n_reads + 1) * 8);
+  if (r->d_codes) dev_free(r->d_codes, (r->n_words + 1) * 8);
+  if (r->d_nmask) dev_free(r->d_nmask, (r->n_words + 1) * 4);
+  delete r;
+  return TK_OK;
+}
+
+// ---------------- table ----------------
+int tk_estimate_distinct(tk_ctx* c, const tk_reads* r, int k, int canonical, double* est) {
+  if (!c || !r || !est) return fail(TK_ERR_ARG, "NULL argument");
+  if (k < 1 || k > 31) return fail(TK_ERR_ARG, "k=%d outside [1,31]", k);
+  if (set_device(c->device)) return TK_ERR_HIP;
+  void* p = nullptr;
+  int rc = ensure_scratch(c, 0, kHllRegs * 4, &p);
+  if (rc) return rc;
+  uint32_t* regs = (uint32_t*)p;
+  TK_HIP(hipMemsetAsync(regs, 0, kHllRegs * 4, c->stream));
+  if (r->n_reads) {
+    int grid = grid_for_waves(c, r->n_reads, 4);
+    TK_LAUNCH(c, "hll", grid, 256, 0, k_hll, reads_view(r), k, canonical, regs);
+  }
+  std::vector<uint32_t> h(kHllRegs);
+  TK_HIP(hipMemcpyAsync(h.data(), regs, kHllRegs * 4, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  const double m = kHllRegs;
+  double s = 0;
+  int zeros = 0;
+  for (uint32_t v : h) {
+    s += std::ldexp(1.0, -(int)v);
+    zeros += v == 0;
+  }
+  const double alpha = 0.7213 / (1.0 + 1.079 / m);
+  double e = alpha * m * m / s;
+  if (e <= 2.5 * m && zeros) e = m * std::log(m / zeros);
+  *est = e;
+  return TK_OK;
+}
+
+int tk_table_create(tk_ctx* c, int k, int canonical, uint64_t expected, double max_load,
+                    tk_table** out) {
+  if (!c || !out) return fail(TK_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (k < 1 || k > 31) return fail(TK_ERR_ARG, "k=%d outside [1,31]", k);
+  if (!(max_load > 0.0 && max_load <= 0.9)) return fail(TK_ERR_ARG, "max_load must be in (0,0.9]");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  tk_table* t = new tk_table();
+  t->device = c->device;
+  t->k = k;
+  t->canonical = canonical ? 1 : 0;
+  t->max_load = max_load;
+  uint64_t slots = (uint64_t)std::ceil((double)std::max<uint64_t>(expected, 64) / max_load);
+  t->nbuckets = next_pow2(ceil_div(slots, 8));
+  int rc = table_alloc_slots(c, t->nbuckets, &t->d_slots);
+  if (!rc) rc = table_ensure_ovf(c, t, 0);
+  if (!rc) { hipError_t e = hipStreamSynchronize(c->stream); if (e != hipSuccess) rc = hip_fail(e, "table init"); }
+  if (rc) { tk_table_free(t); return rc; }
+  t->bytes = t->nbuckets * 64 + t->ovf_cap * 16;
+  *out = t;
+  return TK_OK;
+}
+
+int tk_table_info(const tk_table* t, int* k, int* canonical, uint64_t* capacity, uint64_t* bytes) {
+  if (!t) return fail(TK_ERR_ARG, "table is NULL");
+  if (k) *k = t->k;
+  if (canonical) *canonical = t->canonical;
+  if (capacity) *capacity = t->nbuckets * 8;
+  if (bytes) *bytes = t->bytes;
+  return TK_OK;
+}
+
+int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
+  if (!c || !t || !r) return fail(TK_ERR_ARG, "NULL argument");
+  if (t->device != c->device || r->device != c->device) return fail(TK_ERR_ARG, "device mismatch");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  if (!r->n_reads) return TK_OK;
+  int rc = table_set_final(c, t, false);
+  if (rc) return rc;
+  uint64_t total = 0;
+  if ((rc = reads_windows(c, r, t->k, &total))) return rc;
+  const uint32_t max_nwin = r->max_len >= (uint32_t)t->k ? r->max_len - t->k + 1 : 0;
+  const uint64_t cbits = 63 - 2 * t->k;
+  if ((rc = table_ensure_ovf(c, t, t->ovf_bound + (total >> cbits) + 1))) return rc;
+  t->ovf_bound += (total >> cbits) + 1;
+  uint64_t rb = 0;
+  while (rb < r->n_reads) {
+    // Read chunk whose windows cannot push the table past 0.9 load even if
+    // every window were a new key.
+    double room = 0.9 * (double)(t->nbuckets * 8) - (double)t->n_keys;
+    uint64_t nreads = max_nwin ? (uint64_t)(room / max_nwin) : r->n_reads;
+    if (nreads < 64 && rb + nreads < r->n_reads) {
+      if ((rc = table_grow(c, t, std::max<uint64_t>(2 * t->n_keys, t->nbuckets * 16)))) return rc;
+      continue;
+    }
+    nreads = std::min<uint64_t>(nreads, r->n_reads - rb);
+    nreads = std::min<uint64_t>(nreads, (uint64_t)0x7FFFFFFF * kReadsPerBlock);
+    TK_HIP(hipMemsetAsync(c->d_flags, 0, 16, c->stream));
+    uint64_t grid = ceil_div(nreads, kReadsPerBlock);
+    TableView v = table_view(t);
+    TK_LAUNCH(c, "count", grid, 256, 0, k_count, reads_view(r), rb, rb + nreads, t->k,
+              t->canonical, v, c->d_flags);
+    TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 16, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipStreamSynchronize(c->stream));
+    if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table probe limit reached (flags=%u)", c->h_flags[0]);
+    uint64_t nnew = 0;
+    memcpy(&nnew, &c->h_flags[2], 8);
+    t->n_keys += nnew;
+    rb += nreads;
+    if ((double)t->n_keys > t->max_load * (double)(t->nbuckets * 8)) {
+      if ((rc = table_grow(c, t, t->n_keys))) return rc;
+    }
+  }
+  return TK_OK;
+}
+
+int tk_table_load(tk_ctx* c, tk_table* t, const uint64_t* keys, const uint32_t* counts, uint64_t n) {
+  if (!c || !t || (n && (!keys || !counts))) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  if (!n) return TK_OK;
+  int rc = table_set_final(c, t, false);
+  if (rc) return rc;
+  const uint64_t cbits = 63 - 2 * t->k;
+  uint64_t bound = 0;
+  unsigned __int128 tot = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    tot += counts[i];
+    bound += (counts[i] >> cbits) != 0;
+  }
+  bound += (uint64_t)(tot >> cbits) + 1;
+  if ((rc = table_ensure_ovf(c, t, t->ovf_bound + bound))) return rc;
+  t->ovf_bound += bound;
+  const uint64_t chunk_max = 1ull << 26;
+  uint64_t* dk = nullptr;
+  uint32_t* dc = nullptr;
+  uint64_t cm = std::min<uint64_t>(n, chunk_max);
+  if ((rc = dev_alloc((void**)&dk, cm * 8))) return rc;
+  if ((rc = dev_alloc((void**)&dc, cm * 4))) { dev_free(dk, cm * 8); return rc; }
+  uint64_t i = 0;
+  while (i < n && !rc) {
+    double room = 0.9 * (double)(t->nbuckets * 8) - (double)t->n_keys;
+    uint64_t m = std::min<uint64_t>(std::min<uint64_t>(cm, n - i), room > 0 ? (uint64_t)room : 0);
+    if (m < std::min<uint64_t>(cm, n - i) / 2 || m == 0) {
+      rc = table_grow(c, t, std::max<uint64_t>(t->n_keys + std::min<uint64_t>(cm, n - i), 2 * t->n_keys));
+      continue;
+    }
+    hipError_t e = hipMemcpyAsync(dk, keys + i, m * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dc, counts + i, m * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_flags, 0, 16, c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "load upload"); break; }
+    TableView v = table_view(t);
+    {
+      hipEvent_t a;
+      timing_begin(c, &a);
+      hipLaunchKernelGGL(k_load, dim3(std::min<uint64_t>(ceil_div(m, 256), 65536)), dim3(256), 0,
+                         c->stream, dk, dc, m, v, c->d_flags);
+      timing_end(c, "load", a);
+      if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_load"); break; }
+    }
+    e = hipMemcpyAsync(c->h_flags, c->d_flags, 16, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "load sync"); break; }
+    if (c->h_flags[0] & 8u) { rc = fail(TK_ERR_ARG, "load: key wider than 2k bits"); break; }
+    if (c->h_flags[0]) { rc = fail(TK_ERR_FULL, "load: table probe limit reached"); break; }
+    uint64_t nnew = 0;
+    memcpy(&nnew, &c->h_flags[2], 8);
+    t->n_keys += nnew;
+    i += m;
+    if ((double)t->n_keys > t->max_load * (double)(t->nbuckets * 8)) rc = table_grow(c, t, t->n_keys);
+  }
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(dk, cm * 8);
+  dev_free(dc, cm * 4);
+  return rc;
+}
+
+int tk_table_dump_size(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_t* n) {
+  if (!c || !t || !n) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  int rc = table_set_final(c, t, true);
+  if (rc) return rc;
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
+  TK_HIP(hipMemsetAsync(d, 0, 8, c->stream));
+  TableView v = table_view(t);
+  uint64_t ns = t->nbuckets * 8;
+  TK_LAUNCH(c, "dump_count", std::min<uint64_t>(ceil_div(ns, 256), 65536), 256, 0, k_dump_count,
+            v, ns, (uint64_t)lo, (uint64_t)hi, d);
+  uint64_t h = 0;
+  TK_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  *n = h;
+  return TK_OK;
+}
+
+int tk_table_dump(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_t* keys,
+                  uint32_t* counts, uint64_t cap, uint64_t* n_out) {
+  if (!c || !t || !n_out || (cap && (!keys || !counts))) return fail(TK_ERR_ARG, "NULL argument");
+  uint64_t n = 0;
+  int rc = tk_table_dump_size(c, t, lo, hi, &n);
+  if (rc) return rc;
+  *n_out = n;
+  if (n > cap) return fail(TK_ERR_CAP, "dump needs %llu entries, buffer holds %llu",
+                           (unsigned long long)n, (unsigned long long)cap);
+  if (!n) return TK_OK;
+  uint64_t* dk = nullptr;
+  uint32_t* dc = nullptr;
+  if ((rc = dev_alloc((void**)&dk, n * 8))) return rc;
+  if ((rc = dev_alloc((void**)&dc, n * 4))) { dev_free(dk, n * 8); return rc; }
+  do {
+    unsigned long long* cur = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
+    hipError_t e = hipMemsetAsync(cur, 0, 8, c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "memset"); break; }
+    TableView v = table_view(t);
+    uint64_t ns = t->nbuckets * 8;
+    {
+      hipEvent_t a;
+      timing_begin(c, &a);
+      hipLaunchKernelGGL(k_dump, dim3(std::min<uint64_t>(ceil_div(ns, 256), 65536)), dim3(256), 0,
+                         c->stream, v, ns, (uint64_t)lo, (uint64_t)hi, dk, dc, n, cur);
+      timing_end(c, "dump", a);
+      if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_dump"); break; }
+    }
+    if ((rc = sort_pairs_u64_u32(c, dk, dc, n, 2 * t->k))) break;
+    e = hipMemcpyAsync(keys, dk, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, dc, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "dump download");
+  } while (0);
+  dev_free(dk, n * 8);
+  dev_free(dc, n * 4);
+  return rc;
+}
+
+int tk_table_histo(tk_ctx* c, tk_table* t, uint64_t low, uint64_t high, uint64_t inc,
+                   uint64_t* bins, uint64_t nbins) {
+  if (!c || !t || !bins) return fail(TK_ERR_ARG, "NULL argument");
+  if (inc == 0 || high < low) return fail(TK_ERR_ARG, "bad histo range");
+  const uint64_t base = low > 1 ? (inc >= low ? 1 : low - inc) : 1;
+  const uint64_t ceil_ = high + inc;
+  const uint64_t nb = (ceil_ + inc - base) / inc;
+  if (nb != nbins) return fail(TK_ERR_ARG, "nbins must be %llu", (unsigned long long)nb);
+  if (set_device(c->device)) return TK_ERR_HIP;
+  int rc = table_set_final(c, t, true);
+  if (rc) return rc;
+  void* p = nullptr;
+  if ((rc = ensure_scratch(c, 1, nb * 8, &p))) return rc;
+  TK_HIP(hipMemsetAsync(p, 0, nb * 8, c->stream));
+  TableView v = table_view(t);
+  uint64_t ns = t->nbuckets * 8;
+  TK_LAUNCH(c, "histo", std::min<uint64_t>(ceil_div(ns, 256), 4096), 256, 0, k_histo, v, ns, base,
+            ceil_, inc, nb, (unsigned long long*)p);
+  TK_HIP(hipMemcpyAsync(bins, p, nb * 8, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  return TK_OK;
+}
+
+int tk_table_prune(tk_ctx* c, tk_table* t, uint32_t min_count, const uint8_t* keep, uint64_t* n_kept) {
+  if (!c || !t) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  int rc = table_set_final(c, t, true);
+  if (rc) return rc;
+  const uint64_t kt = (uint64_t)(t->k + 1) * (t->k + 1) * (t->k + 1);
+  uint8_t* dkeep = nullptr;
+  if (keep) {
+    void* p = nullptr;
+    if ((rc = ensure_scratch(c, 1, kt, &p))) return rc;
+    dkeep = (uint8_t*)p;
+    TK_HIP(hipMemcpyAsync(dkeep, keep, kt, hipMemcpyHostToDevice, c->stream));
+  }
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
+  TK_HIP(hipMemsetAsync(d, 0, 8, c->stream));
+  TableView v = table_view(t);
+  uint64_t ns = t->nbuckets * 8;
+  TK_LAUNCH(c, "prune", std::min<uint64_t>(ceil_div(ns, 256), 65536), 256, 0, k_prune, v, ns, t->k,
+            (uint64_t)min_count, dkeep, d);
+  uint64_t h = 0;
+  TK_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  if (n_kept) *n_kept = h;
+  return TK_OK;
+}
+
+int tk_table_lookup(tk_ctx* c, tk_table* t, const uint64_t* keys, uint64_t n, uint32_t* counts) {
+  if (!c || !t || (n && (!keys || !counts))) return fail(TK_ERR_ARG, "NULL argument");
+  if (!n) return TK_OK;
+  if (set_device(c->device)) return TK_ERR_HIP;
+  int rc = table_set_final(c, t, true);
+  if (rc) return rc;
+  uint64_t* dk = nullptr;
+  uint32_t* dc = nullptr;
+  if ((rc = dev_alloc((void**)&dk, n * 8))) return rc;
+  if ((rc = dev_alloc((void**)&dc, n * 4))) { dev_free(dk, n * 8); return rc; }
+  hipError_t e = hipMemcpyAsync(dk, keys, n * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipEvent_t a;
+    timing_begin(c, &a);
+    hipLaunchKernelGGL(k_lookup, dim3(std::min<uint64_t>(ceil_div(n, 256), 65536)), dim3(256), 0,
+                       c->stream, table_view(t), dk, n, dc);
+    timing_end(c, "lookup", a);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(counts, dc, n * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dev_free(dk, n * 8);
+  dev_free(dc, n * 4);
+  if (e != hipSuccess) return hip_fail(e, "lookup");
+  return TK_OK;
+}
+
+int tk_table_free(tk_table* t) {
+  if (!t) return TK_OK;
+  (void)hipSetDevice(t->device);
+  if (t->d_slots) dev_free(t->d_slots, t->nbuckets * 64);
+  if (t->d_ovf) dev_free(t->d_ovf, t->ovf_cap * 16);
+  delete t;
+  return TK_OK;
+}
+
+// ---------------- stats ----------------
+int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stats_opts* o,
+                        tk_read_stat* d_out) {
+  if (!c || !t || !r || !o || (r->n_reads && !d_out)) return fail(TK_ERR_ARG, "NULL argument");
+  if (o->nwin_policy != TK_NWIN_AS_MISSING && o->nwin_policy != TK_NWIN_SKIP)
+    return fail(TK_ERR_ARG, "bad nwin_policy %d", o->nwin_policy);
+  if (o->median_policy < 0 || o->median_policy > 2)
+    return fail(TK_ERR_ARG, "bad median_policy %d", o->median_policy);
+  if (set_device(c->device)) return TK_ERR_HIP;
+  int rc = table_set_final(c, t, true);
+  if (rc) return rc;
+  if (!r->n_reads) return TK_OK;
+  const uint32_t max_nwin = r->max_len >= (uint32_t)t->k ? r->max_len - t->k + 1 : 0;
+  uint32_t slab = 0;
+  int wpb = 0;
+  if (slab_layout(max_nwin, &slab, &wpb))
+    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16384 + t->k - 1);
+  StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
+  int grid = grid_for_waves(c, r->n_reads, wpb);
+  TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb, k_stats, reads_view(r), t->k,
+            t->canonical, table_view(t), sp, d_out, slab);
+  return TK_OK;
+}
+
+int tk_read_cov_stats(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stats_opts* o,
+                      tk_read_stat* out) {
+  if (!c || !r || (r->n_reads && !out)) return fail(TK_ERR_ARG, "NULL argument");
+  if (!r->n_reads) return tk_read_cov_stats_d(c, t, r, o, nullptr);
+  tk_read_stat* d = nullptr;
+  size_t bytes = r->n_reads * sizeof(tk_read_stat);
+  int rc = dev_alloc((void**)&d, bytes);
+  if (rc) return rc;
+  rc = tk_read_cov_stats_d(c, t, r, o, d);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "stats download");
+  }
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(d, bytes);
+  return rc;
+}
+
+// ---------------- selection ----------------
+int tk_select_pairs_d(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint64_t n,
+                      int max_cov, int min_cov, int max_cv, uint32_t max_nwin, uint8_t* dsel,
+                      uint64_t* n_selected) {
+  if (!c || (n && (!dl || !dr || !dsel))) return fail(TK_ERR_ARG, "NULL argument");
+  if (max_cov <= 0) return fail(TK_ERR_ARG, "max_cov must be positive");
+  if ((double)max_cv < 1.2 * std::sqrt((double)max_nwin) + 1.0)
+    return fail(TK_ERR_ARG, "max_cv=%d below the bound 1.2*sqrt(%u)+1 where the CV test is exact",
+                max_cv, max_nwin);
+  if (n > 0xFFFFFFFFull) return fail(TK_ERR_ARG, "too many pairs");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  if (n_selected) *n_selected = 0;
+  if (!n) return TK_OK;
+  void* p = nullptr;
+  int rc = ensure_scratch(c, 1, n * 8 + 256, &p);
+  if (rc) return rc;
+  uint32_t* consumes = (uint32_t*)p;
+  uint32_t* idx = consumes + ((n + 63) / 64) * 64;
+  int grid = (int)std::min<uint64_t>(ceil_div(n, 256), 65536);
+  TK_LAUNCH(c, "select_flags", grid, 256, 0, k_select_flags, dl, dr, n, min_cov, consumes);
+  if ((rc = exclusive_scan_u32(c, consumes, idx, n))) return rc;
+  Lcg48Jump j;
+  const uint64_t M = (1ull << 48) - 1;
+  uint64_t A = 0x5DEECE66Dull, C = 0xB;
+  for (int b = 0; b < 48; b++) {
+    j.a[b] = A;
+    j.c[b] = C;
+    C = (A * C + C) & M;  // (A,C) o (A,C)
+    A = (A * A) & M;
+  }
+  const uint64_t state0 = ((uint64_t)12345 << 16) | 0x330Eull;
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
+  TK_HIP(hipMemsetAsync(d, 0, 8, c->stream));
+  TK_LAUNCH(c, "select", grid, 256, 0, k_select, dl, dr, n, consumes, idx, j, state0, max_cov,
+            dsel, d);
+  uint64_t h = 0;
+  TK_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  if (n_selected) *n_selected = h;
+  return TK_OK;
+}
+
+int tk_select_pairs(tk_ctx* c, const tk_read_stat* l, const tk_read_stat* r, uint64_t n, int max_cov,
+                    int min_cov, int max_cv, uint8_t* sel, uint64_t* n_selected) {
+  if (!c || (n && (!l || !r || !sel))) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  uint32_t max_nwin = 0;
+  for (uint64_t i = 0; i < n; i++) max_nwin = std::max(max_nwin, std::max(l[i].nwin, r[i].nwin));
+  size_t sb = n * sizeof(tk_read_stat);
+  tk_read_stat *dl = nullptr, *dr = nullptr;
+  uint8_t* ds = nullptr;
+  int rc = dev_alloc((void**)&dl, sb);
+  if (!rc) rc = dev_alloc((void**)&dr, sb);
+  if (!rc) rc = dev_alloc((void**)&ds, n + 1);
+  if (!rc && n) {
+    hipError_t e = hipMemcpyAsync(dl, l, sb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, r, sb, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "select upload");
+  }
+  if (!rc) rc = tk_select_pairs_d(c, dl, dr, n, max_cov, min_cov, max_cv, max_nwin, ds, n_selected);
+  if (!rc && n) {
+    hipError_t e = hipMemcpyAsync(sel, ds, n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "select download");
+  }
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(dl, sb);
+  dev_free(dr, sb);
+  dev_free(ds, n + 1);
+  return rc;
+}
+
+// ---------------- ReadsToTranscripts ----------------
+int tk_ctable_build(tk_ctx* c, int k, int canonical, const tk_reads* b, const uint32_t* comp_ids,
+                    tk_ctable** out) {
+  if (!c || !b || !out || (b->n_reads && !comp_ids)) return fail(TK_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (k < 1 || k > 31) return fail(TK_ERR_ARG, "k=%d outside [1,31]", k);
+  for (uint64_t i = 0; i < b->n_reads; i++)
+    if (comp_ids[i] == 0xFFFFFFFFu) return fail(TK_ERR_ARG, "component id 0xFFFFFFFF is reserved");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  uint64_t total = 0;
+  int rc = reads_windows(c, b, k, &total);
+  if (rc) return rc;
+  tk_ctable* t = new tk_ctable();
+  t->device = c->device;
+  t->k = k;
+  t->canonical = canonical ? 1 : 0;
+  t->nbuckets = next_pow2(std::max<uint64_t>(ceil_div(2 * total, 4), 16));
+  uint32_t* dcomp = nullptr;
+  size_t cb = std::max<uint64_t>(b->n_reads, 1) * 4;
+  do {
+    if ((rc = dev_alloc((void**)&t->d_slots, t->nbuckets * 64))) break;
+    if ((rc = dev_alloc((void**)&dcomp, cb))) break;
+    hipError_t e = hipMemsetAsync(t->d_slots, 0, t->nbuckets * 64, c->stream);
+    if (e == hipSuccess && b->n_reads)
+      e = hipMemcpyAsync(dcomp, comp_ids, b->n_reads * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_flags, 0, 16, c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "ctable init"); break; }
+    if (b->n_reads) {
+      hipEvent_t a;
+      timing_begin(c, &a);
+      hipLaunchKernelGGL(k_ctable_build, dim3(ceil_div(b->n_reads, kReadsPerBlock)), dim3(256), 0,
+                         c->stream, reads_view(b), dcomp, k, t->canonical, t->d_slots,
+                         t->nbuckets - 1, c->d_flags);
+      timing_end(c, "ctable_build", a);
+      if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_ctable_build"); break; }
+    }
+    e = hipMemcpyAsync(c->h_flags, c->d_flags, 16, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "ctable sync"); break; }
+    if (c->h_flags[0]) { rc = fail(TK_ERR_FULL, "ctable probe limit reached"); break; }
+    memcpy(&t->n_keys, &c->h_flags[2], 8);
+  } while (0);
+  dev_free(dcomp, cb);
+  t->bytes = t->nbuckets * 64;
+  if (rc) { tk_ctable_free(t); return rc; }
+  *out = t;
+  return TK_OK;
+}
+
+int tk_ctable_info(const tk_ctable* t, uint64_t* n_keys, uint64_t* capacity) {
+  if (!t) return fail(TK_ERR_ARG, "ctable is NULL");
+  if (n_keys) *n_keys = t->n_keys;
+  if (capacity) *capacity = t->nbuckets * 4;
+  return TK_OK;
+}
+
+int tk_r2t_vote_reads_d(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t_vote* d_out) {
+  if (!c || !t || !r || (r->n_reads && !d_out)) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  if (!r->n_reads) return TK_OK;
+  const uint32_t max_nwin = r->max_len >= (uint32_t)t->k ? r->max_len - t->k + 1 : 0;
+  uint32_t slab = 0;
+  int wpb = 0;
+  if (slab_layout(max_nwin, &slab, &wpb))
+    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16384 + t->k - 1);
+  int grid = grid_for_waves(c, r->n_reads, wpb);
+  TK_LAUNCH(c, "vote", grid, wpb * 64, (size_t)slab * 4 * wpb, k_vote, reads_view(r), t->k,
+            t->canonical, t->d_slots, t->nbuckets - 1, d_out, slab);
+  return TK_OK;
+}
+
+int tk_r2t_vote_reads(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t_vote* out) {
+  if (!c || !r || (r->n_reads && !out)) return fail(TK_ERR_ARG, "NULL argument");
+  if (!r->n_reads) return TK_OK;
+  tk_r2t_vote* d = nullptr;
+  size_t bytes = r->n_reads * sizeof(tk_r2t_vote);
+  int rc = dev_alloc((void**)&d, bytes);
+  if (rc) return rc;
+  rc = tk_r2t_vote_reads_d(c, t, r, d);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "vote download");
+  }
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(d, bytes);
+  return rc;
+}
+
+int tk_ctable_free(tk_ctable* t) {
+  if (!t) return TK_OK;
+  (void)hipSetDevice(t->device);
+  if (t->d_slots) dev_free(t->d_slots, t->nbuckets * 64);
+  delete t;
+  return TK_OK;
+}
+
+}  // extern "C"
