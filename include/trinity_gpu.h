@@ -79,6 +79,12 @@ int tk_timing_get(tk_ctx* ctx, const char* kernel, double* total_ms, uint64_t* l
 /* Device memory currently held by objects of this process (bytes). */
 int tk_device_bytes_in_use(uint64_t* bytes);
 
+/* Plain device buffers (inputs kept resident in HBM, outputs of _d calls). */
+int tk_buffer_alloc(tk_ctx* ctx, uint64_t bytes, void** d_ptr);
+int tk_buffer_free(tk_ctx* ctx, void* d_ptr);
+int tk_memcpy_h2d(tk_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
+int tk_memcpy_d2h(tk_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+
 /* ---------------- reads (device-resident, 2-bit packed) ---------------- */
 /*
  * seq: concatenated read sequences (ASCII, no separators); offsets[n+1]:

@@ -1,0 +1,234 @@
+"""The drop-in CLI tools keep the reference command lines and formats.
+
+Host-only paths (jellyfish dump/histo over a database, nbkc_normalize) run on
+the CPU; paths that count or score run the HIP library (marked gpu).
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import brute
+from data import concat, edge_reads, random_reads, repetitive_reads
+from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parents[1]
+BIN = ROOT / "trinityrnaseq_amd" / "bin"
+
+
+def run(args, cwd=None, check=True, **kw):
+    r = subprocess.run([str(a) for a in args], cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, **kw)
+    if check and r.returncode != 0:
+        raise AssertionError(f"{args} -> {r.returncode}\n{r.stderr}")
+    return r
+
+
+def write_db(path, k, canonical, keys, counts):
+    with open(path, "wb") as f:
+        f.write(b"TKJF1\n")
+        f.write(struct.pack("<IIQ", k, canonical, len(keys)))
+        f.write(np.ascontiguousarray(keys, np.uint64).tobytes())
+        f.write(np.ascontiguousarray(counts, np.uint32).tobytes())
+
+
+def write_fasta(path, names, seqs):
+    with open(path, "w") as f:
+        for n, s in zip(names, seqs):
+            f.write(f">{n}\n{s}\n")
+
+
+def dump_text(keys, counts, k, lo=1, hi=0xFFFFFFFF):
+    out = []
+    for a, c in zip(keys.tolist(), counts.tolist()):
+        if lo <= c <= hi:
+            out.append(f">{c}\n{brute.kmer_of(a, k)}\n")
+    return "".join(out)
+
+
+@pytest.fixture(scope="module")
+def sample(tmp_path_factory):
+    d = tmp_path_factory.mktemp("tools")
+    seqs = repetitive_reads(31, 3000, genome_len=2000, read_len=76) + random_reads(32, 300) + edge_reads()
+    names = [f"read{i:06d}/1" for i in range(len(seqs))]
+    fa = d / "reads.fa"
+    write_fasta(fa, names, seqs)
+    seq, off = concat(seqs)
+    keys, counts = O.count(seq, off, 25, True)
+    return d, fa, names, seqs, seq, off, keys, counts
+
+
+def test_version_string():
+    r = run([BIN / "jellyfish", "--version"])
+    name, version = r.stdout.split()[:2]
+    assert version.startswith("2.")  # Trinity:4070-4074
+
+
+def test_dump_and_histo_from_database(sample):
+    d, fa, names, seqs, seq, off, keys, counts = sample
+    db = d / "oracle.jf"
+    write_db(db, 25, 1, keys, counts)
+    assert run([BIN / "jellyfish", "dump", db]).stdout == dump_text(keys, counts, 25)
+    assert run([BIN / "jellyfish", "dump", "-L", "2", db]).stdout == dump_text(keys, counts, 25, 2)
+    col = run([BIN / "jellyfish", "dump", "-c", "-L", "3", "-U", "9", db]).stdout
+    want = "".join(f"{brute.kmer_of(a, 25)} {c}\n" for a, c in zip(keys.tolist(), counts.tolist()) if 3 <= c <= 9)
+    assert col == want
+    run([BIN / "jellyfish", "histo", "-t", "4", "-o", d / "h.histo", db])
+    bins = O.histo(counts)
+    base = O.histo_base(1, 1)
+    want = "".join(f"{base + i} {int(b)}\n" for i, b in enumerate(bins) if b)
+    assert (d / "h.histo").read_text() == want
+    h = run([BIN / "jellyfish", "histo", "-l", "2", "-h", "20", "-i", "3", "-f", db]).stdout
+    bins = O.histo(counts, 2, 20, 3)
+    base = O.histo_base(2, 3)
+    assert h == "".join(f"{base + 3 * i} {int(b)}\n" for i, b in enumerate(bins))
+
+
+def test_tools_reject_unknown_options():
+    r = run([BIN / "jellyfish", "count", "-m", "25", "--bogus", "x.fa"], check=False)
+    assert r.returncode != 0 and "unknown option" in r.stderr
+    r = run([BIN / "ReadsToTranscripts", "-i", "a", "-f", "b", "-o", "c", "-zzz"], check=False)
+    assert r.returncode != 0
+
+
+def _stats_rows(rng, n):
+    rows = []
+    for i in range(n):
+        med = int(rng.integers(0, 400))
+        nw = 126
+        vals = rng.integers(max(med // 2, 1), med * 2 + 2, nw)
+        st = np.zeros(1, O.READ_STAT_DTYPE)[0]
+        st["median"], st["nwin"], st["sum"] = med, nw, int(vals.sum())
+        sq = int((vals.astype(object) ** 2).sum())
+        st["sumsq_lo"], st["sumsq_hi"] = sq & ((1 << 64) - 1), sq >> 64
+        if i % 97 == 0:
+            st["median"], st["nwin"], st["sum"], st["sumsq_lo"], st["sumsq_hi"] = 0, 0, 0, 0, 0
+        rows.append(st)
+    return rows
+
+
+def test_nbkc_normalize_matches_perl_semantics(tmp_path):
+    rng = np.random.default_rng(8)
+    n = 4000
+    left = _stats_rows(rng, n)
+    right = _stats_rows(rng, n)
+    # sorted accessions, with a few unpaired reads on either side
+    lnames = [f"p{i:07d}/1" for i in range(n)]
+    rnames = [f"p{i:07d}/2" for i in range(n)]
+    drop_l = set(range(5, n, 611))
+    drop_r = set(range(7, n, 509))
+    hdr = "acc\tmedian_cov\tmean_cov\tstdev\tpct_stdev_of_avg\n"
+    with open(tmp_path / "L", "w") as f:
+        f.write(hdr)
+        f.writelines(O.format_stats_row(lnames[i], left[i]) for i in range(n) if i not in drop_l)
+    with open(tmp_path / "R", "w") as f:
+        f.write(hdr)
+        f.writelines(O.format_stats_row(rnames[i], right[i]) for i in range(n) if i not in drop_r)
+    for max_cov in (30, 50, 200):
+        r = run([BIN / "nbkc_normalize", "--left", tmp_path / "L", "--right", tmp_path / "R", "--sorted",
+                 "--max_cov", max_cov, "--min_cov", 1, "--max_CV", 10000, "--merged_out", tmp_path / "M"])
+        rows, cores = [], []
+        for i in range(n):
+            if i in drop_l or i in drop_r:
+                continue
+            lf = O.format_stats_row("x", left[i]).split("\t")
+            rf = O.format_stats_row("x", right[i]).split("\t")
+            rows.append((lf[1], lf[2], lf[3], rf[1], rf[2], rf[3]))
+            cores.append(f"p{i:07d}")
+        want = [cores[j] for j in brute.perl_select(rows, max_cov, 1, 10000)]
+        assert r.stdout.split() == want
+        # the merged file alone gives the same selection (nbkc_normalize.pl --stats_file)
+        r2 = run([BIN / "nbkc_normalize", "--stats_file", tmp_path / "M", "--max_cov", max_cov,
+                  "--min_cov", 1, "--max_CV", 10000])
+        assert r2.stdout == r.stdout
+    merged = (tmp_path / "M").read_text().splitlines()
+    assert merged[0].split("\t")[-3:] == ["median_cov", "mean_cov", "stdev"]
+    assert len(merged) == 1 + len(cores)
+
+
+# ---------------------------------------------------------------- GPU paths
+
+@pytest.mark.gpu
+def test_jellyfish_count_dump_histo(sample):
+    d, fa, names, seqs, seq, off, keys, counts = sample
+    run([BIN / "jellyfish", "count", "-t", "4", "-m", "25", "-s", "100000000", "-o", d / "m.jf", "--canonical", fa])
+    assert run([BIN / "jellyfish", "dump", "-L", "1", d / "m.jf"]).stdout == dump_text(keys, counts, 25)
+    run([BIN / "jellyfish", "count", "-t", "4", "-m", "25", "-s", "1000", fa], cwd=d)  # default mer_counts.jf
+    assert (d / "mer_counts.jf").exists()
+    fk, fc = O.count(seq, off, 25, False)
+    assert run([BIN / "jellyfish", "dump", d / "mer_counts.jf"]).stdout == dump_text(fk, fc, 25)
+
+
+@pytest.mark.gpu
+def test_fastaToKmerCoverageStats_matches_oracle(sample):
+    d, fa, names, seqs, seq, off, keys, counts = sample
+    dump = d / "kmers.min2.fa"
+    dump.write_text(dump_text(keys, counts, 25, 2))
+    out = run([BIN / "fastaToKmerCoverageStats", "--reads", fa, "--kmers", dump, "--kmer_size", "25",
+               "--num_threads", "4", "--DS"]).stdout.splitlines(True)
+    assert out[0] == "acc\tmedian_cov\tmean_cov\tstdev\tpct_stdev_of_avg\n"
+    m = counts >= 2
+    st = O.read_stats(seq, off, keys[m], counts[m], 25, True, 1, 1, 0, 0)
+    want = [O.format_stats_row(n, s) for n, s in zip(names, st)]
+    assert out[1:] == want
+    # --kmers_from_reads counts the dictionary itself
+    out2 = run([BIN / "fastaToKmerCoverageStats", "--kmers_from_reads", fa, "--reads", fa, "--DS",
+                "--num_threads", "1"]).stdout.splitlines(True)
+    st2 = O.read_stats(seq, off, keys, counts, 25, True, 1, 1, 0, 0)
+    assert out2[1:] == [O.format_stats_row(n, s) for n, s in zip(names, st2)]
+
+
+@pytest.mark.gpu
+def test_inchworm_load_prune(sample):
+    d, fa, names, seqs, seq, off, keys, counts = sample
+    dump = d / "jellyfish.kmers.fa"
+    dump.write_text(dump_text(keys, counts, 25))
+    r = run([BIN / "inchworm", "--kmers", dump, "--run_inchworm", "-K", "25", "--monitor", "1", "--DS",
+             "--num_threads", "6", "--PARALLEL_IWORM", "--min_any_entropy", "1.5", "-L", "25",
+             "--no_prune_error_kmers", "--minKmerCount", "2", "--dump_pruned", d / "pruned.fa"],
+            cwd=d, check=False)
+    assert r.returncode == 2 and "contig extension" in r.stderr
+    pk, pc = O.prune(keys, counts, 25, 2, 1.5)
+    assert (d / "inchworm.kmer_count").read_text().strip() == str(len(pk))
+    assert (d / "pruned.fa").read_text() == dump_text(pk, pc, 25)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("strand", [False, True])
+def test_reads_to_transcripts(tmp_path, strand):
+    from trinityrnaseq_amd import synth
+    tx = synth.large_transcriptome(91, 300)
+    bundles, comps = [], []
+    i, cid = 0, 0
+    rng = np.random.default_rng(2)
+    while i < tx.n:
+        m = int(rng.integers(1, 4))
+        bundles.append("X".join(tx.seq(j) for j in range(i, min(i + m, tx.n))))
+        comps.append(cid * 3 + 1)
+        cid += 1
+        i += m
+    write_fasta(tmp_path / "bundles.fa", [f"s_{c}" for c in comps], bundles)
+    s, o = synth.se_reads(tx, 20000, 100, seed=9, sub_rate=0.02)
+    reads = [s[int(o[j]):int(o[j + 1])].tobytes().decode() for j in range(len(o) - 1)]
+    reads = [r.lower() if j % 50 == 0 else r for j, r in enumerate(reads)] + edge_reads()
+    names = [f"q{j}/1" for j in range(len(reads))]
+    write_fasta(tmp_path / "reads.fa", names, reads)
+    args = [BIN / "ReadsToTranscripts", "-i", tmp_path / "reads.fa", "-f", tmp_path / "bundles.fa",
+            "-o", tmp_path / "r2c.out", "-t", "4", "-max_mem_reads", "5000", "-p", "10"]
+    if strand:
+        args.append("-strand")
+    run(args)
+    bseq, boff = concat(bundles)
+    keys, cv = O.r2t_build(bseq, boff, comps, 25, not strand)
+    rseq, roff = concat(reads)
+    votes = O.r2t_vote(rseq, roff, keys, cv, 25, not strand)
+    want = []
+    for nm, r, v in zip(names, reads, votes):
+        if v["hits"] and 100 * int(v["hits"]) >= 10 * int(v["nwin"]):
+            want.append(f"{int(v['comp'])}\t>{nm}\t{100.0 * int(v['hits']) / int(v['nwin']):g}\t{r.upper()}\n")
+    assert (tmp_path / "r2c.out").read_text() == "".join(want)

@@ -37,6 +37,10 @@ SIGNATURES = {
     "tk_timing_reset": (C.c_int, [_P]),
     "tk_timing_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), _U64P]),
     "tk_device_bytes_in_use": (C.c_int, [_U64P]),
+    "tk_buffer_alloc": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "tk_buffer_free": (C.c_int, [_P, _P]),
+    "tk_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_uint64]),
+    "tk_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_uint64]),
     "tk_reads_from_host": (C.c_int, [_P, C.c_char_p, _P, C.c_uint64, C.POINTER(_P)]),
     "tk_reads_from_device_d": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(_P)]),
     "tk_reads_info": (C.c_int, [_P, _U64P, _U64P, _U32P]),
@@ -134,6 +138,39 @@ class Context:
         n = C.c_uint64()
         _check(lib().tk_timing_get(self.h, name.encode(), C.byref(ms), C.byref(n)), "tk_timing_get")
         return ms.value, n.value
+
+
+class DeviceBuffer:
+    """Device memory owned by the library (tk_buffer_alloc)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().tk_buffer_alloc(ctx.h, self.nbytes, C.byref(p)), "tk_buffer_alloc")
+        self.ptr = p.value or 0
+
+    def upload(self, a: np.ndarray, offset: int = 0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        _check(lib().tk_memcpy_h2d(self.ctx.h, C.c_void_p(self.ptr + offset), _ptr(a), a.nbytes),
+               "tk_memcpy_h2d")
+
+    def download(self, a: np.ndarray, offset: int = 0):
+        assert offset + a.nbytes <= self.nbytes and a.flags.c_contiguous
+        _check(lib().tk_memcpy_d2h(self.ctx.h, _ptr(a), C.c_void_p(self.ptr + offset), a.nbytes),
+               "tk_memcpy_d2h")
+
+    def free(self):
+        if self.ptr:
+            lib().tk_buffer_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def pack_sequences(seqs) -> tuple[bytes, np.ndarray]:
@@ -281,6 +318,23 @@ def read_cov_stats(ctx: Context, table: Table, reads: Reads, min_count: int = 1,
     _check(lib().tk_read_cov_stats(ctx.h, table.h, reads.h, C.byref(o), _ptr(out)),
            "tk_read_cov_stats")
     return out
+
+
+def read_cov_stats_d(ctx: Context, table: Table, reads: Reads, d_out: int, min_count: int = 1,
+                     missing_value: int = 1, nwin_policy: int = NWIN_AS_MISSING,
+                     median_policy: int = MEDIAN_UPPER) -> None:
+    o = StatsOpts(min_count, missing_value, nwin_policy, median_policy)
+    _check(lib().tk_read_cov_stats_d(ctx.h, table.h, reads.h, C.byref(o), C.c_void_p(d_out)),
+           "tk_read_cov_stats_d")
+
+
+def select_pairs_d(ctx: Context, d_left: int, d_right: int, n: int, max_cov: int, min_cov: int,
+                   max_cv: int, max_nwin: int, d_sel: int) -> int:
+    ns = C.c_uint64()
+    _check(lib().tk_select_pairs_d(ctx.h, C.c_void_p(d_left), C.c_void_p(d_right), n, max_cov,
+                                   min_cov, max_cv, max_nwin, C.c_void_p(d_sel), C.byref(ns)),
+           "tk_select_pairs_d")
+    return ns.value
 
 
 def select_pairs(ctx: Context, left: np.ndarray, right: np.ndarray, max_cov: int, min_cov: int = 1,

@@ -2,6 +2,7 @@
 
 Outputs (git-ignored, shipped with the source tree):
   trinityrnaseq_amd/lib/libtrinity_gpu.so   HIP kernels + C ABI (include/trinity_gpu.h)
+  trinityrnaseq_amd/lib/libtk_synth.so      seeded synthetic read generator (bench/tests)
   trinityrnaseq_amd/bin/*                   CLI tools keeping the reference command lines
 """
 from __future__ import annotations
@@ -74,6 +75,17 @@ def build_library(force: bool = True) -> Path:
     return out
 
 
+def build_synth(force: bool = True) -> Path:
+    LIB.mkdir(exist_ok=True)
+    src = CSRC / "synth" / "tk_synth.cpp"
+    out = LIB / "libtk_synth.so"
+    if force or not _newer(out, [src]):
+        tmp = LIB / "libtk_synth.so.tmp"
+        _run([CXX, "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread", str(src), "-o", str(tmp)])
+        os.replace(tmp, out)
+    return out
+
+
 def build_tools(force: bool = True) -> list[Path]:
     BIN.mkdir(exist_ok=True)
     common = [CSRC / c for c in TOOL_COMMON]
@@ -96,6 +108,7 @@ def build_tools(force: bool = True) -> list[Path]:
 
 def build_all(force: bool = True) -> None:
     build_library(force)
+    build_synth(force)
     build_tools(force)
 
 

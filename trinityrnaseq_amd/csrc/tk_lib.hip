@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "tk_internal.h"
 
@@ -590,7 +591,8 @@ __global__ void k_dump_count(TableView t, uint64_t n_slots, uint64_t lo, uint64_
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t s = t.slots[i];
     if (!s || (s >> t.cshift) == 0) continue;
-    const uint64_t v = slot_count(t, s);
+    uint64_t v = slot_count(t, s);
+    v = v > 0xFFFFFFFFull ? 0xFFFFFFFFull : v;  // counts print saturated at 2^32-1
     c += (v >= lo && v <= hi);
   }
   c = wave_sum_u64(c);
@@ -610,6 +612,7 @@ __global__ void k_dump(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
       const uint64_t s = t.slots[i];
       if (s && (s >> t.cshift) != 0) {
         v = slot_count(t, s);
+        v = v > 0xFFFFFFFFull ? 0xFFFFFFFFull : v;
         take = v >= lo && v <= hi;
         key = s & (t.occ - 1);
       }
@@ -1128,6 +1131,53 @@ int tk_device_bytes_in_use(uint64_t* bytes) {
   return TK_OK;
 }
 
+// ---------------- plain buffers ----------------
+static std::mutex g_buf_mu;
+static std::map<void*, uint64_t> g_bufs;
+
+int tk_buffer_alloc(tk_ctx* c, uint64_t bytes, void** d) {
+  if (!c || !d) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  int rc = dev_alloc(d, bytes);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(g_buf_mu);
+  g_bufs[*d] = bytes;
+  return TK_OK;
+}
+
+int tk_buffer_free(tk_ctx* c, void* d) {
+  if (!c) return fail(TK_ERR_ARG, "NULL argument");
+  if (!d) return TK_OK;
+  uint64_t bytes = 0;
+  {
+    std::lock_guard<std::mutex> g(g_buf_mu);
+    auto it = g_bufs.find(d);
+    if (it == g_bufs.end()) return fail(TK_ERR_ARG, "pointer not from tk_buffer_alloc");
+    bytes = it->second;
+    g_bufs.erase(it);
+  }
+  if (set_device(c->device)) return TK_ERR_HIP;
+  TK_HIP(hipStreamSynchronize(c->stream));
+  dev_free(d, bytes);
+  return TK_OK;
+}
+
+int tk_memcpy_h2d(tk_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  if (bytes) TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  return TK_OK;
+}
+
+int tk_memcpy_d2h(tk_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  if (bytes) TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  return TK_OK;
+}
+
 // ---------------- reads ----------------
 int tk_reads_from_host(tk_ctx* c, const char* seq, const uint64_t* offsets, uint64_t n,
                        tk_reads** out) {
@@ -1313,14 +1363,13 @@ int tk_table_load(tk_ctx* c, tk_table* t, const uint64_t* keys, const uint32_t* 
   if (!n) return TK_OK;
   int rc = table_set_final(c, t, false);
   if (rc) return rc;
+  // Keys that can need a side entry: at most one per loaded pair, and at
+  // most one per 2^cbits of added count.
   const uint64_t cbits = 63 - 2 * t->k;
-  uint64_t bound = 0;
   unsigned __int128 tot = 0;
-  for (uint64_t i = 0; i < n; i++) {
-    tot += counts[i];
-    bound += (counts[i] >> cbits) != 0;
-  }
-  bound += (uint64_t)(tot >> cbits) + 1;
+  for (uint64_t i = 0; i < n; i++) tot += counts[i];
+  const unsigned __int128 carries = (tot >> cbits) + 1;
+  uint64_t bound = carries < n ? (uint64_t)carries : n;
   if ((rc = table_ensure_ovf(c, t, t->ovf_bound + bound))) return rc;
   t->ovf_bound += bound;
   const uint64_t chunk_max = 1ull << 26;

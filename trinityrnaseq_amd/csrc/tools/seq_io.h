@@ -1,0 +1,55 @@
+// seq_io.h -- FASTA/FASTQ ingest for the CLI tools.
+//
+// FASTA records follow PerlLib/Fasta_reader.pm:45-73: a record runs from '>'
+// to the next line starting with '>', the accession is the header's first
+// whitespace-delimited token (Sequence::new, :110), the sequence is the
+// remaining lines joined with all whitespace and control characters removed.
+// FASTQ (4-line records, '@' header) is accepted as jellyfish does.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace tools {
+
+struct SeqSet {
+  std::string seq;                 // concatenated sequences
+  std::vector<uint64_t> off;       // n+1 offsets into seq
+  std::string names;               // concatenated accessions
+  std::vector<uint64_t> name_off;  // n+1 offsets into names
+  uint64_t size() const { return off.empty() ? 0 : off.size() - 1; }
+  std::string name(uint64_t i) const {
+    return names.substr(name_off[i], name_off[i + 1] - name_off[i]);
+  }
+  void clear() {
+    seq.clear();
+    off.assign(1, 0);
+    names.clear();
+    name_off.assign(1, 0);
+  }
+};
+
+// Streaming reader over one file (mmap).  next_batch appends up to max_records
+// records (and stops once max_bases is exceeded); returns false at EOF with
+// nothing appended.
+class SeqReader {
+ public:
+  explicit SeqReader(const std::string& path, bool keep_names = true);
+  ~SeqReader();
+  bool next_batch(SeqSet& out, uint64_t max_records, uint64_t max_bases);
+  uint64_t file_size() const { return size_; }
+
+ private:
+  const char* data_ = nullptr;
+  uint64_t size_ = 0;
+  uint64_t pos_ = 0;
+  bool fastq_ = false;
+  bool keep_names_ = true;
+  int fd_ = -1;
+};
+
+// Read every record of a file.
+SeqSet read_all(const std::string& path, bool keep_names = true);
+
+}  // namespace tools
