@@ -71,6 +71,8 @@ def main() -> int:
     ap.add_argument("--max-cov", type=int, default=50)
     ap.add_argument("--cpu-pairs", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-ref-gb", type=float, default=0.0,
+                    help="also run one device-to-device copy of this many GB (HBM counter reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -98,6 +100,13 @@ def main() -> int:
     d_sel = api.DeviceBuffer(ctx, n_pairs + 64)
     wins_per_step = n_reads * (READ_LEN - K + 1)
     log("inputs resident in HBM")
+    if args.copy_ref_gb > 0:
+        nb = int(args.copy_ref_gb * 1e9)
+        a_, b_ = api.DeviceBuffer(ctx, nb), api.DeviceBuffer(ctx, nb)
+        b_.copy_from(a_, nb)
+        log(f"copy reference: {nb} bytes device-to-device")
+        a_.free()
+        b_.free()
 
     def job():
         reads = api.Reads.from_device(ctx, d_seq.ptr, d_off.ptr, n_reads)

@@ -84,6 +84,9 @@ int tk_buffer_alloc(tk_ctx* ctx, uint64_t bytes, void** d_ptr);
 int tk_buffer_free(tk_ctx* ctx, void* d_ptr);
 int tk_memcpy_h2d(tk_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
 int tk_memcpy_d2h(tk_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+/* Device-to-device copy on the context stream (also a known-size traffic
+ * reference when reading HBM byte counters). */
+int tk_memcpy_d2d(tk_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
 
 /* ---------------- reads (device-resident, 2-bit packed) ---------------- */
 /*

@@ -41,6 +41,7 @@ SIGNATURES = {
     "tk_buffer_free": (C.c_int, [_P, _P]),
     "tk_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_uint64]),
     "tk_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_uint64]),
+    "tk_memcpy_d2d": (C.c_int, [_P, _P, _P, C.c_uint64]),
     "tk_reads_from_host": (C.c_int, [_P, C.c_char_p, _P, C.c_uint64, C.POINTER(_P)]),
     "tk_reads_from_device_d": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(_P)]),
     "tk_reads_info": (C.c_int, [_P, _U64P, _U64P, _U32P]),
@@ -160,6 +161,11 @@ class DeviceBuffer:
         assert offset + a.nbytes <= self.nbytes and a.flags.c_contiguous
         _check(lib().tk_memcpy_d2h(self.ctx.h, _ptr(a), C.c_void_p(self.ptr + offset), a.nbytes),
                "tk_memcpy_d2h")
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int):
+        assert nbytes <= self.nbytes and nbytes <= other.nbytes
+        _check(lib().tk_memcpy_d2d(self.ctx.h, C.c_void_p(self.ptr), C.c_void_p(other.ptr), nbytes),
+               "tk_memcpy_d2d")
 
     def free(self):
         if self.ptr:

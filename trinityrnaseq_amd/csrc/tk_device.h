@@ -201,6 +201,21 @@ __device__ __forceinline__ uint64_t slot_count(const TableView& t, uint64_t s) {
 }
 
 // ---- wave helpers ----
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bit g of the result is set when byte g of x is non-zero.
+__device__ __forceinline__ uint32_t byte_any8(uint64_t x) {
+  x |= x >> 4;
+  x |= x >> 2;
+  x |= x >> 1;
+  x &= 0x0101010101010101ULL;
+  return (uint32_t)((x * 0x0102040810204080ULL) >> 56);
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -226,6 +241,105 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
     v = w < v ? w : v;
   }
   return v;
+}
+
+// Cooperative lookup of the 64 keys held one per lane of a wavefront.  In
+// round q the eight lanes 8g..8g+7 read the eight slots of the bucket of the
+// key held by lane 8q+g: one coalesced 64-byte request per key, and eight
+// independent loads in flight per lane.  Matching lanes deposit the raw
+// field in tmp (64 x u64, LDS); a key whose home bucket is full without a
+// match continues with a full probe (rare at load <= 0.5).  Returns the
+// count of the calling lane's key (0 = absent or pruned).  `want` = false
+// lanes take part in the rounds but skip the follow-up probe.
+__device__ __forceinline__ uint64_t coop_get64(const TableView& t, uint64_t key, bool want,
+                                               uint64_t* tmp, int lane) {
+  const uint64_t tagged = key | t.occ;
+  const uint64_t b = mix64(key) & t.bmask;
+  const int g = lane >> 3, sl = lane & 7;
+  tmp[lane] = 0;
+  uint64_t sv[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint64_t bq = __shfl(b, q * 8 + g);
+    sv[q] = t.slots[(bq << 3) + sl];
+  }
+  wave_sync();
+  uint64_t pending = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint64_t tq = __shfl(tagged, q * 8 + g);
+    const bool m = (sv[q] & t.kmo) == tq;
+    if (m) tmp[q * 8 + g] = sv[q] >> t.cshift;
+    const uint32_t gm = byte_any8(__ballot(m));
+    const uint32_t ge = byte_any8(__ballot(sv[q] == 0));
+    pending |= (uint64_t)(~gm & ~ge & 0xFFu) << (8 * q);
+  }
+  wave_sync();
+  uint64_t f = tmp[lane];
+  if (want) {
+    if ((pending >> lane) & 1ull) {
+      f = table_get(t, key);
+    } else if (f == t.cmax) {
+      const uint64_t* o = ovf_find(t, tagged);
+      f = o ? *o : f;
+    }
+  }
+  wave_sync();
+  return f;
+}
+
+// Cooperative +1 of the 64 keys held one per lane (want = lane has a key).
+// Same bucket/round layout as coop_get64.  In each group the lane holding
+// the key adds 1; otherwise the lowest empty slot of the snapshot is claimed
+// with CAS.  The snapshot may be stale, but a slot's key never changes once
+// set and every "empty" is confirmed by the CAS, so no key is inserted twice.
+// Keys left unresolved (lost CAS race, full home bucket) are finished by
+// their own lane with table_add.  Returns the number of new slots this lane
+// claimed.
+__device__ __forceinline__ uint32_t coop_add64(const TableView& t, uint64_t key, bool want,
+                                               int lane, uint32_t* err) {
+  const uint64_t tagged = key | t.occ;
+  const uint64_t b = mix64(key) & t.bmask;
+  const int g = lane >> 3, sl = lane & 7;
+  const uint64_t one = 1ull << t.cshift;
+  uint64_t todo = __ballot(want);
+  uint32_t claimed = 0;
+  uint64_t sv[8], bq[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    bq[q] = __shfl(b, q * 8 + g);
+    sv[q] = t.slots[(bq[q] << 3) + sl];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint64_t tq = __shfl(tagged, q * 8 + g);
+    const bool act = (todo >> (q * 8 + g)) & 1ull;
+    const bool m = act && (sv[q] & t.kmo) == tq;
+    const bool e = act && sv[q] == 0;
+    const uint32_t mb = (uint32_t)((__ballot(m) >> (8 * g)) & 0xFFu);
+    const uint32_t eb = (uint32_t)((__ballot(e) >> (8 * g)) & 0xFFu);
+    bool done = false;
+    uint64_t* slot = t.slots + (bq[q] << 3) + sl;
+    if (m) {
+      const uint64_t old = atomicAdd((unsigned long long*)slot, (unsigned long long)one);
+      if ((old >> t.cshift) == t.cmax) ovf_add(t, tq, t.cmax + 1, err);
+      done = true;
+    } else if (act && !mb && eb && sl == __builtin_ctz(eb)) {
+      const uint64_t old = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)(tq | one));
+      if (old == 0) {
+        claimed++;
+        done = true;
+      } else if ((old & t.kmo) == tq) {
+        const uint64_t o2 = atomicAdd((unsigned long long*)slot, (unsigned long long)one);
+        if ((o2 >> t.cshift) == t.cmax) ovf_add(t, tq, t.cmax + 1, err);
+        done = true;
+      }
+    }
+    const uint32_t gd = byte_any8(__ballot(done));
+    todo &= ~((uint64_t)gd << (8 * q));
+  }
+  if ((todo >> lane) & 1ull) claimed += table_add(t, key, 1, err);
+  return claimed;
 }
 
 }  // namespace tk

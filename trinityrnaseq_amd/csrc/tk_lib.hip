@@ -409,16 +409,22 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
   __syncthreads();
   const uint32_t total = s_off[nr];
   uint32_t mynew = 0;
-  for (uint32_t w = tid; w < total; w += blockDim.x) {
-    int lo = 0, hi = nr;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (s_off[mid] <= w) lo = mid; else hi = mid;
+  // block-uniform trip count: every lane of a wave takes part in the
+  // cooperative rounds, lanes past the end with want = false
+  for (uint32_t w0 = 0; w0 < total; w0 += blockDim.x) {
+    const uint32_t w = w0 + tid;
+    uint64_t fwd = 0;
+    bool valid = false;
+    if (w < total) {
+      int lo = 0, hi = nr;
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s_off[mid] <= w) lo = mid; else hi = mid;
+      }
+      window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
     }
-    uint64_t fwd;
-    bool valid;
-    window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
-    if (valid) mynew += table_add(t, canon_key(fwd, k, canonical), 1, &flags[0]);
+    const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
+    mynew += coop_add64(t, key, valid, tid & 63, &flags[0]);
   }
   mynew = wave_sum_u32(mynew);
   if ((tid & 63) == 0 && mynew) atomicAdd(&s_new, mynew);
@@ -751,6 +757,7 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
   const int wv = threadIdx.x >> 6;
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* vals = s_dyn + (uint64_t)wv * slab;
+  uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * 64;
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
        r += (uint64_t)gridDim.x * wpb) {
     const uint32_t L = rv.len[r];
@@ -762,12 +769,13 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
       const uint32_t p = base + lane;
       bool has = false;
       uint32_t v = 0;
+      uint64_t fwd = 0;
+      bool valid = false;
+      if (p < nwin) window_at(rv, wb, p, k, fwd, valid);
+      const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
+      const uint64_t c = coop_get64(t, key, valid, tmp, lane);
       if (p < nwin) {
-        uint64_t fwd;
-        bool valid;
-        window_at(rv, wb, p, k, fwd, valid);
         if (valid) {
-          const uint64_t c = table_get(t, canon_key(fwd, k, canonical));
           v = (c != 0 && c >= sp.min_count) ? (c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c)
                                             : sp.missing;
           has = true;
@@ -1037,12 +1045,13 @@ static int grid_for_waves(tk_ctx* ctx, uint64_t n_items, int waves_per_block) {
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
 }
 
-// LDS slab per wave for the per-read kernels; returns waves per block.
+// LDS per wave for the per-read kernels: a value slab of `slab` u32 plus a
+// 64 x u64 lookup scratch; at most 64 KiB per block.  Returns waves per block.
 static int slab_layout(uint32_t max_nwin, uint32_t* slab, int* wpb) {
   uint32_t s = std::max<uint32_t>(64, (max_nwin + 63) / 64 * 64);
-  if ((uint64_t)s * 4 > 65536) return -1;
+  if ((uint64_t)s * 4 + 512 > 65536) return -1;
   int w = 4;
-  while (w > 1 && (uint64_t)s * 4 * w > 65536) w >>= 1;
+  while (w > 1 && ((uint64_t)s * 4 + 512) * w > 65536) w >>= 1;
   *slab = s;
   *wpb = w;
   return 0;
@@ -1174,6 +1183,17 @@ int tk_memcpy_d2h(tk_ctx* c, void* dst, const void* src, uint64_t bytes) {
   if (!c || (bytes && (!dst || !src))) return fail(TK_ERR_ARG, "NULL argument");
   if (set_device(c->device)) return TK_ERR_HIP;
   if (bytes) TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  return TK_OK;
+}
+
+int tk_memcpy_d2d(tk_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return fail(TK_ERR_ARG, "NULL argument");
+  if (set_device(c->device)) return TK_ERR_HIP;
+  hipEvent_t a;
+  timing_begin(c, &a);
+  if (bytes) TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  timing_end(c, "memcpy_d2d", a);
   TK_HIP(hipStreamSynchronize(c->stream));
   return TK_OK;
 }
@@ -1574,10 +1594,11 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
   uint32_t slab = 0;
   int wpb = 0;
   if (slab_layout(max_nwin, &slab, &wpb))
-    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16384 + t->k - 1);
+    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16256 + t->k - 1);
   StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
   int grid = grid_for_waves(c, r->n_reads, wpb);
-  TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb, k_stats, reads_view(r), t->k,
+  TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 64 * 8, k_stats,
+            reads_view(r), t->k,
             t->canonical, table_view(t), sp, d_out, slab);
   return TK_OK;
 }
@@ -1737,7 +1758,7 @@ int tk_r2t_vote_reads_d(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t
   uint32_t slab = 0;
   int wpb = 0;
   if (slab_layout(max_nwin, &slab, &wpb))
-    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16384 + t->k - 1);
+    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16256 + t->k - 1);
   int grid = grid_for_waves(c, r->n_reads, wpb);
   TK_LAUNCH(c, "vote", grid, wpb * 64, (size_t)slab * 4 * wpb, k_vote, reads_view(r), t->k,
             t->canonical, t->d_slots, t->nbuckets - 1, d_out, slab);
