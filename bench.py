@@ -141,7 +141,8 @@ def main() -> int:
     ms = 1000 * total / args.steps
 
     kern = {}
-    for name in ["pack", "read_words", "hll", "count", "stats", "select", "select_flags",
+    for name in ["pack", "read_words", "hll", "count", "part_hist", "part_scatter", "part_count",
+                 "add_keys", "compact", "dump_count", "ovf_fold", "stats", "select", "select_flags",
                  "scan_u64", "scan_u32", "windows", "rehash"]:
         tot, n = ctx.kernel_time(name)
         if n:
@@ -149,7 +150,7 @@ def main() -> int:
     count_bytes = wins_per_step * COUNT_B_PER_WIN + n_reads * COUNT_B_PER_READ
     stats_bytes = wins_per_step * STATS_B_PER_WIN + n_reads * STATS_B_PER_READ
     roof = {}
-    for name, by in (("count", count_bytes), ("stats", stats_bytes)):
+    for name, by in (("count", count_bytes), ("part_count", count_bytes), ("stats", stats_bytes)):
         if name in kern:
             sec = kern[name]["total_ms"] / 1000.0 / args.steps
             roof[name] = {"achieved_gbs": by / sec / 1e9, "bytes_per_step": by,

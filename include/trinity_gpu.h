@@ -220,6 +220,85 @@ int tk_r2t_vote_reads_d(tk_ctx* ctx, const tk_ctable* c, const tk_reads* reads,
                         tk_r2t_vote* d_out);
 int tk_ctable_free(tk_ctable* c);
 
+/* ---------------- batched components (Trinity phase 2) ---------------- */
+/*
+ * Phase 2 runs one Trinity per Chrysalis component through ParaFly
+ * (Trinity:2510-2529,3652; write_partitioned_trinity_cmds.pl:79-93), each
+ * counting its reads (inchworm --reads, Trinity:2669-2672) and assigning
+ * them to its contigs (ReadsToTranscripts -p 50, Trinity:1044-1046,2241).
+ * A batch holds many components: reads[read_comp_off[c], read_comp_off[c+1])
+ * and contigs[contig_comp_off[c], contig_comp_off[c+1]) belong to component
+ * c.  One run counts every component's reads in its own table, scores each
+ * read against its component (tk_read_cov_stats semantics) and votes it
+ * against its component's contigs (tk_r2t_vote semantics; comp = index of
+ * the contig within the component).  The reads/contigs objects must outlive
+ * the batch.
+ */
+typedef struct tk_batch tk_batch;
+typedef struct {
+    double window_load;   /* table slots >= component windows / window_load (0: 0.7) */
+    int32_t stats_waves;  /* wavefronts per block of the per-read kernels (0: 4)      */
+    int32_t use_graph;    /* replay the run as one captured hipGraph                  */
+} tk_batch_opts;
+
+int tk_batch_create(tk_ctx* ctx, int k, int canonical, const tk_reads* reads,
+                    const uint64_t* read_comp_off, const tk_reads* contigs,
+                    const uint64_t* contig_comp_off, uint64_t n_comp,
+                    const tk_batch_opts* opts, tk_batch** out);
+/* d_stats / d_votes: device arrays, one entry per read (NULL skips that part). */
+int tk_batch_run_d(tk_ctx* ctx, tk_batch* b, const tk_stats_opts* opts,
+                   tk_read_stat* d_stats, tk_r2t_vote* d_votes);
+int tk_batch_run(tk_ctx* ctx, tk_batch* b, const tk_stats_opts* opts,
+                 tk_read_stat* stats, tk_r2t_vote* votes);
+/* Time the window_load x stats_waves candidates on this batch, keep the fastest. */
+int tk_batch_autotune(tk_ctx* ctx, tk_batch* b, const tk_stats_opts* opts,
+                      tk_read_stat* d_stats, tk_r2t_vote* d_votes, double* best_ms);
+int tk_batch_info(const tk_batch* b, uint64_t* device_bytes, double* window_load,
+                  int* stats_waves);
+int tk_batch_free(tk_batch* b);
+
+/* ---------------- resident normalization service ---------------- */
+/*
+ * insilico_read_normalization.pl runs one sample per process
+ * (Trinity:3421-3446): jellyfish count, fastaToKmerCoverageStats on both
+ * ends, pair merge + selection (util/insilico_read_normalization.pl:617-645,
+ * 837-845,951-976).  The service runs that job for many samples at once on
+ * one GPU: `slots` workers with their own streams overlap their kernels; a
+ * job is admitted when its device footprint fits hbm_budget_bytes (packed
+ * reads first, then the count table sized from its distinct-k-mer estimate);
+ * the table stays on the device from count to selection.
+ */
+typedef struct tk_server tk_server;
+typedef struct {
+    int32_t device;
+    int32_t slots;             /* concurrent jobs (1..16)                      */
+    int32_t k;                 /* 25                                            */
+    int32_t min_count;         /* dump -L: 2 for normalization                  */
+    int32_t max_cov;           /* nbkc_normalize.pl --max_cov                   */
+    int32_t max_cv;            /* nbkc_normalize.pl --max_CV                    */
+    double max_load;           /* count-table load factor                       */
+    uint64_t hbm_budget_bytes; /* admission budget (0: 64 GiB)                 */
+} tk_server_opts;
+
+typedef struct {
+    int32_t status;            /* TK_OK or the failing call's status            */
+    uint64_t n_selected;
+    double distinct_estimate;
+    double wait_s;             /* submit -> admitted                            */
+    double run_s;              /* admitted -> finished                          */
+    double latency_s;          /* submit -> finished                            */
+    char message[256];
+} tk_job_result;
+
+int tk_server_create(const tk_server_opts* opts, tk_server** out);
+/* seq/offsets: both.fa layout, left read i and right read n_pairs+i form pair
+ * i (sorted accession order).  selected (n_pairs bytes, may be NULL) gets the
+ * pairs nbkc_normalize.pl keeps.  Buffers must stay valid until tk_server_wait. */
+int tk_server_submit(tk_server* s, const char* seq, const uint64_t* offsets, uint64_t n_pairs,
+                     uint8_t* selected, uint64_t* job_id);
+int tk_server_wait(tk_server* s, uint64_t job_id, tk_job_result* result);
+int tk_server_free(tk_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,10 +43,24 @@ def _count(api, ctx, seqs_or_pair, k=25, canonical=True, expected=None, max_load
     return r, t, seq, off
 
 
+@pytest.fixture(params=["direct", "regions"])
+def layout(request, monkeypatch):
+    """Count through the direct kernel, or through the region-partitioned
+    passes (1 MB regions so that small tables span many regions)."""
+    if request.param == "direct":
+        monkeypatch.setenv("TK_PARTITION", "0")
+    else:
+        monkeypatch.setenv("TK_PARTITION", "1")
+        monkeypatch.setenv("TK_REGION_MB", "1")
+    return request.param
+
+
 @pytest.mark.parametrize("canonical", [True, False])
-def test_count_dump_small_se(api, ctx, small_se, canonical):
-    say("count_dump_small_se canonical", canonical)
+def test_count_dump_small_se(api, ctx, small_se, canonical, layout):
+    say("count_dump_small_se canonical", canonical, layout)
     r, t, seq, off = _count(api, ctx, small_se, 25, canonical)
+    if layout == "regions":
+        assert t.info()["capacity"] // 8 > (1 << 14)  # spans several regions
     gk, gc = t.dump()
     ok, oc = O.count(seq, off, 25, canonical)
     assert np.array_equal(gk, ok)
@@ -59,8 +73,8 @@ def test_count_dump_small_se(api, ctx, small_se, canonical):
 
 
 @pytest.mark.parametrize("k", [1, 5, 13, 25, 31])
-def test_count_edge_cases_all_k(api, ctx, k):
-    say("count_edge_cases k", k)
+def test_count_edge_cases_all_k(api, ctx, k, layout):
+    say("count_edge_cases k", k, layout)
     seqs = random_reads(100 + k, 500, 0, 200, n_rate=0.03) + edge_reads() + repetitive_reads(k, 300)
     for canonical in (True, False):
         r, t, seq, off = _count(api, ctx, seqs, k, canonical, expected=64)
@@ -79,9 +93,9 @@ def test_empty_and_short_reads(api, ctx):
         assert len(st) == len(seqs) and (len(st) == 0 or int(st["nwin"].max()) <= 100)
 
 
-def test_count_overflow_high_multiplicity(api, ctx):
+def test_count_overflow_high_multiplicity(api, ctx, layout):
     """Counts far past the 13-bit slot field (k=25) fold through the side table."""
-    say("count_overflow")
+    say("count_overflow", layout)
     base = "ACGTTGCAAGGCTTACCGATAGCTAGCATCGACTAGCATCAG"
     seqs = [base] * 20000 + [brute.rc(base)] * 5000 + ["GATTACAGATTACAGATTACAGATTACA"] * 9000
     r, t, seq, off = _count(api, ctx, seqs, 25, True, expected=64)
@@ -97,8 +111,8 @@ def test_count_overflow_high_multiplicity(api, ctx):
     assert int(st["median"][0]) == 50000
 
 
-def test_table_growth(api, ctx):
-    say("table_growth")
+def test_table_growth(api, ctx, layout):
+    say("table_growth", layout)
     seqs = random_reads(9, 20000, 100, 150, n_rate=0.0)
     r, t, seq, off = _count(api, ctx, seqs, 25, True, expected=64)
     cap = t.info()["capacity"]
@@ -119,8 +133,9 @@ def test_hll_estimate(api, ctx, small_se):
 
 @pytest.mark.parametrize("median_policy", [0, 1, 2])
 @pytest.mark.parametrize("nwin_policy", [0, 1])
-def test_read_stats_parity(api, ctx, small_se, median_policy, nwin_policy):
+def test_read_stats_parity(api, ctx, small_se, median_policy, nwin_policy, monkeypatch):
     say("read_stats", median_policy, nwin_policy)
+    monkeypatch.setenv("TK_REGION_MB", "1")
     seq0, off0 = small_se
     extra = edge_reads() + random_reads(3, 300, 0, 300, n_rate=0.05)
     eseq, eoff = concat(extra)
@@ -239,3 +254,130 @@ def test_r2t_parity(api, ctx, canonical):
     got = ct.vote(api.Reads(ctx, eseq, eoff))
     want = O.r2t_vote(eseq, eoff, keys, cv, 25, canonical)
     assert np.array_equal(got, want.astype(got.dtype))
+
+
+@pytest.mark.parametrize("shift", [0, 1, 3])
+def test_reads_from_device_unaligned(api, ctx, shift):
+    """Packing from device-resident ASCII at any byte alignment (and ending at
+    the very end of the allocation) equals packing from host memory."""
+    say("reads_from_device_unaligned", shift)
+    seqs = random_reads(55 + shift, 3000, 0, 400, n_rate=0.02) + edge_reads()
+    seq, off = concat(seqs)
+    buf = api.DeviceBuffer(ctx, len(seq) + shift)
+    buf.upload(np.frombuffer(seq, np.uint8), offset=shift)
+    doff = api.DeviceBuffer(ctx, off.nbytes)
+    doff.upload(off)
+    r = api.Reads.from_device(ctx, buf.ptr + shift, doff.ptr, len(seqs))
+    t = api.Table(ctx, 25, True, 1 << 16)
+    t.count(r)
+    gk, gc = t.dump()
+    ok, oc = O.count(seq, off, 25, True)
+    assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+    st = api.read_cov_stats(ctx, t, r)
+    assert np.array_equal(st, O.read_stats(seq, off, ok, oc, 25, True, 1, 1, 0, 0).astype(st.dtype))
+
+
+def _components(seed, n_comp, max_reads=400):
+    """Small Chrysalis-like components: transcripts, reads drawn from them
+    (both strands, 1% errors), and contig-like pieces of the transcripts."""
+    from trinityrnaseq_amd import synth
+    rng = np.random.default_rng(seed)
+    comps = []
+    for c in range(n_comp):
+        ntx = int(rng.integers(1, 4))
+        tx = synth.Transcriptome(seed * 1000 + c, ntx, 0, 300, 1500)
+        nreads = int(min(max_reads, 10 * (1 + rng.pareto(0.8))))
+        s, o = synth.se_reads(tx, nreads, 150, seed=seed * 7 + c, sub_rate=0.01)
+        reads = [s[int(o[i]):int(o[i + 1])].tobytes().decode() for i in range(nreads)]
+        contigs = []
+        for j in range(ntx):
+            t = tx.seq(j)
+            cut = int(rng.integers(100, max(101, len(t) - 100)))
+            contigs += [t[:cut], t[cut - 30:]]
+        comps.append((reads, contigs))
+    comps[3] = ([], comps[3][1])          # a component without reads
+    comps[5] = (comps[5][0], [])          # a component without contigs
+    return comps
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_components_match_per_component_oracle(api, ctx, use_graph):
+    say("batch_components graph", use_graph)
+    comps = _components(11, 40)
+    all_reads = [r for rs, _ in comps for r in rs]
+    all_contigs = [c for _, cs in comps for c in cs]
+    ro = np.cumsum([0] + [len(rs) for rs, _ in comps]).astype(np.uint64)
+    co = np.cumsum([0] + [len(cs) for _, cs in comps]).astype(np.uint64)
+    reads = api.Reads.from_list(ctx, all_reads)
+    contigs = api.Reads.from_list(ctx, all_contigs)
+    b = api.Batch(ctx, reads, ro, contigs, co, use_graph=use_graph)
+    n = len(all_reads)
+    d_st = api.DeviceBuffer(ctx, n * 32)
+    d_vo = api.DeviceBuffer(ctx, n * 12)
+    for rep in range(3):  # replays must agree with the first run
+        b.run_d(d_st.ptr, d_vo.ptr, min_count=2)
+        st = np.zeros(n, api.READ_STAT_DTYPE)
+        vo = np.zeros(n, api.R2T_VOTE_DTYPE)
+        d_st.download(st)
+        d_vo.download(vo)
+        for c, (rs, cs) in enumerate(comps):
+            if not rs:
+                continue
+            seq, off = concat(rs)
+            ok, oc = O.count(seq, off, 25, True)
+            a, z = int(ro[c]), int(ro[c + 1])
+            want = O.read_stats(seq, off, ok, oc, 25, True, 2, 1, 0, 0)
+            assert np.array_equal(st[a:z], want.astype(st.dtype)), (rep, c)
+            if cs:
+                bseq, boff = concat(cs)
+                keys, cv = O.r2t_build(bseq, boff, np.arange(len(cs)), 25, True)
+                wv = O.r2t_vote(seq, off, keys, cv, 25, True)
+            else:
+                wv = np.zeros(len(rs), O.R2T_VOTE_DTYPE)
+                wv["comp"] = 0xFFFFFFFF
+                wv["nwin"] = [max(0, len(r) - 24) for r in rs]
+            assert np.array_equal(vo[a:z], wv.astype(vo.dtype)), (rep, c)
+
+
+def _plain_job(api, ctx, seq, off, n_pairs, max_cov=50):
+    r = api.Reads(ctx, seq.tobytes(), off)
+    t = api.Table(ctx, 25, True, int(api.estimate_distinct(ctx, r, 25, True) * 1.05) + 4096)
+    t.count(r)
+    st = api.read_cov_stats(ctx, t, r, min_count=2)
+    sel, ns = api.select_pairs(ctx, st[:n_pairs], st[n_pairs:], max_cov, 1, 10000)
+    return sel, ns
+
+
+def test_server_results_equal_alone_and_in_the_mix(api, ctx):
+    """Jobs admitted concurrently (tight budget forces queueing) give exactly
+    the selections of the same jobs run alone."""
+    say("server_mix")
+    from trinityrnaseq_amd import synth
+    samples = []
+    for j, n in enumerate([20000, 60000, 5000, 90000, 30000, 45000]):
+        tx = synth.large_transcriptome(3000 + j, 800)
+        buf = np.empty(2 * n * 150, np.uint8)
+        synth.pe_reads(tx, n, 150, seed=3000 + j, out=buf)
+        off = np.arange(2 * n + 1, dtype=np.uint64) * np.uint64(150)
+        samples.append((buf, off, n))
+    alone = [_plain_job(api, ctx, b, o, n) for b, o, n in samples]
+    srv = api.Server(slots=3, hbm_budget_bytes=96 << 20)
+    try:
+        ids = [srv.submit(b, o, n) for b, o, n in samples]
+        for (sel_a, ns_a), jid in zip(alone, ids):
+            res, sel = srv.wait(jid)
+            assert res.n_selected == ns_a
+            assert np.array_equal(sel, sel_a)
+            assert res.latency_s >= res.run_s >= 0
+    finally:
+        srv.close()
+    # the smallest sample also matches the oracle's Perl-semantics selection
+    b, o, n = samples[2]
+    seq = b.tobytes()
+    ok, oc = O.count(seq, o, 25, True)
+    st = O.read_stats(seq, o, ok, oc, 25, True, 2, 1, 0, 0)
+    rows = [O.format_stats_row("x", s).split("\t") for s in st]
+    want = O.select_pairs([float(x[1]) for x in rows[:n]], [float(x[1]) for x in rows[n:]],
+                          [float(x[2]) for x in rows[:n]], [float(x[2]) for x in rows[n:]],
+                          [float(x[3]) for x in rows[:n]], [float(x[3]) for x in rows[n:]], 50, 1, 10000)
+    assert np.array_equal(alone[2][0], want)

@@ -68,12 +68,38 @@ SIGNATURES = {
     "tk_r2t_vote_reads": (C.c_int, [_P, _P, _P, _P]),
     "tk_r2t_vote_reads_d": (C.c_int, [_P, _P, _P, _P]),
     "tk_ctable_free": (C.c_int, [_P]),
+    "tk_batch_create": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _P, C.c_uint64, _P, C.POINTER(_P)]),
+    "tk_batch_run_d": (C.c_int, [_P, _P, _P, _P, _P]),
+    "tk_batch_run": (C.c_int, [_P, _P, _P, _P, _P]),
+    "tk_batch_autotune": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(C.c_double)]),
+    "tk_batch_info": (C.c_int, [_P, _U64P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "tk_batch_free": (C.c_int, [_P]),
+    "tk_server_create": (C.c_int, [_P, C.POINTER(_P)]),
+    "tk_server_submit": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _U64P]),
+    "tk_server_wait": (C.c_int, [_P, C.c_uint64, _P]),
+    "tk_server_free": (C.c_int, [_P]),
 }
 
 
 class StatsOpts(C.Structure):
     _fields_ = [("min_count", C.c_uint32), ("missing_value", C.c_uint32),
                 ("nwin_policy", C.c_int32), ("median_policy", C.c_int32)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("window_load", C.c_double), ("stats_waves", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("k", C.c_int32), ("min_count", C.c_int32),
+                ("max_cov", C.c_int32), ("max_cv", C.c_int32), ("max_load", C.c_double),
+                ("hbm_budget_bytes", C.c_uint64)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_selected", C.c_uint64), ("distinct_estimate", C.c_double),
+                ("wait_s", C.c_double), ("run_s", C.c_double), ("latency_s", C.c_double),
+                ("message", C.c_char * 256)]
 
 
 class TkError(RuntimeError):
@@ -391,3 +417,99 @@ def device_bytes_in_use() -> int:
     b = C.c_uint64()
     _check(lib().tk_device_bytes_in_use(C.byref(b)), "tk_device_bytes_in_use")
     return b.value
+
+
+class Batch:
+    """Many components (reads + contigs each) counted, scored and voted in one run."""
+
+    def __init__(self, ctx: Context, reads: Reads, read_comp_off, contigs: Reads, contig_comp_off,
+                 k: int = 25, canonical: bool = True, window_load: float = 0.0, stats_waves: int = 0,
+                 use_graph: bool = True):
+        self.ctx = ctx
+        self._keep = (reads, contigs)
+        self.n_reads = reads.info()[0]
+        ro = np.ascontiguousarray(read_comp_off, dtype=np.uint64)
+        co = np.ascontiguousarray(contig_comp_off, dtype=np.uint64)
+        assert len(ro) == len(co)
+        o = BatchOpts(window_load, stats_waves, int(use_graph))
+        self.h = C.c_void_p()
+        _check(lib().tk_batch_create(ctx.h, k, int(canonical), reads.h, _ptr(ro), contigs.h, _ptr(co),
+                                     len(ro) - 1, C.byref(o), C.byref(self.h)), "tk_batch_create")
+
+    def run(self, min_count: int = 1, missing_value: int = 1, nwin_policy: int = NWIN_AS_MISSING,
+            median_policy: int = MEDIAN_UPPER):
+        st = np.zeros(self.n_reads, dtype=READ_STAT_DTYPE)
+        vo = np.zeros(self.n_reads, dtype=R2T_VOTE_DTYPE)
+        o = StatsOpts(min_count, missing_value, nwin_policy, median_policy)
+        _check(lib().tk_batch_run(self.ctx.h, self.h, C.byref(o), _ptr(st), _ptr(vo)), "tk_batch_run")
+        return st, vo
+
+    def run_d(self, d_stats: int, d_votes: int, min_count: int = 1, missing_value: int = 1,
+              nwin_policy: int = NWIN_AS_MISSING, median_policy: int = MEDIAN_UPPER):
+        o = StatsOpts(min_count, missing_value, nwin_policy, median_policy)
+        _check(lib().tk_batch_run_d(self.ctx.h, self.h, C.byref(o), C.c_void_p(d_stats or None),
+                                    C.c_void_p(d_votes or None)), "tk_batch_run_d")
+
+    def autotune(self, d_stats: int, d_votes: int, min_count: int = 1) -> float:
+        o = StatsOpts(min_count, 1, NWIN_AS_MISSING, MEDIAN_UPPER)
+        ms = C.c_double()
+        _check(lib().tk_batch_autotune(self.ctx.h, self.h, C.byref(o), C.c_void_p(d_stats or None),
+                                       C.c_void_p(d_votes or None), C.byref(ms)), "tk_batch_autotune")
+        return ms.value
+
+    def info(self):
+        b = C.c_uint64(); wl = C.c_double(); w = C.c_int()
+        _check(lib().tk_batch_info(self.h, C.byref(b), C.byref(wl), C.byref(w)), "tk_batch_info")
+        return {"bytes": b.value, "window_load": wl.value, "stats_waves": w.value}
+
+    def free(self):
+        if self.h:
+            lib().tk_batch_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Resident normalization service (count + coverage stats + selection per sample)."""
+
+    def __init__(self, device: int = 0, slots: int = 4, k: int = 25, min_count: int = 2, max_cov: int = 50,
+                 max_cv: int = 10000, max_load: float = 0.5, hbm_budget_bytes: int = 0):
+        o = ServerOpts(device, slots, k, min_count, max_cov, max_cv, max_load, hbm_budget_bytes)
+        self.h = C.c_void_p()
+        _check(lib().tk_server_create(C.byref(o), C.byref(self.h)), "tk_server_create")
+        self._keep = {}
+
+    def submit(self, seq: np.ndarray, offsets: np.ndarray, n_pairs: int, want_selected: bool = True) -> int:
+        seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        assert len(offsets) == 2 * n_pairs + 1
+        sel = np.zeros(max(n_pairs, 1), np.uint8) if want_selected else None
+        jid = C.c_uint64()
+        _check(lib().tk_server_submit(self.h, _ptr(seq), _ptr(offsets), n_pairs,
+                                      _ptr(sel) if sel is not None else None, C.byref(jid)), "tk_server_submit")
+        self._keep[jid.value] = (seq, offsets, sel)
+        return jid.value
+
+    def wait(self, job_id: int):
+        r = JobResult()
+        _check(lib().tk_server_wait(self.h, job_id, C.byref(r)), "tk_server_wait")
+        seq, off, sel = self._keep.pop(job_id)
+        if r.status:
+            raise TkError(r.status, "server job", r.message.decode(errors="replace"))
+        return r, (sel[: (len(off) - 1) // 2] if sel is not None else None)
+
+    def close(self):
+        if self.h:
+            lib().tk_server_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

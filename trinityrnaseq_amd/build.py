@@ -32,7 +32,8 @@ HIP_FLAGS = [
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-pthread", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["tk_lib.hip", "tk_sort.hip"]
+HIP_SOURCES = ["tk_lib.hip", "tk_sort.hip", "tk_batch.hip"]
+HOST_SOURCES = ["tk_server.cpp"]
 TOOLS = {
     "jellyfish": ["tools/jellyfish_main.cpp"],
     "fastaToKmerCoverageStats": ["tools/fastaToKmerCoverageStats_main.cpp"],
@@ -66,11 +67,18 @@ def build_library(force: bool = True) -> Path:
         objs.append(o)
         if force or not _newer(o, [s] + headers):
             jobs.append([HIPCC, *HIP_FLAGS, "-c", str(s), "-o", str(o)])
+    for src in HOST_SOURCES:
+        s = CSRC / src
+        o = OBJ / (s.stem + ".o")
+        objs.append(o)
+        if force or not _newer(o, [s] + headers):
+            jobs.append([CXX, "-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", f"-I{ROOT / 'include'}",
+                         "-c", str(s), "-o", str(o)])
     with ThreadPoolExecutor(max_workers=max(1, len(jobs))) as ex:
         list(ex.map(_run, jobs))
     out = LIB / "libtrinity_gpu.so"
     tmp = LIB / "libtrinity_gpu.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, out)
     return out
 

@@ -17,6 +17,12 @@
 // Finalisation folds the side table back: the main field of every key with
 // a side entry becomes all-ones and the side entry holds the full count.
 // A field of 0 on an occupied, finalised slot is a pruned key (tombstone).
+//
+// Buckets are grouped into regions of 2^rbits consecutive buckets; the table
+// holds nb = P * 2^rbits buckets.  A key's home bucket is
+// mulhi(mix64(key), nb) and probing wraps inside the home region, so every
+// key stays in one region.  Counting can then be done region by region with
+// the active region resident in the Infinity Cache.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -75,8 +81,11 @@ __device__ __forceinline__ uint64_t canon_key(uint64_t fwd, int k, int canonical
 }
 
 struct TableView {
-  uint64_t* slots;    // nbuckets * 8
-  uint64_t bmask;     // nbuckets - 1
+  uint64_t* slots;    // base of the slot array (shared by the tables of a batch)
+  uint64_t boff;      // first bucket of this table
+  uint64_t nb;        // buckets
+  uint64_t rmask;     // buckets per region - 1
+  uint32_t rbits;     // log2(buckets per region)
   uint64_t occ;       // occupied bit (1 << 2k)
   uint64_t kmo;       // key + occupied bits
   uint64_t cmax;      // all-ones count field value
@@ -86,6 +95,14 @@ struct TableView {
   uint64_t omask;     // side capacity - 1
   uint32_t max_probe; // buckets probed before giving up
 };
+
+__device__ __forceinline__ uint64_t bucket0(const TableView& t, uint64_t key) {
+  return t.boff + __umul64hi(mix64(key), t.nb);
+}
+__device__ __forceinline__ uint64_t bucket_next(const TableView& t, uint64_t b) {
+  const uint64_t r = b - t.boff;
+  return t.boff + ((r & ~t.rmask) | ((r + 1) & t.rmask));
+}
 
 // ---- side table (field carries) ----
 __device__ __forceinline__ uint64_t ovf_slot0(const TableView& t, uint64_t tagged) {
@@ -134,13 +151,14 @@ __device__ __forceinline__ void field_add(const TableView& t, uint64_t* s, uint6
 }
 
 // Insert-or-add c >= 1 occurrences of key.  Returns 1 when a new slot was
-// claimed.  Sets err bit 0 when the probe limit is reached.
+// claimed, 0 when the key was present, 2 when max_probe buckets were full
+// (nothing added; the caller decides).
 __device__ inline uint32_t table_add(const TableView& t, uint64_t key, uint64_t c,
                                      uint32_t* err) {
   const uint64_t tagged = key | t.occ;
   const uint64_t r = c & t.cmax;
   const uint64_t q = c - r;
-  uint64_t b = mix64(key) & t.bmask;
+  uint64_t b = bucket0(t, key);
   for (uint32_t p = 0; p < t.max_probe; p++) {
     uint64_t* line = t.slots + (b << 3);
     const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(line);
@@ -162,16 +180,15 @@ __device__ inline uint32_t table_add(const TableView& t, uint64_t key, uint64_t 
         return 0u;
       }
     }
-    b = (b + 1) & t.bmask;
+    b = bucket_next(t, b);
   }
-  atomicOr(err, 1u);
-  return 0u;
+  return 2u;
 }
 
 // Count of key in a finalised table; 0 when absent or pruned.
 __device__ __forceinline__ uint64_t table_get(const TableView& t, uint64_t key) {
   const uint64_t tagged = key | t.occ;
-  uint64_t b = mix64(key) & t.bmask;
+  uint64_t b = bucket0(t, key);
   for (uint32_t p = 0; p < t.max_probe; p++) {
     const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(t.slots + (b << 3));
     ulonglong2 v0 = l2[0], v1 = l2[1], v2 = l2[2], v3 = l2[3];
@@ -187,7 +204,7 @@ __device__ __forceinline__ uint64_t table_get(const TableView& t, uint64_t key) 
       }
       if (s == 0) return 0;
     }
-    b = (b + 1) & t.bmask;
+    b = bucket_next(t, b);
   }
   return 0;
 }
@@ -254,7 +271,7 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 __device__ __forceinline__ uint64_t coop_get64(const TableView& t, uint64_t key, bool want,
                                                uint64_t* tmp, int lane) {
   const uint64_t tagged = key | t.occ;
-  const uint64_t b = mix64(key) & t.bmask;
+  const uint64_t b = bucket0(t, key);
   const int g = lane >> 3, sl = lane & 7;
   tmp[lane] = 0;
   uint64_t sv[8];
@@ -293,52 +310,76 @@ __device__ __forceinline__ uint64_t coop_get64(const TableView& t, uint64_t key,
 // the key adds 1; otherwise the lowest empty slot of the snapshot is claimed
 // with CAS.  The snapshot may be stale, but a slot's key never changes once
 // set and every "empty" is confirmed by the CAS, so no key is inserted twice.
+// All eight rounds' atomics are issued before any result is inspected, so a
+// wavefront waits for one atomic round trip per 64 keys, not eight.
 // Keys left unresolved (lost CAS race, full home bucket) are finished by
-// their own lane with table_add.  Returns the number of new slots this lane
+// their own lane with table_add; `spilled` reports a key that found no room
+// within max_probe buckets.  Returns the number of new slots this lane
 // claimed.
 __device__ __forceinline__ uint32_t coop_add64(const TableView& t, uint64_t key, bool want,
-                                               int lane, uint32_t* err) {
+                                               int lane, uint32_t* err, bool& spilled) {
   const uint64_t tagged = key | t.occ;
-  const uint64_t b = mix64(key) & t.bmask;
+  const uint64_t b = bucket0(t, key);
   const int g = lane >> 3, sl = lane & 7;
   const uint64_t one = 1ull << t.cshift;
   uint64_t todo = __ballot(want);
-  uint32_t claimed = 0;
-  uint64_t sv[8], bq[8];
+  uint64_t sv[8], bq[8], tq[8], old[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     bq[q] = __shfl(b, q * 8 + g);
     sv[q] = t.slots[(bq[q] << 3) + sl];
   }
+  uint32_t kind = 0;  // per round, 2 bits: 1 = add issued, 2 = CAS issued
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    const uint64_t tq = __shfl(tagged, q * 8 + g);
+    tq[q] = __shfl(tagged, q * 8 + g);
+    old[q] = 0;
     const bool act = (todo >> (q * 8 + g)) & 1ull;
-    const bool m = act && (sv[q] & t.kmo) == tq;
+    const bool m = act && (sv[q] & t.kmo) == tq[q];
     const bool e = act && sv[q] == 0;
     const uint32_t mb = (uint32_t)((__ballot(m) >> (8 * g)) & 0xFFu);
     const uint32_t eb = (uint32_t)((__ballot(e) >> (8 * g)) & 0xFFu);
-    bool done = false;
-    uint64_t* slot = t.slots + (bq[q] << 3) + sl;
+    unsigned long long* slot = (unsigned long long*)(t.slots + (bq[q] << 3) + sl);
     if (m) {
-      const uint64_t old = atomicAdd((unsigned long long*)slot, (unsigned long long)one);
-      if ((old >> t.cshift) == t.cmax) ovf_add(t, tq, t.cmax + 1, err);
-      done = true;
+      old[q] = atomicAdd(slot, (unsigned long long)one);
+      kind |= 1u << (2 * q);
     } else if (act && !mb && eb && sl == __builtin_ctz(eb)) {
-      const uint64_t old = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)(tq | one));
-      if (old == 0) {
+      old[q] = atomicCAS(slot, 0ULL, (unsigned long long)(tq[q] | one));
+      kind |= 2u << (2 * q);
+    }
+  }
+  uint32_t claimed = 0;
+  uint64_t carry = 0;  // bit w: the +1 of lane w's key wrapped its count field
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint32_t kq = (kind >> (2 * q)) & 3u;
+    bool done = false, wrap = false;
+    if (kq == 1u) {
+      wrap = (old[q] >> t.cshift) == t.cmax;
+      done = true;
+    } else if (kq == 2u) {
+      if (old[q] == 0) {
         claimed++;
         done = true;
-      } else if ((old & t.kmo) == tq) {
-        const uint64_t o2 = atomicAdd((unsigned long long*)slot, (unsigned long long)one);
-        if ((o2 >> t.cshift) == t.cmax) ovf_add(t, tq, t.cmax + 1, err);
+      } else if ((old[q] & t.kmo) == tq[q]) {
+        unsigned long long* slot = (unsigned long long*)(t.slots + (bq[q] << 3) + sl);
+        const uint64_t o2 = atomicAdd(slot, (unsigned long long)one);
+        wrap = (o2 >> t.cshift) == t.cmax;
         done = true;
       }
     }
     const uint32_t gd = byte_any8(__ballot(done));
     todo &= ~((uint64_t)gd << (8 * q));
+    carry |= (uint64_t)byte_any8(__ballot(wrap)) << (8 * q);
   }
-  if ((todo >> lane) & 1ull) claimed += table_add(t, key, 1, err);
+  // the key's own lane records a carry in its own table's side table
+  if ((carry >> lane) & 1ull) ovf_add(t, tagged, t.cmax + 1, err);
+  spilled = false;
+  if ((todo >> lane) & 1ull) {
+    const uint32_t r = table_add(t, key, 1, err);
+    if (r == 2u) spilled = true;
+    else claimed += r;
+  }
   return claimed;
 }
 

@@ -72,7 +72,8 @@ struct tk_table {
   int k = 25;
   int canonical = 1;
   double max_load = 0.5;
-  uint64_t nbuckets = 0;
+  uint64_t nb = 0;                 // buckets (regions * 2^rbits)
+  uint32_t rbits = 0;              // log2(buckets per region)
   uint64_t* d_slots = nullptr;
   uint64_t ovf_cap = 0;            // pairs
   uint64_t* d_ovf = nullptr;
@@ -80,6 +81,8 @@ struct tk_table {
   uint64_t n_keys = 0;             // distinct keys claimed (exact, from device counter)
   bool finalized = true;           // an empty table is trivially final
   uint64_t bytes = 0;
+  tk_table* compact = nullptr;     // keys with count >= compact_min, built for scoring
+  uint32_t compact_min = 0;
 };
 
 struct tk_ctable {

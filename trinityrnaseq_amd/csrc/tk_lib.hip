@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "tk_internal.h"
+#include "tk_perread.h"
 
 namespace tk {
 
@@ -156,7 +157,10 @@ static ReadsView reads_view(const tk_reads* r) {
 TableView table_view(const tk_table* t) {
   TableView v;
   v.slots = t->d_slots;
-  v.bmask = t->nbuckets - 1;
+  v.boff = 0;
+  v.nb = t->nb;
+  v.rbits = t->rbits;
+  v.rmask = (1ull << t->rbits) - 1;
   v.occ = 1ull << (2 * t->k);
   v.kmo = (v.occ << 1) - 1;
   v.cshift = 2 * t->k + 1;
@@ -164,28 +168,40 @@ TableView table_view(const tk_table* t) {
   v.cmax = (1ull << v.cbits) - 1;
   v.ovf = t->d_ovf;
   v.omask = t->ovf_cap - 1;
-  v.max_probe = (uint32_t)std::min<uint64_t>(t->nbuckets, 4096);
+  v.max_probe = (uint32_t)std::min<uint64_t>(1ull << t->rbits, 4096);
   return v;
 }
 
 // =====================================================================
 // Read packing
 // =====================================================================
-__global__ void k_read_words(const uint64_t* __restrict__ off, uint64_t n,
-                             uint32_t* __restrict__ len, uint64_t* __restrict__ nwords,
-                             uint32_t* __restrict__ maxlen) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t L = 0;
-  if (i < n) {
-    uint64_t l64 = off[i + 1] - off[i];
-    L = l64 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)l64;
-    len[i] = L;
-    nwords[i] = (l64 + 31) / 32;
-  } else if (i == n) {
-    nwords[i] = 0;
+// Lengths and word counts of every read; one atomicMax per block for the
+// longest read.  Grid-stride (the grid is sized to the device, not to n).
+__global__ __launch_bounds__(256) void k_read_words(const uint64_t* __restrict__ off, uint64_t n,
+                                                    uint32_t* __restrict__ len,
+                                                    uint64_t* __restrict__ nwords,
+                                                    uint32_t* __restrict__ maxlen) {
+  __shared__ uint32_t s_max[4];
+  uint32_t m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < n) {
+      const uint64_t l64 = off[i + 1] - off[i];
+      const uint32_t L = l64 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)l64;
+      len[i] = L;
+      nwords[i] = (l64 + 31) / 32;
+      m = L > m ? L : m;
+    } else {
+      nwords[i] = 0;
+    }
   }
-  L = wave_max_u32(L);
-  if ((threadIdx.x & 63) == 0 && L) atomicMax(maxlen, L);
+  m = wave_max_u32(m);
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = s_max[i] > m ? s_max[i] : m;
+    if (m) atomicMax(maxlen, m);
+  }
 }
 
 __device__ __forceinline__ uint32_t base_code(uint8_t ch, uint32_t& bad) {
@@ -203,14 +219,17 @@ __device__ __forceinline__ uint32_t base_code(uint8_t ch, uint32_t& bad) {
 }
 
 // One thread per output word: 32 bases -> 64-bit code word + 32-bit mask.
+// A block's 256 words cover at most 8 KiB of contiguous ASCII, which the
+// block first stages in LDS with coalesced aligned 4-byte loads.
 __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
                                               const uint64_t* __restrict__ off,
                                               const uint64_t* __restrict__ word_off,
                                               const uint32_t* __restrict__ len,
-                                              uint64_t n_reads, uint64_t n_words,
+                                              uint64_t n_reads, uint64_t n_words, uint64_t n_bytes,
                                               uint64_t* __restrict__ codes,
                                               uint32_t* __restrict__ nmask) {
   __shared__ uint64_t s_range[2];
+  __shared__ uint32_t s_txt[2056];
   const uint64_t w0 = (uint64_t)blockIdx.x * blockDim.x;
   if (w0 >= n_words) return;
   const uint64_t wl = std::min<uint64_t>(w0 + blockDim.x, n_words) - 1;
@@ -224,9 +243,28 @@ __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
     s_range[threadIdx.x] = lo;
   }
   __syncthreads();
+  const uint64_t rf = s_range[0], rl = s_range[1];
+  const uint64_t b_lo = off[rf] + (w0 - word_off[rf]) * 32;
+  const uint64_t b_hi = std::min<uint64_t>(off[rl] + (wl - word_off[rl]) * 32 + 32, off[rl + 1]);
+  // a0: b_lo rounded down to a 4-byte aligned address (may precede seq)
+  const int64_t mis = (int64_t)(reinterpret_cast<uintptr_t>(seq) & 3);
+  const int64_t a0 = (((int64_t)b_lo + mis) & ~(int64_t)3) - mis;
+  const uint32_t n4 = (uint32_t)(((int64_t)b_hi - a0 + 3) >> 2);
+  for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x) {
+    const int64_t a = a0 + 4 * (int64_t)i;
+    uint32_t v = 0;
+    if (a >= 0 && (uint64_t)a + 4 <= n_bytes) {
+      v = *reinterpret_cast<const uint32_t*>(seq + a);
+    } else {
+      for (int j = 0; j < 4; j++)
+        if (a + j >= 0 && (uint64_t)(a + j) < n_bytes) v |= (uint32_t)(uint8_t)seq[a + j] << (8 * j);
+    }
+    s_txt[i] = v;
+  }
+  __syncthreads();
   const uint64_t w = w0 + threadIdx.x;
-  if (w >= n_words) return;
-  uint64_t lo = s_range[0], hi = s_range[1] + 1;
+  if (w > wl) return;
+  uint64_t lo = rf, hi = rl + 1;
   while (hi - lo > 1) {
     uint64_t mid = (lo + hi) >> 1;
     if (word_off[mid] <= w) lo = mid; else hi = mid;
@@ -235,13 +273,13 @@ __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
   const uint64_t j = w - word_off[r];
   const uint32_t L = len[r];
   const uint64_t b0 = j * 32;
-  const char* src = seq + off[r] + b0;
+  const uint8_t* src = reinterpret_cast<const uint8_t*>(s_txt) + ((int64_t)(off[r] + b0) - a0);
   const uint32_t nb = (uint32_t)std::min<uint64_t>(32, L - b0);
   uint64_t code = 0;
   uint32_t mask = 0;
   for (uint32_t i = 0; i < 32; i++) {
     uint32_t bad = 1, c = 0;
-    if (i < nb) c = base_code((uint8_t)src[i], bad);
+    if (i < nb) c = base_code(src[i], bad);
     code = (code << 2) | c;
     mask = (mask << 1) | bad;
   }
@@ -277,7 +315,7 @@ static int reads_build(tk_ctx* ctx, const char* d_seq, const uint64_t* d_off, ui
     hipError_t e = hipMemsetAsync(ctx->d_flags, 0, 64 * 4, ctx->stream);
     if (e != hipSuccess) { rc = hip_fail(e, "memset flags"); break; }
     {
-      uint64_t grid = ceil_div(n + 1, 256);
+      uint64_t grid = std::min<uint64_t>(ceil_div(n + 1, 256), (uint64_t)std::max(ctx->num_cu, 1) * 8);
       hipEvent_t a;
       timing_begin(ctx, &a);
       hipLaunchKernelGGL(k_read_words, dim3(grid), dim3(256), 0, ctx->stream, d_off, n,
@@ -302,7 +340,7 @@ static int reads_build(tk_ctx* ctx, const char* d_seq, const uint64_t* d_off, ui
       hipEvent_t a;
       timing_begin(ctx, &a);
       hipLaunchKernelGGL(k_pack, dim3(ceil_div(nwords, 256)), dim3(256), 0, ctx->stream,
-                         d_seq, d_off, r->d_word_off, r->d_len, n, nwords, r->d_codes,
+                         d_seq, d_off, r->d_word_off, r->d_len, n, nwords, n_bases, r->d_codes,
                          r->d_nmask);
       timing_end(ctx, "pack", a);
       if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_pack"); break; }
@@ -344,14 +382,18 @@ static int reads_windows(tk_ctx* ctx, const tk_reads* rc_, int k, uint64_t* tota
 constexpr int kHllBits = 12;
 constexpr int kHllRegs = 1 << kHllBits;
 
-__global__ __launch_bounds__(256) void k_hll(ReadsView rv, int k, int canonical,
+// Windows whose hash has its top `sbits` bits zero (a 2^-sbits sample chosen
+// by key, so every occurrence of a sampled key is sampled) update the
+// registers; the estimate is scaled by 2^sbits.
+__global__ __launch_bounds__(256) void k_hll(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
+                                             int canonical, uint32_t sbits,
                                              uint32_t* __restrict__ regs) {
   __shared__ uint32_t s_reg[kHllRegs];
   for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) s_reg[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const uint64_t wpb = blockDim.x >> 6;
-  for (uint64_t r = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); r < rv.n_reads;
+  for (uint64_t r = r_begin + (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); r < r_end;
        r += (uint64_t)gridDim.x * wpb) {
     const uint32_t L = rv.len[r];
     const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
@@ -361,9 +403,11 @@ __global__ __launch_bounds__(256) void k_hll(ReadsView rv, int k, int canonical,
       bool valid;
       window_at(rv, wb, p, k, fwd, valid);
       if (!valid) continue;
-      const uint64_t h = mix64(canon_key(fwd, k, canonical) ^ 0x5bd1e9955bd1e995ULL);
+      const uint64_t h0 = mix64(canon_key(fwd, k, canonical) ^ 0x5bd1e9955bd1e995ULL);
+      if (sbits && (h0 >> (64 - sbits))) continue;
+      const uint64_t h = h0 << sbits;
       const uint32_t idx = (uint32_t)(h >> (64 - kHllBits));
-      const uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1));
+      const uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1 + sbits));
       const uint32_t rho = (uint32_t)__clzll(rest) + 1;
       atomicMax(&s_reg[idx], rho);
     }
@@ -424,13 +468,152 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
       window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
     }
     const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
-    mynew += coop_add64(t, key, valid, tid & 63, &flags[0]);
+    bool lost = false;
+    mynew += coop_add64(t, key, valid, tid & 63, &flags[0], lost);
+    if (lost) atomicOr(&flags[0], 1u);
   }
   mynew = wave_sum_u32(mynew);
   if ((tid & 63) == 0 && mynew) atomicAdd(&s_new, mynew);
   __syncthreads();
   if (tid == 0 && s_new)
     atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)s_new);
+}
+
+// ---------------------------------------------------------------------
+// Region-partitioned counting (tables of several Infinity-Cache-sized
+// regions).  Pass 1 groups the canonical keys of a read chunk by region
+// (per-block histogram, one exclusive scan, scatter to exact positions);
+// pass 2 streams the region-major key array through the cooperative insert,
+// so the few regions being updated at any time stay cache-resident.
+// ---------------------------------------------------------------------
+constexpr int kPartReads = 256;
+constexpr uint32_t kMaxRegions = 2048;
+
+// Window prefix of the block's reads [r0, r0+nr), nr <= 256, blockDim 256.
+__device__ __forceinline__ uint32_t block_windows_256(const ReadsView& rv, uint64_t r0, int nr,
+                                                      int k, uint32_t* s_off, uint64_t* s_wb,
+                                                      uint32_t* s_wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  uint32_t nw = 0;
+  if (tid < nr) {
+    const uint32_t L = rv.len[r0 + tid];
+    nw = L >= (uint32_t)k ? L - k + 1 : 0;
+    s_wb[tid] = rv.word_off[r0 + tid];
+  }
+  uint32_t x = nw;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wsum[wv] = x;
+  __syncthreads();
+  uint32_t add = 0;
+  for (int i = 0; i < wv; i++) add += s_wsum[i];
+  s_off[tid + 1] = x + add;
+  if (tid == 0) s_off[0] = 0;
+  __syncthreads();
+  return s_off[nr];
+}
+
+__device__ __forceinline__ int read_of_window(const uint32_t* s_off, int nr, uint32_t w) {
+  int lo = 0, hi = nr;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_off[mid] <= w) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_part_hist(ReadsView rv, uint64_t r_begin, uint64_t r_end,
+                                                   int k, int canonical, TableView t, uint32_t nparts,
+                                                   uint32_t nblocks, uint64_t* __restrict__ hist) {
+  __shared__ uint32_t s_off[kPartReads + 1];
+  __shared__ uint64_t s_wb[kPartReads];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_cnt[kMaxRegions];
+  const uint64_t r0 = r_begin + (uint64_t)blockIdx.x * kPartReads;
+  const int nr = (int)std::min<uint64_t>(kPartReads, r_end - r0);
+  for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) s_cnt[i] = 0;
+  const uint32_t total = block_windows_256(rv, r0, nr, k, s_off, s_wb, s_wsum);
+  for (uint32_t w = threadIdx.x; w < total; w += blockDim.x) {
+    const int j = read_of_window(s_off, nr, w);
+    uint64_t fwd;
+    bool valid;
+    window_at(rv, s_wb[j], w - s_off[j], k, fwd, valid);
+    if (valid) atomicAdd(&s_cnt[bucket0(t, canon_key(fwd, k, canonical)) >> t.rbits], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x)
+    hist[(uint64_t)i * nblocks + blockIdx.x] = s_cnt[i];
+}
+
+__global__ __launch_bounds__(256) void k_part_scatter(ReadsView rv, uint64_t r_begin, uint64_t r_end,
+                                                      int k, int canonical, TableView t,
+                                                      uint32_t nparts, uint32_t nblocks,
+                                                      const uint64_t* __restrict__ base,
+                                                      uint64_t* __restrict__ keys_out) {
+  __shared__ uint32_t s_off[kPartReads + 1];
+  __shared__ uint64_t s_wb[kPartReads];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_rank[kMaxRegions];
+  __shared__ uint64_t s_base[kMaxRegions];
+  const uint64_t r0 = r_begin + (uint64_t)blockIdx.x * kPartReads;
+  const int nr = (int)std::min<uint64_t>(kPartReads, r_end - r0);
+  for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+    s_rank[i] = 0;
+    s_base[i] = base[(uint64_t)i * nblocks + blockIdx.x];
+  }
+  const uint32_t total = block_windows_256(rv, r0, nr, k, s_off, s_wb, s_wsum);
+  for (uint32_t w = threadIdx.x; w < total; w += blockDim.x) {
+    const int j = read_of_window(s_off, nr, w);
+    uint64_t fwd;
+    bool valid;
+    window_at(rv, s_wb[j], w - s_off[j], k, fwd, valid);
+    if (valid) {
+      const uint64_t key = canon_key(fwd, k, canonical);
+      const uint32_t p = (uint32_t)(bucket0(t, key) >> t.rbits);
+      keys_out[s_base[p] + atomicAdd(&s_rank[p], 1u)] = key;
+    }
+  }
+}
+
+// Pass 2: 4096 keys per block, cooperative insert with a short probe limit;
+// keys whose region is crowded past it go to the spill list, or, once that is
+// full, are marked in place with bit 63; both are finished after the table
+// grows.  flags[0] error bits, flags[2..3] new slots, flags[4..5] spill
+// cursor, flags[6] marks present.
+constexpr uint32_t kPartKeysPerBlock = 4096;
+
+__global__ __launch_bounds__(256) void k_part_count(uint64_t* __restrict__ keys, uint64_t n,
+                                                    TableView t, uint64_t* __restrict__ spill,
+                                                    uint64_t spill_cap, uint32_t* __restrict__ flags) {
+  const uint64_t a = (uint64_t)blockIdx.x * kPartKeysPerBlock;
+  const uint64_t e = std::min<uint64_t>(n, a + kPartKeysPerBlock);
+  const int lane = threadIdx.x & 63;
+  uint32_t mynew = 0;
+  uint32_t spill_err = 0;
+  for (uint64_t i0 = a; i0 < e; i0 += blockDim.x) {
+    const uint64_t i = i0 + threadIdx.x;
+    const bool want = i < e;
+    const uint64_t key = want ? __builtin_nontemporal_load(keys + i) : 0;
+    bool lost = false;
+    mynew += coop_add64(t, key, want, lane, &flags[0], lost);
+    if (lost) {
+      const unsigned long long at =
+          atomicAdd(reinterpret_cast<unsigned long long*>(&flags[4]), 1ull);
+      if (at < spill_cap) {
+        spill[at] = key;
+      } else {
+        keys[i] = key | (1ull << 63);  // left for the retry pass
+        spill_err = 1;
+      }
+    }
+  }
+  if (spill_err) atomicOr(&flags[6], 1u);
+  mynew = wave_sum_u32(mynew);
+  if (lane == 0 && mynew)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)mynew);
 }
 
 // Move every slot of an old table into a larger one (slot values unchanged).
@@ -440,7 +623,7 @@ __global__ void k_rehash(const uint64_t* __restrict__ old, uint64_t n_slots, Tab
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t s = old[i];
     if (!s) continue;
-    uint64_t b = mix64(s & (t.occ - 1)) & t.bmask;
+    uint64_t b = bucket0(t, s & (t.occ - 1));
     bool done = false;
     for (uint32_t p = 0; p < t.max_probe && !done; p++) {
       uint64_t* line = t.slots + (b << 3);
@@ -451,7 +634,7 @@ __global__ void k_rehash(const uint64_t* __restrict__ old, uint64_t n_slots, Tab
           break;
         }
       }
-      b = (b + 1) & t.bmask;
+      b = bucket_next(t, b);
     }
     if (!done) atomicOr(&flags[0], 1u);
   }
@@ -468,7 +651,7 @@ __global__ void k_ovf_rehash(const uint64_t* __restrict__ old, uint64_t n_pairs,
 }
 
 __device__ inline uint64_t* table_find_slot(const TableView& t, uint64_t tagged) {
-  uint64_t b = mix64(tagged & (t.occ - 1)) & t.bmask;
+  uint64_t b = bucket0(t, tagged & (t.occ - 1));
   for (uint32_t p = 0; p < t.max_probe; p++) {
     uint64_t* line = t.slots + (b << 3);
     for (int j = 0; j < 8; j++) {
@@ -476,7 +659,7 @@ __device__ inline uint64_t* table_find_slot(const TableView& t, uint64_t tagged)
       if ((s & t.kmo) == tagged) return &line[j];
       if (s == 0) return nullptr;
     }
-    b = (b + 1) & t.bmask;
+    b = bucket_next(t, b);
   }
   return nullptr;
 }
@@ -529,21 +712,44 @@ static int table_ensure_ovf(tk_ctx* ctx, tk_table* t, uint64_t bound) {
     TK_HIP(hipStreamSynchronize(ctx->stream));
     dev_free(old, oldcap * 16);
   }
-  t->bytes = t->nbuckets * 64 + t->ovf_cap * 16;
+  t->bytes = t->nb * 64 + t->ovf_cap * 16;
   return TK_OK;
 }
 
+// Region size for large tables (bytes); a region is the unit counted while
+// resident in the Infinity Cache.
+// TK_REGION_MB overrides the 64 MB default (read when a table is laid out).
+static uint64_t region_bytes() {
+  const char* e = getenv("TK_REGION_MB");
+  const uint64_t mb = e ? strtoull(e, nullptr, 10) : 64;
+  return next_pow2(std::max<uint64_t>(mb, 1)) << 20;
+}
+
+// Bucket layout holding `keys` at load max_load: nb = regions * 2^rbits.
+static void table_layout(uint64_t keys, double max_load, uint64_t* nb, uint32_t* rbits) {
+  const uint64_t need = std::max<uint64_t>(ceil_div((uint64_t)std::ceil((double)keys / max_load), 8), 8);
+  const uint64_t rb_max = region_bytes() / 64;
+  uint64_t rb = std::min<uint64_t>(next_pow2(need), rb_max);
+  uint32_t bits = 0;
+  while ((1ull << bits) < rb) bits++;
+  *rbits = bits;
+  *nb = ceil_div(need, rb) * rb;
+}
+
+// Rehash into a table that holds max(min_keys, 2 x current) keys at max_load.
 static int table_grow(tk_ctx* ctx, tk_table* t, uint64_t min_keys) {
-  uint64_t nb = t->nbuckets;
-  while ((double)min_keys > t->max_load * (double)(nb * 8)) nb <<= 1;
-  if (nb == t->nbuckets) return TK_OK;
+  uint64_t nb = 0;
+  uint32_t rbits = 0;
+  const uint64_t doubled = (uint64_t)(2.0 * t->max_load * (double)(t->nb * 8));
+  table_layout(std::max<uint64_t>(min_keys, doubled), t->max_load, &nb, &rbits);
   uint64_t* ns = nullptr;
   int rc = table_alloc_slots(ctx, nb, &ns);
   if (rc) return rc;
   uint64_t* old = t->d_slots;
-  uint64_t oldnb = t->nbuckets;
+  uint64_t oldnb = t->nb;
   t->d_slots = ns;
-  t->nbuckets = nb;
+  t->nb = nb;
+  t->rbits = rbits;
   TK_HIP(hipMemsetAsync(ctx->d_flags, 0, 4, ctx->stream));
   TableView v = table_view(t);
   TK_LAUNCH(ctx, "rehash", std::min<uint64_t>(ceil_div(oldnb * 8, 256), 65536), 256, 0,
@@ -551,13 +757,24 @@ static int table_grow(tk_ctx* ctx, tk_table* t, uint64_t min_keys) {
   TK_HIP(hipMemcpyAsync(ctx->h_flags, ctx->d_flags, 4, hipMemcpyDeviceToHost, ctx->stream));
   TK_HIP(hipStreamSynchronize(ctx->stream));
   dev_free(old, oldnb * 64);
-  t->bytes = t->nbuckets * 64 + t->ovf_cap * 16;
+  t->bytes = t->nb * 64 + t->ovf_cap * 16;
   if (ctx->h_flags[0]) return fail(TK_ERR_FULL, "rehash into %llu buckets failed",
                                    (unsigned long long)nb);
+  (void)oldnb;
   return TK_OK;
 }
 
+// Free the scoring copy of a table (it is rebuilt on demand).
+static void drop_compact(tk_table* t) {
+  if (t->compact) {
+    tk_table_free(t->compact);
+    t->compact = nullptr;
+    t->compact_min = 0;
+  }
+}
+
 static int table_set_final(tk_ctx* ctx, tk_table* t, bool final_) {
+  if (!final_) drop_compact(t);  // the table is about to change
   if (t->finalized == final_) return TK_OK;
   TK_HIP(hipMemsetAsync(ctx->d_flags, 0, 4, ctx->stream));
   TableView v = table_view(t);
@@ -580,7 +797,9 @@ __global__ void k_load(const uint64_t* __restrict__ keys, const uint32_t* __rest
     const uint64_t key = keys[i];
     if (c == 0) continue;
     if (key & ~(t.occ - 1)) { atomicOr(&flags[0], 8u); continue; }
-    mynew += table_add(t, key, c, &flags[0]);
+    const uint32_t r = table_add(t, key, c, &flags[0]);
+    if (r == 2u) atomicOr(&flags[0], 1u);
+    else mynew += r;
   }
   mynew = wave_sum_u32(mynew);
   if ((threadIdx.x & 63) == 0 && mynew)
@@ -719,36 +938,28 @@ __global__ void k_lookup(TableView t, const uint64_t* __restrict__ keys, uint64_
   }
 }
 
+// Copy the keys of a finalised table whose count is >= min_count into dst.
+__global__ void k_compact(TableView src, uint64_t n_slots, uint64_t min_count, TableView dst,
+                          uint32_t* __restrict__ flags) {
+  uint32_t mynew = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = src.slots[i];
+    if (!s || (s >> src.cshift) == 0) continue;
+    const uint64_t c = slot_count(src, s);
+    if (c < min_count) continue;
+    const uint32_t r = table_add(dst, s & (src.occ - 1), c, &flags[0]);
+    if (r == 2u) atomicOr(&flags[0], 1u);
+    else mynew += r;
+  }
+  mynew = wave_sum_u32(mynew);
+  if ((threadIdx.x & 63) == 0 && mynew)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)mynew);
+}
+
 // =====================================================================
 // Per-read coverage statistics: one wavefront per read
 // =====================================================================
-struct StatsParams {
-  uint32_t min_count;
-  uint32_t missing;
-  int32_t nwin_policy;
-  int32_t median_policy;
-};
-
-// kth smallest (0-based) of vals[0..n) by radix select over bits hb..0.
-__device__ uint32_t wave_select(const uint32_t* vals, uint32_t n, uint32_t kth, int hb,
-                                int lane) {
-  uint32_t prefix = 0, pmask = 0;
-  for (int bit = hb; bit >= 0; --bit) {
-    uint32_t c = 0;
-    for (uint32_t i = lane; i < n; i += 64) {
-      const uint32_t v = vals[i];
-      c += (((v & pmask) == prefix) && !((v >> bit) & 1u)) ? 1u : 0u;
-    }
-    c = wave_sum_u32(c);
-    if (kth >= c) {
-      kth -= c;
-      prefix |= 1u << bit;
-    }
-    pmask |= 1u << bit;
-  }
-  return prefix;
-}
-
 __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonical, TableView t,
                                                StatsParams sp, tk_read_stat* __restrict__ out,
                                                uint32_t slab) {
@@ -759,81 +970,8 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
   uint32_t* vals = s_dyn + (uint64_t)wv * slab;
   uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * 64;
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
-       r += (uint64_t)gridDim.x * wpb) {
-    const uint32_t L = rv.len[r];
-    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
-    const uint64_t wb = rv.word_off[r];
-    uint32_t n = 0, vmax = 0;
-    uint64_t sum = 0, sqlo = 0, sqhi = 0;
-    for (uint32_t base = 0; base < nwin; base += 64) {
-      const uint32_t p = base + lane;
-      bool has = false;
-      uint32_t v = 0;
-      uint64_t fwd = 0;
-      bool valid = false;
-      if (p < nwin) window_at(rv, wb, p, k, fwd, valid);
-      const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
-      const uint64_t c = coop_get64(t, key, valid, tmp, lane);
-      if (p < nwin) {
-        if (valid) {
-          v = (c != 0 && c >= sp.min_count) ? (c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c)
-                                            : sp.missing;
-          has = true;
-        } else if (sp.nwin_policy == TK_NWIN_AS_MISSING) {
-          v = sp.missing;
-          has = true;
-        }
-      }
-      const uint64_t m = __ballot(has);
-      if (has) {
-        vals[n + __popcll(m & ((1ull << lane) - 1))] = v;
-        sum += v;
-        const uint64_t sq = (uint64_t)v * v;
-        sqlo += sq;
-        sqhi += (sqlo < sq);
-        vmax = v > vmax ? v : vmax;
-      }
-      n += (uint32_t)__popcll(m);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    sum = wave_sum_u64(sum);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint64_t ol = __shfl_xor(sqlo, o);
-      const uint64_t oh = __shfl_xor(sqhi, o);
-      const uint64_t nl = sqlo + ol;
-      sqhi = sqhi + oh + (nl < sqlo);
-      sqlo = nl;
-    }
-    uint32_t med = 0;
-    if (n) {
-      vmax = wave_max_u32(vmax);
-      const int hb = 31 - __clz(vmax | 1u);
-      if (sp.median_policy == TK_MEDIAN_LOWER) {
-        med = wave_select(vals, n, (n - 1) / 2, hb, lane);
-      } else if (sp.median_policy == TK_MEDIAN_MEAN_FLOOR && !(n & 1)) {
-        const uint64_t a = wave_select(vals, n, n / 2 - 1, hb, lane);
-        const uint64_t b = wave_select(vals, n, n / 2, hb, lane);
-        med = (uint32_t)((a + b) / 2);
-      } else {
-        med = wave_select(vals, n, n / 2, hb, lane);
-      }
-    }
-    if (lane == 0) {
-      tk_read_stat st;
-      st.median = med;
-      st.nwin = n;
-      st.sum = sum;
-      st.sumsq_lo = sqlo;
-      st.sumsq_hi = sqhi;
-      out[r] = st;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
+       r += (uint64_t)gridDim.x * wpb)
+    stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
 }
 
 // =====================================================================
@@ -884,46 +1022,6 @@ __global__ void k_select(const tk_read_stat* __restrict__ L, const tk_read_stat*
 // =====================================================================
 // ReadsToTranscripts
 // =====================================================================
-// ctable: 64-byte buckets of four 16-byte slots (tagged key, ~comp).
-__device__ inline void ctable_add(uint64_t* slots, uint64_t bmask, uint64_t occ, uint64_t key,
-                                  uint32_t comp, uint32_t& mynew, uint32_t* err) {
-  const uint64_t tagged = key | occ;
-  uint64_t b = mix64(key) & bmask;
-  for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
-    uint64_t* line = slots + (b << 3);
-    for (int j = 0; j < 4; j++) {
-      uint64_t s = line[2 * j];
-      if (s == 0) {
-        s = atomicCAS((unsigned long long*)&line[2 * j], 0ULL, (unsigned long long)tagged);
-        if (s == 0) { mynew++; s = tagged; }
-      }
-      if (s == tagged) {
-        atomicMax((unsigned int*)&line[2 * j + 1], ~comp);
-        return;
-      }
-    }
-    b = (b + 1) & bmask;
-  }
-  atomicOr(err, 1u);
-}
-
-__device__ __forceinline__ uint32_t ctable_get(const uint64_t* slots, uint64_t bmask, uint64_t occ,
-                                               uint64_t key) {
-  const uint64_t tagged = key | occ;
-  uint64_t b = mix64(key) & bmask;
-  for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
-    const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(slots + (b << 3));
-    ulonglong2 v[4] = {l2[0], l2[1], l2[2], l2[3]};
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (v[j].x == tagged) return ~(uint32_t)v[j].y;
-      if (v[j].x == 0) return 0xFFFFFFFFu;
-    }
-    b = (b + 1) & bmask;
-  }
-  return 0xFFFFFFFFu;
-}
-
 __global__ __launch_bounds__(256) void k_ctable_build(ReadsView rv, const uint32_t* __restrict__ comp,
                                                       int k, int canonical, uint64_t* slots,
                                                       uint64_t bmask, uint32_t* __restrict__ flags) {
@@ -965,7 +1063,7 @@ __global__ __launch_bounds__(256) void k_ctable_build(ReadsView rv, const uint32
     uint64_t fwd;
     bool valid;
     window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
-    if (valid) ctable_add(slots, bmask, occ, canon_key(fwd, k, canonical), s_comp[lo], mynew, &flags[0]);
+    if (valid) ctable_add(slots, 0, bmask, occ, canon_key(fwd, k, canonical), s_comp[lo], mynew, &flags[0]);
   }
   mynew = wave_sum_u32(mynew);
   if ((tid & 63) == 0 && mynew) atomicAdd(&s_new, mynew);
@@ -982,61 +1080,9 @@ __global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical
   const int wv = threadIdx.x >> 6;
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* hits = s_dyn + (uint64_t)wv * slab;
-  const uint64_t occ = 1ull << (2 * k);
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
-       r += (uint64_t)gridDim.x * wpb) {
-    const uint32_t L = rv.len[r];
-    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
-    const uint64_t wb = rv.word_off[r];
-    uint32_t n = 0;
-    for (uint32_t base = 0; base < nwin; base += 64) {
-      const uint32_t p = base + lane;
-      uint32_t c = 0xFFFFFFFFu;
-      if (p < nwin) {
-        uint64_t fwd;
-        bool valid;
-        window_at(rv, wb, p, k, fwd, valid);
-        if (valid) c = ctable_get(slots, bmask, occ, canon_key(fwd, k, canonical));
-      }
-      const bool has = c != 0xFFFFFFFFu;
-      const uint64_t m = __ballot(has);
-      if (has) hits[n + __popcll(m & ((1ull << lane) - 1))] = c;
-      n += (uint32_t)__popcll(m);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // Walk the distinct components in ascending order; keep the first with
-    // the strictly largest number of hits (ties -> smallest id).
-    uint32_t best = 0xFFFFFFFFu, bestn = 0;
-    uint64_t prev = 0;  // candidates are > prev-1
-    bool first = true;
-    while (n) {
-      uint32_t cand = 0xFFFFFFFFu;
-      for (uint32_t i = lane; i < n; i += 64) {
-        const uint32_t v = hits[i];
-        if ((first || v > (uint32_t)prev) && v < cand) cand = v;
-      }
-      cand = wave_min_u32(cand);
-      if (cand == 0xFFFFFFFFu) break;
-      uint32_t cnt = 0;
-      for (uint32_t i = lane; i < n; i += 64) cnt += hits[i] == cand;
-      cnt = wave_sum_u32(cnt);
-      if (cnt > bestn) { bestn = cnt; best = cand; }
-      prev = cand;
-      first = false;
-    }
-    if (lane == 0) {
-      tk_r2t_vote v;
-      v.comp = best;
-      v.hits = bestn;
-      v.nwin = nwin;
-      out[r] = v;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
+       r += (uint64_t)gridDim.x * wpb)
+    vote_read(rv, r, k, canonical, slots, 0, bmask, hits, lane, out + r);
 }
 
 static int grid_for_waves(tk_ctx* ctx, uint64_t n_items, int waves_per_block) {
@@ -1271,34 +1317,17 @@ int tk_reads_free(tk_reads* r) {
 }
 
 // ---------------- table ----------------
+static int hll_estimate(tk_ctx* c, const tk_reads* r, uint64_t rb, uint64_t re, int k, int canonical,
+                        uint64_t windows, double* est);
+
 int tk_estimate_distinct(tk_ctx* c, const tk_reads* r, int k, int canonical, double* est) {
   if (!c || !r || !est) return fail(TK_ERR_ARG, "NULL argument");
   if (k < 1 || k > 31) return fail(TK_ERR_ARG, "k=%d outside [1,31]", k);
   if (set_device(c->device)) return TK_ERR_HIP;
-  void* p = nullptr;
-  int rc = ensure_scratch(c, 0, kHllRegs * 4, &p);
+  uint64_t total = 0;
+  int rc = reads_windows(c, r, k, &total);
   if (rc) return rc;
-  uint32_t* regs = (uint32_t*)p;
-  TK_HIP(hipMemsetAsync(regs, 0, kHllRegs * 4, c->stream));
-  if (r->n_reads) {
-    int grid = grid_for_waves(c, r->n_reads, 4);
-    TK_LAUNCH(c, "hll", grid, 256, 0, k_hll, reads_view(r), k, canonical, regs);
-  }
-  std::vector<uint32_t> h(kHllRegs);
-  TK_HIP(hipMemcpyAsync(h.data(), regs, kHllRegs * 4, hipMemcpyDeviceToHost, c->stream));
-  TK_HIP(hipStreamSynchronize(c->stream));
-  const double m = kHllRegs;
-  double s = 0;
-  int zeros = 0;
-  for (uint32_t v : h) {
-    s += std::ldexp(1.0, -(int)v);
-    zeros += v == 0;
-  }
-  const double alpha = 0.7213 / (1.0 + 1.079 / m);
-  double e = alpha * m * m / s;
-  if (e <= 2.5 * m && zeros) e = m * std::log(m / zeros);
-  *est = e;
-  return TK_OK;
+  return hll_estimate(c, r, 0, r->n_reads, k, canonical, total, est);
 }
 
 int tk_table_create(tk_ctx* c, int k, int canonical, uint64_t expected, double max_load,
@@ -1313,13 +1342,12 @@ int tk_table_create(tk_ctx* c, int k, int canonical, uint64_t expected, double m
   t->k = k;
   t->canonical = canonical ? 1 : 0;
   t->max_load = max_load;
-  uint64_t slots = (uint64_t)std::ceil((double)std::max<uint64_t>(expected, 64) / max_load);
-  t->nbuckets = next_pow2(ceil_div(slots, 8));
-  int rc = table_alloc_slots(c, t->nbuckets, &t->d_slots);
+  table_layout(std::max<uint64_t>(expected, 64), max_load, &t->nb, &t->rbits);
+  int rc = table_alloc_slots(c, t->nb, &t->d_slots);
   if (!rc) rc = table_ensure_ovf(c, t, 0);
   if (!rc) { hipError_t e = hipStreamSynchronize(c->stream); if (e != hipSuccess) rc = hip_fail(e, "table init"); }
   if (rc) { tk_table_free(t); return rc; }
-  t->bytes = t->nbuckets * 64 + t->ovf_cap * 16;
+  t->bytes = t->nb * 64 + t->ovf_cap * 16;
   *out = t;
   return TK_OK;
 }
@@ -1328,9 +1356,202 @@ int tk_table_info(const tk_table* t, int* k, int* canonical, uint64_t* capacity,
   if (!t) return fail(TK_ERR_ARG, "table is NULL");
   if (k) *k = t->k;
   if (canonical) *canonical = t->canonical;
-  if (capacity) *capacity = t->nbuckets * 8;
+  if (capacity) *capacity = t->nb * 8;
   if (bytes) *bytes = t->bytes;
   return TK_OK;
+}
+
+// HyperLogLog estimate of the distinct keys of reads [rb, re).  Large inputs
+// (>= 64M windows) update the registers from a 1/16 key sample.
+static int hll_estimate(tk_ctx* c, const tk_reads* r, uint64_t rb, uint64_t re, int k, int canonical,
+                        uint64_t windows, double* est) {
+  void* p = nullptr;
+  int rc = ensure_scratch(c, 0, kHllRegs * 4, &p);
+  if (rc) return rc;
+  uint32_t* regs = (uint32_t*)p;
+  const uint32_t sbits = windows >= (64ull << 20) ? 4 : 0;
+  TK_HIP(hipMemsetAsync(regs, 0, kHllRegs * 4, c->stream));
+  if (re > rb) {
+    int grid = grid_for_waves(c, re - rb, 4);
+    TK_LAUNCH(c, "hll", grid, 256, 0, k_hll, reads_view(r), rb, re, k, canonical, sbits, regs);
+  }
+  std::vector<uint32_t> h(kHllRegs);
+  TK_HIP(hipMemcpyAsync(h.data(), regs, kHllRegs * 4, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  const double m = kHllRegs;
+  double sum = 0;
+  int zeros = 0;
+  for (uint32_t v : h) {
+    sum += std::ldexp(1.0, -(int)v);
+    zeros += v == 0;
+  }
+  const double alpha = 0.7213 / (1.0 + 1.079 / m);
+  double e = alpha * m * m / sum;
+  if (e <= 2.5 * m && zeros) e = m * std::log(m / zeros);
+  *est = std::ldexp(e, (int)sbits);
+  return TK_OK;
+}
+
+// TK_COMPACT=1 scores against a compact copy when min_count > 1 (off by
+// default: measured no faster on the 20M-pair workload once build time counts).
+static bool compact_enabled() {
+  const char* e = getenv("TK_COMPACT");
+  return e && e[0] == '1';
+}
+
+// TK_PARTITION=0 keeps large tables on the direct (unpartitioned) count.
+static bool partition_enabled() {
+  const char* e = getenv("TK_PARTITION");
+  return !(e && e[0] == '0');
+}
+
+__global__ void k_add_keys(const uint64_t* __restrict__ keys, uint64_t n, TableView t,
+                           uint32_t* __restrict__ flags) {
+  uint32_t mynew = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = table_add(t, keys[i], 1, &flags[0]);
+    if (r == 2u) atomicOr(&flags[0], 1u);
+    else mynew += r;
+  }
+  mynew = wave_sum_u32(mynew);
+  if ((threadIdx.x & 63) == 0 && mynew)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)mynew);
+}
+
+// Insert keys marked with bit 63 (clearing the mark on success).
+__global__ void k_retry_marked(uint64_t* __restrict__ keys, uint64_t n, TableView t,
+                               uint32_t* __restrict__ flags) {
+  uint32_t mynew = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = keys[i];
+    if (!(v >> 63)) continue;
+    const uint32_t r = table_add(t, v & ~(1ull << 63), 1, &flags[0]);
+    if (r == 2u) {
+      atomicOr(&flags[6], 1u);
+    } else {
+      mynew += r;
+      keys[i] = v & ~(1ull << 63);
+    }
+  }
+  mynew = wave_sum_u32(mynew);
+  if ((threadIdx.x & 63) == 0 && mynew)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)mynew);
+}
+
+// Count reads [rb, rb+nreads) through the region-partitioned passes.
+static int count_partitioned(tk_ctx* c, tk_table* t, const tk_reads* r, uint64_t rb, uint64_t nreads,
+                             uint64_t windows_bound) {
+  int rc0 = TK_OK;
+  // When the chunk could overfill the table, size it from an estimate of the
+  // chunk's distinct keys first, so that the regions keys are grouped by are
+  // the final ones.
+  if ((double)(t->n_keys + windows_bound) > 0.9 * (double)(t->nb * 8)) {
+    double est = 0;
+    if ((rc0 = hll_estimate(c, r, rb, rb + nreads, t->k, t->canonical, windows_bound, &est))) return rc0;
+    const double want = (double)t->n_keys + 1.05 * est + 4096;
+    if (want > t->max_load * (double)(t->nb * 8))
+      if ((rc0 = table_grow(c, t, (uint64_t)want))) return rc0;
+  }
+  const uint32_t nparts = (uint32_t)(t->nb >> t->rbits);
+  if (nparts > kMaxRegions) return fail(TK_ERR_ARG, "table has %u regions (limit %u)", nparts, kMaxRegions);
+  const uint64_t nblocks = ceil_div(nreads, kPartReads);
+  if (nblocks * nparts > 0x7FFFFFFFull) return fail(TK_ERR_ARG, "partition histogram too large");
+  const uint64_t nh = nblocks * nparts;
+  uint64_t *hist = nullptr, *base = nullptr, *keys = nullptr;
+  int rc = dev_alloc((void**)&hist, (nh + 1) * 8);
+  if (rc) return rc;
+  if ((rc = dev_alloc((void**)&base, (nh + 1) * 8))) { dev_free(hist, (nh + 1) * 8); return rc; }
+  TableView v = table_view(t);
+  uint64_t nkeys = 0;
+  do {
+    hipError_t e = hipMemsetAsync(hist + nh, 0, 8, c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "memset"); break; }
+    {
+      hipEvent_t a;
+      timing_begin(c, &a);
+      hipLaunchKernelGGL(k_part_hist, dim3(nblocks), dim3(256), 0, c->stream, reads_view(r), rb,
+                         rb + nreads, t->k, t->canonical, v, nparts, (uint32_t)nblocks, hist);
+      timing_end(c, "part_hist", a);
+      if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_part_hist"); break; }
+    }
+    if ((rc = exclusive_scan_u64(c, hist, base, nh + 1))) break;
+    e = hipMemcpyAsync(&nkeys, base + nh, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "part total"); break; }
+    if (!nkeys) break;
+    if ((rc = dev_alloc((void**)&keys, nkeys * 8))) break;
+    {
+      hipEvent_t a;
+      timing_begin(c, &a);
+      hipLaunchKernelGGL(k_part_scatter, dim3(nblocks), dim3(256), 0, c->stream, reads_view(r), rb,
+                         rb + nreads, t->k, t->canonical, v, nparts, (uint32_t)nblocks, base, keys);
+      timing_end(c, "part_scatter", a);
+      if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_part_scatter"); break; }
+    }
+  } while (0);
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(hist, (nh + 1) * 8);
+  dev_free(base, (nh + 1) * 8);
+  if (rc || !nkeys) {
+    if (keys) dev_free(keys, nkeys * 8);
+    return rc;
+  }
+  // pass 2
+  const uint64_t spill_cap = std::max<uint64_t>(1 << 20, std::min<uint64_t>(nkeys / 16, 64ull << 20));
+  void* sp = nullptr;
+  if ((rc = ensure_scratch(c, 1, spill_cap * 8, &sp))) { dev_free(keys, nkeys * 8); return rc; }
+  TableView v2 = v;
+  v2.max_probe = std::min<uint32_t>(v.max_probe, 32);
+  hipError_t e = hipMemsetAsync(c->d_flags, 0, 32, c->stream);
+  if (e == hipSuccess) {
+    hipEvent_t a;
+    timing_begin(c, &a);
+    hipLaunchKernelGGL(k_part_count, dim3(ceil_div(nkeys, kPartKeysPerBlock)), dim3(256), 0, c->stream,
+                       keys, nkeys, v2, (uint64_t*)sp, spill_cap, c->d_flags);
+    timing_end(c, "part_count", a);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(c->h_flags, c->d_flags, 32, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) { dev_free(keys, nkeys * 8); return hip_fail(e, "k_part_count"); }
+  uint64_t nnew = 0, nspill = 0;
+  memcpy(&nnew, &c->h_flags[2], 8);
+  memcpy(&nspill, &c->h_flags[4], 8);
+  bool marked = c->h_flags[6] != 0;
+  t->n_keys += nnew;
+  if (c->h_flags[0]) { dev_free(keys, nkeys * 8); return fail(TK_ERR_FULL, "count: table error flags=%u", c->h_flags[0]); }
+  // Crowded regions: grow, add the spilled occurrences, then retry marked
+  // keys (growing again while any still finds no room).
+  for (int round = 0; (nspill || marked) && !rc; round++) {
+    if (round > 8) { rc = fail(TK_ERR_FULL, "count: keys still find no room after growing"); break; }
+    const uint64_t pending = std::min(nspill, spill_cap) + (marked ? nkeys : 0);
+    if ((rc = table_grow(c, t, t->n_keys + std::min<uint64_t>(pending, nkeys)))) break;
+    e = hipMemsetAsync(c->d_flags, 0, 32, c->stream);
+    TableView v3 = table_view(t);
+    if (e == hipSuccess && nspill) {
+      const uint64_t ns = std::min(nspill, spill_cap);
+      hipLaunchKernelGGL(k_add_keys, dim3(std::min<uint64_t>(ceil_div(ns, 256), 4096)), dim3(256), 0,
+                         c->stream, (const uint64_t*)sp, ns, v3, c->d_flags);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && marked) {
+      hipLaunchKernelGGL(k_retry_marked, dim3(std::min<uint64_t>(ceil_div(nkeys, 256), 65536)), dim3(256), 0,
+                         c->stream, keys, nkeys, v3, c->d_flags);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_flags, c->d_flags, 32, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "spill retry"); break; }
+    if (c->h_flags[0] & 1u) { rc = fail(TK_ERR_FULL, "count: spilled keys found no room"); break; }
+    memcpy(&nnew, &c->h_flags[2], 8);
+    t->n_keys += nnew;
+    nspill = 0;
+    marked = c->h_flags[6] != 0;
+  }
+  dev_free(keys, nkeys * 8);
+  return rc;
 }
 
 int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
@@ -1348,12 +1569,27 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
   t->ovf_bound += (total >> cbits) + 1;
   uint64_t rb = 0;
   while (rb < r->n_reads) {
+    const uint32_t nparts = (uint32_t)(t->nb >> t->rbits);
+    if (partition_enabled() && nparts >= 2 && nparts <= kMaxRegions && max_nwin) {
+      size_t fr = 0, tot = 0;
+      TK_HIP(hipMemGetInfo(&fr, &tot));
+      const uint64_t budget = std::min<uint64_t>((uint64_t)(0.45 * (double)fr), 96ull << 30);
+      uint64_t nreads = std::max<uint64_t>(kPartReads, budget / 8 / max_nwin);
+      nreads = std::min<uint64_t>(nreads, r->n_reads - rb);
+      const uint64_t bound = (rb == 0 && nreads == r->n_reads) ? total : nreads * max_nwin;
+      if ((rc = count_partitioned(c, t, r, rb, nreads, bound))) return rc;
+      rb += nreads;
+      if ((double)t->n_keys > t->max_load * (double)(t->nb * 8)) {
+        if ((rc = table_grow(c, t, t->n_keys))) return rc;
+      }
+      continue;
+    }
     // Read chunk whose windows cannot push the table past 0.9 load even if
     // every window were a new key.
-    double room = 0.9 * (double)(t->nbuckets * 8) - (double)t->n_keys;
+    double room = 0.9 * (double)(t->nb * 8) - (double)t->n_keys;
     uint64_t nreads = max_nwin ? (uint64_t)(room / max_nwin) : r->n_reads;
     if (nreads < 64 && rb + nreads < r->n_reads) {
-      if ((rc = table_grow(c, t, std::max<uint64_t>(2 * t->n_keys, t->nbuckets * 16)))) return rc;
+      if ((rc = table_grow(c, t, 2 * t->n_keys))) return rc;
       continue;
     }
     nreads = std::min<uint64_t>(nreads, r->n_reads - rb);
@@ -1370,7 +1606,7 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
     memcpy(&nnew, &c->h_flags[2], 8);
     t->n_keys += nnew;
     rb += nreads;
-    if ((double)t->n_keys > t->max_load * (double)(t->nbuckets * 8)) {
+    if ((double)t->n_keys > t->max_load * (double)(t->nb * 8)) {
       if ((rc = table_grow(c, t, t->n_keys))) return rc;
     }
   }
@@ -1400,7 +1636,7 @@ int tk_table_load(tk_ctx* c, tk_table* t, const uint64_t* keys, const uint32_t* 
   if ((rc = dev_alloc((void**)&dc, cm * 4))) { dev_free(dk, cm * 8); return rc; }
   uint64_t i = 0;
   while (i < n && !rc) {
-    double room = 0.9 * (double)(t->nbuckets * 8) - (double)t->n_keys;
+    double room = 0.9 * (double)(t->nb * 8) - (double)t->n_keys;
     uint64_t m = std::min<uint64_t>(std::min<uint64_t>(cm, n - i), room > 0 ? (uint64_t)room : 0);
     if (m < std::min<uint64_t>(cm, n - i) / 2 || m == 0) {
       rc = table_grow(c, t, std::max<uint64_t>(t->n_keys + std::min<uint64_t>(cm, n - i), 2 * t->n_keys));
@@ -1428,7 +1664,7 @@ int tk_table_load(tk_ctx* c, tk_table* t, const uint64_t* keys, const uint32_t* 
     memcpy(&nnew, &c->h_flags[2], 8);
     t->n_keys += nnew;
     i += m;
-    if ((double)t->n_keys > t->max_load * (double)(t->nbuckets * 8)) rc = table_grow(c, t, t->n_keys);
+    if ((double)t->n_keys > t->max_load * (double)(t->nb * 8)) rc = table_grow(c, t, t->n_keys);
   }
   (void)hipStreamSynchronize(c->stream);
   dev_free(dk, cm * 8);
@@ -1444,7 +1680,7 @@ int tk_table_dump_size(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_
   unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
   TK_HIP(hipMemsetAsync(d, 0, 8, c->stream));
   TableView v = table_view(t);
-  uint64_t ns = t->nbuckets * 8;
+  uint64_t ns = t->nb * 8;
   TK_LAUNCH(c, "dump_count", std::min<uint64_t>(ceil_div(ns, 256), 65536), 256, 0, k_dump_count,
             v, ns, (uint64_t)lo, (uint64_t)hi, d);
   uint64_t h = 0;
@@ -1473,7 +1709,7 @@ int tk_table_dump(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_t* ke
     hipError_t e = hipMemsetAsync(cur, 0, 8, c->stream);
     if (e != hipSuccess) { rc = hip_fail(e, "memset"); break; }
     TableView v = table_view(t);
-    uint64_t ns = t->nbuckets * 8;
+    uint64_t ns = t->nb * 8;
     {
       hipEvent_t a;
       timing_begin(c, &a);
@@ -1508,7 +1744,7 @@ int tk_table_histo(tk_ctx* c, tk_table* t, uint64_t low, uint64_t high, uint64_t
   if ((rc = ensure_scratch(c, 1, nb * 8, &p))) return rc;
   TK_HIP(hipMemsetAsync(p, 0, nb * 8, c->stream));
   TableView v = table_view(t);
-  uint64_t ns = t->nbuckets * 8;
+  uint64_t ns = t->nb * 8;
   TK_LAUNCH(c, "histo", std::min<uint64_t>(ceil_div(ns, 256), 4096), 256, 0, k_histo, v, ns, base,
             ceil_, inc, nb, (unsigned long long*)p);
   TK_HIP(hipMemcpyAsync(bins, p, nb * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1519,6 +1755,7 @@ int tk_table_histo(tk_ctx* c, tk_table* t, uint64_t low, uint64_t high, uint64_t
 int tk_table_prune(tk_ctx* c, tk_table* t, uint32_t min_count, const uint8_t* keep, uint64_t* n_kept) {
   if (!c || !t) return fail(TK_ERR_ARG, "NULL argument");
   if (set_device(c->device)) return TK_ERR_HIP;
+  drop_compact(t);
   int rc = table_set_final(c, t, true);
   if (rc) return rc;
   const uint64_t kt = (uint64_t)(t->k + 1) * (t->k + 1) * (t->k + 1);
@@ -1532,7 +1769,7 @@ int tk_table_prune(tk_ctx* c, tk_table* t, uint32_t min_count, const uint8_t* ke
   unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
   TK_HIP(hipMemsetAsync(d, 0, 8, c->stream));
   TableView v = table_view(t);
-  uint64_t ns = t->nbuckets * 8;
+  uint64_t ns = t->nb * 8;
   TK_LAUNCH(c, "prune", std::min<uint64_t>(ceil_div(ns, 256), 65536), 256, 0, k_prune, v, ns, t->k,
             (uint64_t)min_count, dkeep, d);
   uint64_t h = 0;
@@ -1572,13 +1809,55 @@ int tk_table_lookup(tk_ctx* c, tk_table* t, const uint64_t* keys, uint64_t n, ui
 int tk_table_free(tk_table* t) {
   if (!t) return TK_OK;
   (void)hipSetDevice(t->device);
-  if (t->d_slots) dev_free(t->d_slots, t->nbuckets * 64);
+  drop_compact(t);
+  if (t->d_slots) dev_free(t->d_slots, t->nb * 64);
   if (t->d_ovf) dev_free(t->d_ovf, t->ovf_cap * 16);
   delete t;
   return TK_OK;
 }
 
 // ---------------- stats ----------------
+// Scoring with min_count > 1 treats keys below min_count as missing.  When
+// those are most of the table (error k-mers of a read set), score against a
+// compact copy holding only the keys >= min_count: an order of magnitude
+// fewer, mostly cache-resident buckets.  Returns the table to score against.
+static int scoring_table(tk_ctx* c, tk_table* t, uint32_t min_count, tk_table** use) {
+  *use = t;
+  if (min_count <= 1 || !compact_enabled()) return TK_OK;
+  if (t->compact && t->compact_min == min_count) { *use = t->compact; return TK_OK; }
+  drop_compact(t);
+  uint64_t n_ge = 0;
+  int rc = tk_table_dump_size(c, t, min_count, 0xFFFFFFFFu, &n_ge);
+  if (rc) return rc;
+  if ((double)n_ge > 0.5 * (double)t->n_keys) return TK_OK;
+  tk_table* ct = nullptr;
+  if ((rc = tk_table_create(c, t->k, t->canonical, n_ge + 64, 0.5, &ct))) return rc;
+  // every carry of the source may land in the copy's side table
+  if ((rc = table_ensure_ovf(c, ct, std::max<uint64_t>(t->ovf_bound, 1)))) { tk_table_free(ct); return rc; }
+  ct->finalized = false;
+  TK_HIP(hipMemsetAsync(c->d_flags, 0, 16, c->stream));
+  TableView sv = table_view(t), dv = table_view(ct);
+  const uint64_t ns = t->nb * 8;
+  {
+    hipEvent_t a;
+    timing_begin(c, &a);
+    hipLaunchKernelGGL(k_compact, dim3(std::min<uint64_t>(ceil_div(ns, 256), 65536)), dim3(256), 0,
+                       c->stream, sv, ns, (uint64_t)min_count, dv, c->d_flags);
+    timing_end(c, "compact", a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { tk_table_free(ct); return hip_fail(e, "k_compact"); }
+  }
+  TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 16, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  if (c->h_flags[0]) { tk_table_free(ct); return fail(TK_ERR_FULL, "compact copy found no room"); }
+  memcpy(&ct->n_keys, &c->h_flags[2], 8);
+  if ((rc = table_set_final(c, ct, true))) { tk_table_free(ct); return rc; }
+  t->compact = ct;
+  t->compact_min = min_count;
+  *use = ct;
+  return TK_OK;
+}
+
 int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stats_opts* o,
                         tk_read_stat* d_out) {
   if (!c || !t || !r || !o || (r->n_reads && !d_out)) return fail(TK_ERR_ARG, "NULL argument");
@@ -1595,11 +1874,13 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
   int wpb = 0;
   if (slab_layout(max_nwin, &slab, &wpb))
     return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16256 + t->k - 1);
+  tk_table* use = t;
+  if ((rc = scoring_table(c, t, o->min_count, &use))) return rc;
   StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
   int grid = grid_for_waves(c, r->n_reads, wpb);
   TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 64 * 8, k_stats,
             reads_view(r), t->k,
-            t->canonical, table_view(t), sp, d_out, slab);
+            t->canonical, table_view(use), sp, d_out, slab);
   return TK_OK;
 }
 

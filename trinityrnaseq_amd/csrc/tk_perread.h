@@ -1,0 +1,214 @@
+// tk_perread.h -- per-read device routines shared by the single-sample
+// kernels (tk_lib.hip) and the batched-component kernels (tk_batch.hip).
+#pragma once
+#include "tk_device.h"
+#include "../../include/trinity_gpu.h"
+
+namespace tk {
+
+struct StatsParams {
+  uint32_t min_count;
+  uint32_t missing;
+  int32_t nwin_policy;
+  int32_t median_policy;
+};
+
+// kth smallest (0-based) of vals[0..n) by radix select over bits hb..0.
+__device__ inline uint32_t wave_select(const uint32_t* vals, uint32_t n, uint32_t kth, int hb,
+                                int lane) {
+  uint32_t prefix = 0, pmask = 0;
+  for (int bit = hb; bit >= 0; --bit) {
+    uint32_t c = 0;
+    for (uint32_t i = lane; i < n; i += 64) {
+      const uint32_t v = vals[i];
+      c += (((v & pmask) == prefix) && !((v >> bit) & 1u)) ? 1u : 0u;
+    }
+    c = wave_sum_u32(c);
+    if (kth >= c) {
+      kth -= c;
+      prefix |= 1u << bit;
+    }
+    pmask |= 1u << bit;
+  }
+  return prefix;
+}
+
+// fastaToKmerCoverageStats for read r, one wavefront: window values staged
+// in `vals` (LDS, >= nwin entries), radix-select median, exact integer sums.
+// tmp: 64 x u64 LDS scratch of the cooperative lookup.  Lane 0 writes *out.
+__device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int canonical,
+                                  const TableView& t, const StatsParams& sp, uint32_t* vals,
+                                  uint64_t* tmp, int lane, tk_read_stat* out) {
+    const uint32_t L = rv.len[r];
+    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
+    const uint64_t wb = rv.word_off[r];
+    uint32_t n = 0, vmax = 0;
+    uint64_t sum = 0, sqlo = 0, sqhi = 0;
+    for (uint32_t base = 0; base < nwin; base += 64) {
+      const uint32_t p = base + lane;
+      bool has = false;
+      uint32_t v = 0;
+      uint64_t fwd = 0;
+      bool valid = false;
+      if (p < nwin) window_at(rv, wb, p, k, fwd, valid);
+      const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
+      const uint64_t c = coop_get64(t, key, valid, tmp, lane);
+      if (p < nwin) {
+        if (valid) {
+          v = (c != 0 && c >= sp.min_count) ? (c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c)
+                                            : sp.missing;
+          has = true;
+        } else if (sp.nwin_policy == TK_NWIN_AS_MISSING) {
+          v = sp.missing;
+          has = true;
+        }
+      }
+      const uint64_t m = __ballot(has);
+      if (has) {
+        vals[n + __popcll(m & ((1ull << lane) - 1))] = v;
+        sum += v;
+        const uint64_t sq = (uint64_t)v * v;
+        sqlo += sq;
+        sqhi += (sqlo < sq);
+        vmax = v > vmax ? v : vmax;
+      }
+      n += (uint32_t)__popcll(m);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    sum = wave_sum_u64(sum);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t ol = __shfl_xor(sqlo, o);
+      const uint64_t oh = __shfl_xor(sqhi, o);
+      const uint64_t nl = sqlo + ol;
+      sqhi = sqhi + oh + (nl < sqlo);
+      sqlo = nl;
+    }
+    uint32_t med = 0;
+    if (n) {
+      vmax = wave_max_u32(vmax);
+      const int hb = 31 - __clz(vmax | 1u);
+      if (sp.median_policy == TK_MEDIAN_LOWER) {
+        med = wave_select(vals, n, (n - 1) / 2, hb, lane);
+      } else if (sp.median_policy == TK_MEDIAN_MEAN_FLOOR && !(n & 1)) {
+        const uint64_t a = wave_select(vals, n, n / 2 - 1, hb, lane);
+        const uint64_t b = wave_select(vals, n, n / 2, hb, lane);
+        med = (uint32_t)((a + b) / 2);
+      } else {
+        med = wave_select(vals, n, n / 2, hb, lane);
+      }
+    }
+    if (lane == 0) {
+      tk_read_stat st;
+      st.median = med;
+      st.nwin = n;
+      st.sum = sum;
+      st.sumsq_lo = sqlo;
+      st.sumsq_hi = sqhi;
+      *out = st;
+    }
+  wave_sync();
+}
+
+// Component table: 64-byte buckets of four 16-byte slots (tagged key, ~comp);
+// buckets [boff, boff + bmask] belong to one table.
+__device__ inline void ctable_add(uint64_t* slots, uint64_t boff, uint64_t bmask, uint64_t occ,
+                                  uint64_t key, uint32_t comp, uint32_t& mynew, uint32_t* err) {
+  const uint64_t tagged = key | occ;
+  uint64_t b = mix64(key) & bmask;
+  for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
+    uint64_t* line = slots + ((boff + b) << 3);
+    for (int j = 0; j < 4; j++) {
+      uint64_t s = line[2 * j];
+      if (s == 0) {
+        s = atomicCAS((unsigned long long*)&line[2 * j], 0ULL, (unsigned long long)tagged);
+        if (s == 0) { mynew++; s = tagged; }
+      }
+      if (s == tagged) {
+        atomicMax((unsigned int*)&line[2 * j + 1], ~comp);
+        return;
+      }
+    }
+    b = (b + 1) & bmask;
+  }
+  atomicOr(err, 1u);
+}
+
+__device__ __forceinline__ uint32_t ctable_get(const uint64_t* slots, uint64_t boff, uint64_t bmask,
+                                               uint64_t occ, uint64_t key) {
+  const uint64_t tagged = key | occ;
+  uint64_t b = mix64(key) & bmask;
+  for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
+    const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(slots + ((boff + b) << 3));
+    ulonglong2 v[4] = {l2[0], l2[1], l2[2], l2[3]};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (v[j].x == tagged) return ~(uint32_t)v[j].y;
+      if (v[j].x == 0) return 0xFFFFFFFFu;
+    }
+    b = (b + 1) & bmask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+// ReadsToTranscripts vote of read r, one wavefront: component ids of the
+// hitting windows staged in `hits` (LDS), distinct ids walked in ascending
+// order so the first strictly largest count wins (ties -> smallest id).
+__device__ inline void vote_read(const ReadsView& rv, uint64_t r, int k, int canonical,
+                                 const uint64_t* slots, uint64_t boff, uint64_t bmask,
+                                 uint32_t* hits, int lane, tk_r2t_vote* out) {
+  const uint64_t occ = 1ull << (2 * k);
+    const uint32_t L = rv.len[r];
+    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
+    const uint64_t wb = rv.word_off[r];
+    uint32_t n = 0;
+    for (uint32_t base = 0; base < nwin; base += 64) {
+      const uint32_t p = base + lane;
+      uint32_t c = 0xFFFFFFFFu;
+      if (p < nwin) {
+        uint64_t fwd;
+        bool valid;
+        window_at(rv, wb, p, k, fwd, valid);
+        if (valid) c = ctable_get(slots, boff, bmask, occ, canon_key(fwd, k, canonical));
+      }
+      const bool has = c != 0xFFFFFFFFu;
+      const uint64_t m = __ballot(has);
+      if (has) hits[n + __popcll(m & ((1ull << lane) - 1))] = c;
+      n += (uint32_t)__popcll(m);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Walk the distinct components in ascending order; keep the first with
+    // the strictly largest number of hits (ties -> smallest id).
+    uint32_t best = 0xFFFFFFFFu, bestn = 0;
+    uint64_t prev = 0;  // candidates are > prev-1
+    bool first = true;
+    while (n) {
+      uint32_t cand = 0xFFFFFFFFu;
+      for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t v = hits[i];
+        if ((first || v > (uint32_t)prev) && v < cand) cand = v;
+      }
+      cand = wave_min_u32(cand);
+      if (cand == 0xFFFFFFFFu) break;
+      uint32_t cnt = 0;
+      for (uint32_t i = lane; i < n; i += 64) cnt += hits[i] == cand;
+      cnt = wave_sum_u32(cnt);
+      if (cnt > bestn) { bestn = cnt; best = cand; }
+      prev = cand;
+      first = false;
+    }
+    if (lane == 0) {
+      tk_r2t_vote v;
+      v.comp = best;
+      v.hits = bestn;
+      v.nwin = nwin;
+      *out = v;
+    }
+  wave_sync();
+}
+
+}  // namespace tk

@@ -1,0 +1,238 @@
+// tk_server.cpp -- resident normalization service on one GPU.
+//
+// The reference normalizes one sample per insilico_read_normalization.pl
+// process (Trinity:3421-3446; util/insilico_read_normalization.pl:270-464):
+// jellyfish count (:617-633), fastaToKmerCoverageStats on left and right
+// (:837-845), nbkc_merge_left_right_stats.pl + nbkc_normalize.pl (:951-976).
+// The server keeps that job's data on the device from count to selection
+// and runs many samples at once:
+//   * `slots` worker threads, each with its own context (HIP stream), so the
+//     kernels of different samples overlap on the GPU;
+//   * admission by device footprint in two stages: the packed reads (known
+//     from the input size) before upload, the count table once its HyperLogLog
+//     estimate is known; a job waits until its bytes fit the budget;
+//   * results: selected pair flags (host), distinct estimate, per-stage times.
+#include <stdint.h>
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/trinity_gpu.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double secs(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double>(b - a).count();
+}
+
+struct Job {
+  uint64_t id = 0;
+  const char* seq = nullptr;        // both.fa layout: pairs' left reads, then right reads
+  const uint64_t* offsets = nullptr;
+  uint64_t n_pairs = 0;
+  uint8_t* selected = nullptr;      // host, n_pairs
+  Clock::time_point t_submit;
+  tk_job_result res{};
+  bool done = false;
+};
+
+}  // namespace
+
+struct tk_server {
+  tk_server_opts opts{};
+  std::vector<std::thread> workers;
+  std::mutex mu;
+  std::condition_variable cv_queue, cv_mem, cv_done;
+  std::deque<std::shared_ptr<Job>> queue;
+  std::map<uint64_t, std::shared_ptr<Job>> jobs;
+  uint64_t next_id = 1;
+  uint64_t budget = 0, in_use = 0;
+  bool stop = false;
+  int init_error = 0;
+  std::string init_msg;
+};
+
+namespace {
+
+// Wait until `bytes` fit the budget (a job larger than the whole budget runs
+// alone), then reserve them.
+void reserve(tk_server* s, uint64_t bytes) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_mem.wait(lk, [&] { return s->in_use == 0 || s->in_use + bytes <= s->budget || s->stop; });
+  s->in_use += bytes;
+}
+
+void release(tk_server* s, uint64_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->in_use -= bytes;
+  }
+  s->cv_mem.notify_all();
+}
+
+int run_job(tk_server* s, tk_ctx* ctx, Job& j) {
+  const tk_server_opts& o = s->opts;
+  const uint64_t n_reads = 2 * j.n_pairs;
+  const uint64_t n_bases = j.offsets[n_reads];
+  // stage 1: reads (ASCII staging + packed codes + stats output)
+  const uint64_t read_bytes = n_bases + n_bases * 3 / 8 + n_reads * (16 + sizeof(tk_read_stat)) + j.n_pairs;
+  const auto t0 = Clock::now();
+  reserve(s, read_bytes);
+  const auto t1 = Clock::now();
+  j.res.wait_s = secs(j.t_submit, t1);
+  int rc = TK_OK;
+  tk_reads* reads = nullptr;
+  tk_table* table = nullptr;
+  void* d_stats = nullptr;
+  void* d_sel = nullptr;
+  uint64_t table_bytes = 0;
+  do {
+    if ((rc = tk_reads_from_host(ctx, j.seq, j.offsets, n_reads, &reads))) break;
+    double est = 0;
+    if ((rc = tk_estimate_distinct(ctx, reads, o.k, 1, &est))) break;
+    j.res.distinct_estimate = est;
+    const uint64_t expected = (uint64_t)(est * 1.05) + 4096;
+    // stage 2: the table (slots at load <= max_load, 8 bytes each)
+    table_bytes = (uint64_t)((double)expected / o.max_load * 8.0) + (64ull << 20);
+    reserve(s, table_bytes);
+    if ((rc = tk_table_create(ctx, o.k, 1, expected, o.max_load, &table))) break;
+    if ((rc = tk_count(ctx, table, reads))) break;
+    if ((rc = tk_buffer_alloc(ctx, n_reads * sizeof(tk_read_stat), &d_stats))) break;
+    if ((rc = tk_buffer_alloc(ctx, j.n_pairs + 64, &d_sel))) break;
+    tk_stats_opts so{(uint32_t)o.min_count, 1, TK_NWIN_AS_MISSING, TK_MEDIAN_UPPER};
+    if ((rc = tk_read_cov_stats_d(ctx, table, reads, &so, (tk_read_stat*)d_stats))) break;
+    uint32_t max_len = 0;
+    tk_reads_info(reads, nullptr, nullptr, &max_len);
+    const uint32_t max_nwin = max_len >= (uint32_t)o.k ? max_len - o.k + 1 : 0;
+    const tk_read_stat* dl = (const tk_read_stat*)d_stats;
+    uint64_t nsel = 0;
+    if ((rc = tk_select_pairs_d(ctx, dl, dl + j.n_pairs, j.n_pairs, o.max_cov, 1, o.max_cv, max_nwin,
+                                (uint8_t*)d_sel, &nsel)))
+      break;
+    if (j.selected && (rc = tk_memcpy_d2h(ctx, j.selected, d_sel, j.n_pairs))) break;
+    j.res.n_selected = nsel;
+  } while (0);
+  tk_buffer_free(ctx, d_sel);
+  tk_buffer_free(ctx, d_stats);
+  tk_table_free(table);
+  tk_reads_free(reads);
+  if (table_bytes) release(s, table_bytes);
+  release(s, read_bytes);
+  const auto t2 = Clock::now();
+  j.res.run_s = secs(t1, t2);
+  j.res.latency_s = secs(j.t_submit, t2);
+  (void)t0;
+  return rc;
+}
+
+void worker(tk_server* s, int idx) {
+  tk_ctx* ctx = nullptr;
+  int rc = tk_init(s->opts.device, &ctx);
+  if (rc) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->init_error = rc;
+    s->init_msg = tk_last_error();
+  }
+  (void)idx;
+  while (true) {
+    std::shared_ptr<Job> j;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_queue.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+      if (s->queue.empty()) break;  // stopping
+      j = s->queue.front();
+      s->queue.pop_front();
+    }
+    if (!ctx) {
+      j->res.status = rc;
+    } else {
+      j->res.status = run_job(s, ctx, *j);
+      if (j->res.status) {
+        const char* m = tk_last_error();
+        strncpy(j->res.message, m, sizeof(j->res.message) - 1);
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      j->done = true;
+    }
+    s->cv_done.notify_all();
+  }
+  if (ctx) tk_ctx_free(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tk_server_create(const tk_server_opts* o, tk_server** out) {
+  if (!o || !out) return -1;
+  *out = nullptr;
+  if (o->slots < 1 || o->slots > 16 || o->k < 1 || o->k > 31 || !(o->max_load > 0 && o->max_load <= 0.9) ||
+      o->max_cov <= 0)
+    return TK_ERR_ARG;
+  tk_server* s = new tk_server();
+  s->opts = *o;
+  s->budget = o->hbm_budget_bytes ? o->hbm_budget_bytes : (64ull << 30);
+  for (int i = 0; i < o->slots; i++) s->workers.emplace_back(worker, s, i);
+  *out = s;
+  return TK_OK;
+}
+
+int tk_server_submit(tk_server* s, const char* seq, const uint64_t* offsets, uint64_t n_pairs,
+                     uint8_t* selected, uint64_t* job_id) {
+  if (!s || !offsets || !job_id || (n_pairs && !seq)) return TK_ERR_ARG;
+  auto j = std::make_shared<Job>();
+  j->seq = seq;
+  j->offsets = offsets;
+  j->n_pairs = n_pairs;
+  j->selected = selected;
+  j->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return TK_ERR_STATE;
+    j->id = s->next_id++;
+    s->jobs[j->id] = j;
+    s->queue.push_back(j);
+  }
+  s->cv_queue.notify_one();
+  *job_id = j->id;
+  return TK_OK;
+}
+
+int tk_server_wait(tk_server* s, uint64_t job_id, tk_job_result* res) {
+  if (!s || !res) return TK_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return TK_ERR_ARG;
+  std::shared_ptr<Job> j = it->second;
+  s->cv_done.wait(lk, [&] { return j->done; });
+  *res = j->res;
+  s->jobs.erase(job_id);
+  return TK_OK;
+}
+
+int tk_server_free(tk_server* s) {
+  if (!s) return TK_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_queue.notify_all();
+  s->cv_mem.notify_all();
+  for (auto& t : s->workers) t.join();
+  delete s;
+  return TK_OK;
+}
+
+}  // extern "C"
