@@ -62,6 +62,178 @@ def cpu_baseline(tx, n_pairs_sample: int, max_cov: int) -> dict:
             "count_s": t1 - t0, "stats_s": t2 - t1}
 
 
+def make_components(n_comp: int, seed: int = 1003):
+    """Chrysalis-style components: reads per component ~ power law (alpha 1.8)
+    on [10, 200k]; 1-20 transcripts of 300-5000 nt; 150-nt reads at 1%
+    error; contigs = 1-3 pieces of each transcript (at most 50)."""
+    from trinityrnaseq_amd import synth
+    rng = np.random.default_rng(seed)
+    a = 1.8
+    lo, hi = 10.0, 200_000.0
+    u = rng.random(n_comp)
+    sizes = (lo ** (1 - a) + u * (hi ** (1 - a) - lo ** (1 - a))) ** (1 / (1 - a))
+    sizes = np.maximum(10, sizes.astype(np.int64))
+    rbuf, cparts, rcount, ccount = [], [], [], []
+    for c in range(n_comp):
+        ntx = int(rng.integers(1, 21))
+        tx = synth.Transcriptome(seed * 100_003 + c, ntx, 0, 300, 5000)
+        s, _ = synth.se_reads(tx, int(sizes[c]), READ_LEN, seed=seed * 7 + c, sub_rate=0.01)
+        rbuf.append(s)
+        rcount.append(int(sizes[c]))
+        contigs = []
+        for j in range(ntx):
+            t = tx.seq(j)
+            cuts = sorted(rng.integers(50, max(51, len(t) - 50), size=int(rng.integers(0, 3))))
+            b0 = 0
+            for x in list(cuts) + [len(t)]:
+                if x - b0 >= 30:
+                    contigs.append(t[b0:x])
+                b0 = x
+        contigs = contigs[:50] or [tx.seq(0)]
+        cparts.append(contigs)
+        ccount.append(len(contigs))
+    reads = np.concatenate(rbuf)
+    roff = np.arange(int(sum(rcount)) + 1, dtype=np.uint64) * np.uint64(READ_LEN)
+    cseq = "".join(x for cs in cparts for x in cs).encode()
+    clen = [len(x) for cs in cparts for x in cs]
+    coff = np.zeros(len(clen) + 1, np.uint64)
+    coff[1:] = np.cumsum(clen)
+    rco = np.zeros(n_comp + 1, np.uint64)
+    rco[1:] = np.cumsum(rcount)
+    cco = np.zeros(n_comp + 1, np.uint64)
+    cco[1:] = np.cumsum(ccount)
+    return reads, roff, rco, cseq, coff, cco
+
+
+def bench_components(args) -> dict:
+    from trinityrnaseq_amd import api
+    t = time.perf_counter()
+    reads, roff, rco, cseq, coff, cco = make_components(args.components)
+    n_reads = len(roff) - 1
+    log(f"generated {args.components} components, {n_reads} reads, {len(coff) - 1} contigs "
+        f"in {time.perf_counter() - t:.1f}s")
+    ctx = api.Context(0)
+    r = api.Reads(ctx, reads.tobytes(), roff)
+    cg = api.Reads(ctx, cseq, coff)
+    wins = r.windows(K)
+    b = api.Batch(ctx, r, rco, cg, cco, use_graph=True)
+    d_st = api.DeviceBuffer(ctx, n_reads * 32)
+    d_vo = api.DeviceBuffer(ctx, n_reads * 12)
+    tuned_ms = b.autotune(d_st.ptr, d_vo.ptr, min_count=1)
+    info = b.info()
+    for _ in range(args.warmup):
+        b.run_d(d_st.ptr, d_vo.ptr)
+    ctx.sync()
+    lat = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        b.run_d(d_st.ptr, d_vo.ptr)
+        ctx.sync()
+        lat.append(time.perf_counter() - t)
+    batched_s = sum(lat) / len(lat)
+    # plain form: one job per component (the ParaFly fan-out), on a prefix
+    n_plain = min(args.components, args.plain_components)
+    t = time.perf_counter()
+    w_plain = 0
+    for c in range(n_plain):
+        a, z = int(rco[c]), int(rco[c + 1])
+        rr = api.Reads(ctx, reads[a * READ_LEN:z * READ_LEN].tobytes(), roff[a:z + 1] - roff[a])
+        w_plain += rr.windows(K)
+        tb = api.Table(ctx, K, True, int(api.estimate_distinct(ctx, rr, K, True) * 1.05) + 64)
+        tb.count(rr)
+        api.read_cov_stats(ctx, tb, rr)
+        ca, cz = int(cco[c]), int(cco[c + 1])
+        cr = api.Reads(ctx, cseq[int(coff[ca]):int(coff[cz])], coff[ca:cz + 1] - coff[ca])
+        ct = api.CTable(ctx, cr, np.arange(cz - ca, dtype=np.uint32), K, True)
+        ct.vote(rr)
+        tb.free(); ct.free(); rr.free(); cr.free()
+    plain_s = time.perf_counter() - t
+    plain_rate = w_plain / plain_s
+    out = {
+        "metric": METRIC + " [batched components]", "value": wins / batched_s, "unit": "k-mers/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batched_s * 1000, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded components generated in-process)",
+        "config": {"workload": f"components: {args.components} Chrysalis-style components (power law "
+                               f"alpha=1.8 on [10,200k] reads), count + coverage stats + R2T vote per "
+                               f"component, one graph replay", "reads": n_reads, "windows": wins,
+                   "form": "batched (hipGraph)", "autotuned": info},
+        "reads_per_s": n_reads / batched_s,
+        "plain": {"components": n_plain, "k-mers/s": plain_rate, "seconds": plain_s,
+                  "note": "one job per component through the per-sample entry points"},
+        "batched_over_plain": (wins / batched_s) / plain_rate,
+        "autotune_best_ms": tuned_ms,
+    }
+    for x in (d_st, d_vo):
+        x.free()
+    b.free()
+    return out
+
+
+def bench_serving(args) -> dict:
+    from trinityrnaseq_amd import api, synth
+    rng = np.random.default_rng(2000)
+    lo, hi = 0.5e6 * args.serve_scale, 30e6 * args.serve_scale
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), size=args.samples)).astype(np.int64)
+    t = time.perf_counter()
+    samples = []
+    for j, n in enumerate(sizes):
+        tx = synth.large_transcriptome(2000 + j, 20_000)
+        buf = np.empty(2 * int(n) * READ_LEN, np.uint8)
+        synth.pe_reads(tx, int(n), READ_LEN, seed=2000 + j, out=buf)
+        off = np.arange(2 * int(n) + 1, dtype=np.uint64) * np.uint64(READ_LEN)
+        samples.append((buf, off, int(n)))
+    total_pairs = int(sizes.sum())
+    log(f"generated {args.samples} samples, {total_pairs} pairs in {time.perf_counter() - t:.1f}s")
+    # serial plain GPU form: one job at a time
+    srv1 = api.Server(slots=1, max_cov=args.max_cov)
+    t = time.perf_counter()
+    serial_lat = []
+    for b, o, n in samples:
+        jid = srv1.submit(b, o, n)
+        res, _ = srv1.wait(jid)
+        serial_lat.append(res.latency_s)
+    serial_s = time.perf_counter() - t
+    srv1.close()
+    # served: burst for capacity, then Poisson arrivals at 0.8 x capacity
+    srv = api.Server(slots=args.slots, max_cov=args.max_cov, hbm_budget_bytes=int(args.hbm_budget_gb * 2**30))
+    t = time.perf_counter()
+    ids = [srv.submit(b, o, n) for b, o, n in samples]
+    for jid in ids:
+        srv.wait(jid)
+    burst_s = time.perf_counter() - t
+    lam = 0.8 * args.samples / burst_s
+    gaps = rng.exponential(1.0 / lam, size=args.samples)
+    t0 = time.perf_counter()
+    ids = []
+    for (b, o, n), g in zip(samples, gaps):
+        time.sleep(g)
+        ids.append(srv.submit(b, o, n))
+    lat = []
+    for jid in ids:
+        res, _ = srv.wait(jid)
+        lat.append(res.latency_s)
+    poisson_s = time.perf_counter() - t0
+    srv.close()
+    lat = sorted(lat)
+    q = lambda v, x: v[min(len(v) - 1, int(round(x * (len(v) - 1))))]
+    return {
+        "metric": METRIC + " [served stream]", "value": 2 * total_pairs / burst_s, "unit": "reads/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(burst_s * 1000, 3),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded samples generated in-process)",
+        "config": {"workload": f"serving: {args.samples} samples, read pairs log-uniform "
+                               f"[{lo:.0f}, {hi:.0f}] (scale {args.serve_scale} of 0.5M-30M), 2x150 PE, "
+                               f"count + coverage stats + max_cov={args.max_cov} selection per sample",
+                   "slots": args.slots, "hbm_budget_gb": args.hbm_budget_gb, "total_pairs": total_pairs},
+        "serial_plain_reads_per_s": 2 * total_pairs / serial_s,
+        "served_over_serial": serial_s / burst_s,
+        "poisson": {"lambda_per_s": lam, "p50_s": q(lat, 0.5), "p99_s": q(lat, 0.99),
+                    "reads_per_s": 2 * total_pairs / poisson_s},
+        "serial_latency_p50_s": q(sorted(serial_lat), 0.5), "serial_latency_p99_s": q(sorted(serial_lat), 0.99),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,11 +245,24 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-ref-gb", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GB (HBM counter reference)")
+    ap.add_argument("--workload", choices=["large_pe", "components", "serving"], default="large_pe")
+    ap.add_argument("--components", type=int, default=10_000)
+    ap.add_argument("--plain-components", type=int, default=1000)
+    ap.add_argument("--samples", type=int, default=64)
+    ap.add_argument("--serve-scale", type=float, default=0.1)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--hbm-budget-gb", type=float, default=96.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.workload == "components":
+        print(json.dumps(bench_components(args)), flush=True)
+        return 0
+    if args.workload == "serving":
+        print(json.dumps(bench_serving(args)), flush=True)
+        return 0
 
     from trinityrnaseq_amd import api, synth
 

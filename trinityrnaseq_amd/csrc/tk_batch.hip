@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_batch_stats(ReadsView rv, const uint32_
   const int wv = threadIdx.x >> 6;
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* vals = s_dyn + (uint64_t)wv * slab;
-  uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * 64;
+  uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * kLookupScratch;
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads; r += (uint64_t)gridDim.x * wpb) {
     const TableView t = comp_view(base, lay[read_comp[r]]);
     stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
@@ -332,11 +332,11 @@ static int batch_enqueue(tk_ctx* c, tk_batch* b, const tk_stats_opts* o, tk_read
   }
   uint32_t slab = std::max<uint32_t>(64, (b->max_nwin + 63) / 64 * 64);
   int wpb = b->stats_waves;
-  while (wpb > 1 && ((uint64_t)slab * 4 + 512) * wpb > 65536) wpb >>= 1;
+  while (wpb > 1 && ((uint64_t)slab * 4 + 8 * kLookupScratch) * wpb > 65536) wpb >>= 1;
   if (d_stats && rv.n_reads) {
     StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
     hipLaunchKernelGGL(k_batch_stats, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb),
-                       (size_t)slab * 4 * wpb + (size_t)wpb * 512, c->stream, rv, b->d_read_comp, b->d_lay,
+                       (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, c->stream, rv, b->d_read_comp, b->d_lay,
                        b->k, b->canonical, base, sp, d_stats, slab);
   }
   if (d_votes && rv.n_reads) {
@@ -450,9 +450,9 @@ int tk_batch_create(tk_ctx* c, int k, int canonical, const tk_reads* reads, cons
       b->comp_cwindows[i] += cl[r] >= (uint32_t)k ? cl[r] - k + 1 : 0;
     }
   }
-  if ((uint64_t)(std::max<uint32_t>(64, (b->max_nwin + 63) / 64 * 64)) * 4 + 512 > 65536) {
+  if ((uint64_t)(std::max<uint32_t>(64, (b->max_nwin + 63) / 64 * 64)) * 4 + 8 * kLookupScratch > 65536) {
     delete b;
-    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16256 + k - 1);
+    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16128 + k - 1);
   }
   rc = dev_alloc((void**)&b->d_read_comp, std::max<uint64_t>(reads->n_reads, 1) * 4);
   if (!rc) rc = dev_alloc((void**)&b->d_contig_comp, std::max<uint64_t>(contigs->n_reads, 1) * 4);

@@ -31,6 +31,8 @@ namespace tk {
 
 constexpr int kWave = 64;
 constexpr int kBucketSlots = 8;
+// u64 entries of LDS lookup scratch per wavefront (coop_get128)
+constexpr int kLookupScratch = 128;
 
 __host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   x ^= x >> 33;
@@ -303,6 +305,51 @@ __device__ __forceinline__ uint64_t coop_get64(const TableView& t, uint64_t key,
   }
   wave_sync();
   return f;
+}
+
+// coop_get64 for two keys per lane (128 windows): all sixteen bucket loads
+// are issued before any is resolved, so a 150-nt read costs one memory round
+// trip instead of two.  tmp: 128 x u64 LDS scratch.
+__device__ __forceinline__ void coop_get128(const TableView& t, uint64_t keyA, bool wantA,
+                                            uint64_t keyB, bool wantB, uint64_t* tmp, int lane,
+                                            uint64_t& cA, uint64_t& cB) {
+  const uint64_t tagA = keyA | t.occ, tagB = keyB | t.occ;
+  const uint64_t bA = bucket0(t, keyA), bB = bucket0(t, keyB);
+  const int g = lane >> 3, sl = lane & 7;
+  tmp[lane] = 0;
+  tmp[64 + lane] = 0;
+  uint64_t sa[8], sb[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    sa[q] = t.slots[(__shfl(bA, q * 8 + g) << 3) + sl];
+    sb[q] = t.slots[(__shfl(bB, q * 8 + g) << 3) + sl];
+  }
+  wave_sync();
+  uint64_t pendA = 0, pendB = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint64_t ta = __shfl(tagA, q * 8 + g);
+    const uint64_t tb = __shfl(tagB, q * 8 + g);
+    const bool ma = (sa[q] & t.kmo) == ta;
+    const bool mb = (sb[q] & t.kmo) == tb;
+    if (ma) tmp[q * 8 + g] = sa[q] >> t.cshift;
+    if (mb) tmp[64 + q * 8 + g] = sb[q] >> t.cshift;
+    pendA |= (uint64_t)(~byte_any8(__ballot(ma)) & ~byte_any8(__ballot(sa[q] == 0)) & 0xFFu) << (8 * q);
+    pendB |= (uint64_t)(~byte_any8(__ballot(mb)) & ~byte_any8(__ballot(sb[q] == 0)) & 0xFFu) << (8 * q);
+  }
+  wave_sync();
+  uint64_t fa = tmp[lane], fb = tmp[64 + lane];
+  if (wantA) {
+    if ((pendA >> lane) & 1ull) fa = table_get(t, keyA);
+    else if (fa == t.cmax) { const uint64_t* o = ovf_find(t, tagA); fa = o ? *o : fa; }
+  }
+  if (wantB) {
+    if ((pendB >> lane) & 1ull) fb = table_get(t, keyB);
+    else if (fb == t.cmax) { const uint64_t* o = ovf_find(t, tagB); fb = o ? *o : fb; }
+  }
+  wave_sync();
+  cA = fa;
+  cB = fb;
 }
 
 // Cooperative +1 of the 64 keys held one per lane (want = lane has a key).

@@ -218,32 +218,45 @@ __device__ __forceinline__ uint32_t base_code(uint8_t ch, uint32_t& bad) {
   return c;
 }
 
+// blk_read[b] = read holding word 256*b (thread per read; reads are
+// ordered, so every 256-word block start lies in exactly one read).
+__global__ void k_block_starts(const uint64_t* __restrict__ word_off, uint64_t n,
+                               uint64_t* __restrict__ blk_read) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t a = word_off[r], e = word_off[r + 1];
+    for (uint64_t w = (a + 255) & ~255ull; w < e; w += 256) blk_read[w >> 8] = r;
+  }
+}
+
 // One thread per output word: 32 bases -> 64-bit code word + 32-bit mask.
 // A block's 256 words cover at most 8 KiB of contiguous ASCII, which the
-// block first stages in LDS with coalesced aligned 4-byte loads.
+// block first stages in LDS with coalesced aligned 4-byte loads; the word
+// offsets of its reads are staged too, so each thread finds its read with a
+// short LDS binary search.
+constexpr int kPackReadsLds = 512;
+
 __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
                                               const uint64_t* __restrict__ off,
                                               const uint64_t* __restrict__ word_off,
                                               const uint32_t* __restrict__ len,
+                                              const uint64_t* __restrict__ blk_read,
                                               uint64_t n_reads, uint64_t n_words, uint64_t n_bytes,
                                               uint64_t* __restrict__ codes,
                                               uint32_t* __restrict__ nmask) {
-  __shared__ uint64_t s_range[2];
   __shared__ uint32_t s_txt[2056];
+  __shared__ uint64_t s_woff[kPackReadsLds + 1];
   const uint64_t w0 = (uint64_t)blockIdx.x * blockDim.x;
   if (w0 >= n_words) return;
   const uint64_t wl = std::min<uint64_t>(w0 + blockDim.x, n_words) - 1;
-  if (threadIdx.x < 2) {
-    const uint64_t target = threadIdx.x == 0 ? w0 : wl;
-    uint64_t lo = 0, hi = n_reads;  // largest r with word_off[r] <= target
-    while (hi - lo > 1) {
-      uint64_t mid = (lo + hi) >> 1;
-      if (word_off[mid] <= target) lo = mid; else hi = mid;
-    }
-    s_range[threadIdx.x] = lo;
-  }
-  __syncthreads();
-  const uint64_t rf = s_range[0], rl = s_range[1];
+  // first read of this block and of the next bound the reads of words w0..wl
+  const uint64_t rf = blk_read[blockIdx.x];
+  const uint64_t rnext = (wl + 1 < n_words) ? blk_read[blockIdx.x + 1] : n_reads - 1;
+  uint64_t rl = rnext;
+  while (rl > rf && word_off[rl] > wl) rl--;  // exact read of word wl (skips empty reads)
+  const bool lds = rl - rf + 2 <= kPackReadsLds + 1;
+  if (lds)
+    for (uint64_t i = threadIdx.x; i < rl - rf + 2; i += blockDim.x) s_woff[i] = word_off[rf + i];
   const uint64_t b_lo = off[rf] + (w0 - word_off[rf]) * 32;
   const uint64_t b_hi = std::min<uint64_t>(off[rl] + (wl - word_off[rl]) * 32 + 32, off[rl + 1]);
   // a0: b_lo rounded down to a 4-byte aligned address (may precede seq)
@@ -264,13 +277,14 @@ __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
   __syncthreads();
   const uint64_t w = w0 + threadIdx.x;
   if (w > wl) return;
-  uint64_t lo = rf, hi = rl + 1;
+  uint64_t lo = rf, hi = rl + 1;  // largest r in [rf, rl] with word_off[r] <= w
   while (hi - lo > 1) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (word_off[mid] <= w) lo = mid; else hi = mid;
+    const uint64_t mid = (lo + hi) >> 1;
+    const uint64_t wm = lds ? s_woff[mid - rf] : word_off[mid];
+    if (wm <= w) lo = mid; else hi = mid;
   }
   const uint64_t r = lo;
-  const uint64_t j = w - word_off[r];
+  const uint64_t j = w - (lds ? s_woff[r - rf] : word_off[r]);
   const uint32_t L = len[r];
   const uint64_t b0 = j * 32;
   const uint8_t* src = reinterpret_cast<const uint8_t*>(s_txt) + ((int64_t)(off[r] + b0) - a0);
@@ -337,11 +351,17 @@ static int reads_build(tk_ctx* ctx, const char* d_seq, const uint64_t* d_off, ui
     if (e == hipSuccess) e = hipMemsetAsync(r->d_nmask + nwords, 0xFF, 4, ctx->stream);
     if (e != hipSuccess) { rc = hip_fail(e, "pad"); break; }
     if (nwords) {
+      const uint64_t nblk = ceil_div(nwords, 256);
+      void* bp = nullptr;
+      if ((rc = ensure_scratch(ctx, 1, nblk * 8, &bp))) break;
+      uint64_t* blk_read = (uint64_t*)bp;
       hipEvent_t a;
       timing_begin(ctx, &a);
-      hipLaunchKernelGGL(k_pack, dim3(ceil_div(nwords, 256)), dim3(256), 0, ctx->stream,
-                         d_seq, d_off, r->d_word_off, r->d_len, n, nwords, n_bases, r->d_codes,
-                         r->d_nmask);
+      hipLaunchKernelGGL(k_block_starts, dim3(std::min<uint64_t>(ceil_div(n, 256), 65536)), dim3(256), 0,
+                         ctx->stream, r->d_word_off, n, blk_read);
+      hipLaunchKernelGGL(k_pack, dim3(nblk), dim3(256), 0, ctx->stream,
+                         d_seq, d_off, r->d_word_off, r->d_len, blk_read, n, nwords, n_bases,
+                         r->d_codes, r->d_nmask);
       timing_end(ctx, "pack", a);
       if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_pack"); break; }
     }
@@ -384,36 +404,58 @@ constexpr int kHllRegs = 1 << kHllBits;
 
 // Windows whose hash has its top `sbits` bits zero (a 2^-sbits sample chosen
 // by key, so every occurrence of a sampled key is sampled) update the
-// registers; the estimate is scaled by 2^sbits.
+// registers; the estimate is scaled by 2^sbits.  One thread per window of a
+// block of 64 consecutive reads (as in k_count), block-private registers in
+// LDS merged with one atomicMax per register.
 __global__ __launch_bounds__(256) void k_hll(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
                                              int canonical, uint32_t sbits,
                                              uint32_t* __restrict__ regs) {
   __shared__ uint32_t s_reg[kHllRegs];
-  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) s_reg[i] = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const uint64_t wpb = blockDim.x >> 6;
-  for (uint64_t r = r_begin + (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); r < r_end;
-       r += (uint64_t)gridDim.x * wpb) {
-    const uint32_t L = rv.len[r];
-    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
-    const uint64_t wb = rv.word_off[r];
-    for (uint32_t p = lane; p < nwin; p += 64) {
+  __shared__ uint32_t s_off[65];
+  __shared__ uint64_t s_wb[64];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kHllRegs; i += blockDim.x) s_reg[i] = 0;
+  for (uint64_t r0 = r_begin + (uint64_t)blockIdx.x * 64; r0 < r_end; r0 += (uint64_t)gridDim.x * 64) {
+    const int nr = (int)std::min<uint64_t>(64, r_end - r0);
+    __syncthreads();
+    if (tid < 64) {
+      uint32_t nw = 0;
+      if (tid < nr) {
+        const uint32_t L = rv.len[r0 + tid];
+        nw = L >= (uint32_t)k ? L - k + 1 : 0;
+        s_wb[tid] = rv.word_off[r0 + tid];
+      }
+      uint32_t x = nw;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (tid >= o) x += y;
+      }
+      s_off[tid + 1] = x;
+      if (tid == 0) s_off[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t total = s_off[nr];
+    for (uint32_t w = tid; w < total; w += blockDim.x) {
+      int lo = 0, hi = nr;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_off[mid] <= w) lo = mid; else hi = mid;
+      }
       uint64_t fwd;
       bool valid;
-      window_at(rv, wb, p, k, fwd, valid);
+      window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
       if (!valid) continue;
       const uint64_t h0 = mix64(canon_key(fwd, k, canonical) ^ 0x5bd1e9955bd1e995ULL);
       if (sbits && (h0 >> (64 - sbits))) continue;
       const uint64_t h = h0 << sbits;
       const uint32_t idx = (uint32_t)(h >> (64 - kHllBits));
       const uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1 + sbits));
-      const uint32_t rho = (uint32_t)__clzll(rest) + 1;
-      atomicMax(&s_reg[idx], rho);
+      atomicMax(&s_reg[idx], (uint32_t)__clzll(rest) + 1);
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x)
+  for (int i = tid; i < kHllRegs; i += blockDim.x)
     if (s_reg[i]) atomicMax(&regs[i], s_reg[i]);
 }
 
@@ -968,7 +1010,7 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
   const int wv = threadIdx.x >> 6;
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* vals = s_dyn + (uint64_t)wv * slab;
-  uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * 64;
+  uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * kLookupScratch;
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
        r += (uint64_t)gridDim.x * wpb)
     stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
@@ -1095,9 +1137,9 @@ static int grid_for_waves(tk_ctx* ctx, uint64_t n_items, int waves_per_block) {
 // 64 x u64 lookup scratch; at most 64 KiB per block.  Returns waves per block.
 static int slab_layout(uint32_t max_nwin, uint32_t* slab, int* wpb) {
   uint32_t s = std::max<uint32_t>(64, (max_nwin + 63) / 64 * 64);
-  if ((uint64_t)s * 4 + 512 > 65536) return -1;
+  if ((uint64_t)s * 4 + 8 * kLookupScratch > 65536) return -1;
   int w = 4;
-  while (w > 1 && ((uint64_t)s * 4 + 512) * w > 65536) w >>= 1;
+  while (w > 1 && ((uint64_t)s * 4 + 8 * kLookupScratch) * w > 65536) w >>= 1;
   *slab = s;
   *wpb = w;
   return 0;
@@ -1372,7 +1414,7 @@ static int hll_estimate(tk_ctx* c, const tk_reads* r, uint64_t rb, uint64_t re, 
   const uint32_t sbits = windows >= (64ull << 20) ? 4 : 0;
   TK_HIP(hipMemsetAsync(regs, 0, kHllRegs * 4, c->stream));
   if (re > rb) {
-    int grid = grid_for_waves(c, re - rb, 4);
+    const uint64_t grid = std::min<uint64_t>(ceil_div(re - rb, 64), (uint64_t)std::max(c->num_cu, 1) * 16);
     TK_LAUNCH(c, "hll", grid, 256, 0, k_hll, reads_view(r), rb, re, k, canonical, sbits, regs);
   }
   std::vector<uint32_t> h(kHllRegs);
@@ -1873,12 +1915,12 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
   uint32_t slab = 0;
   int wpb = 0;
   if (slab_layout(max_nwin, &slab, &wpb))
-    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16256 + t->k - 1);
+    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16128 + t->k - 1);
   tk_table* use = t;
   if ((rc = scoring_table(c, t, o->min_count, &use))) return rc;
   StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
   int grid = grid_for_waves(c, r->n_reads, wpb);
-  TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 64 * 8, k_stats,
+  TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, k_stats,
             reads_view(r), t->k,
             t->canonical, table_view(use), sp, d_out, slab);
   return TK_OK;
@@ -2039,7 +2081,7 @@ int tk_r2t_vote_reads_d(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t
   uint32_t slab = 0;
   int wpb = 0;
   if (slab_layout(max_nwin, &slab, &wpb))
-    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16256 + t->k - 1);
+    return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16128 + t->k - 1);
   int grid = grid_for_waves(c, r->n_reads, wpb);
   TK_LAUNCH(c, "vote", grid, wpb * 64, (size_t)slab * 4 * wpb, k_vote, reads_view(r), t->k,
             t->canonical, t->d_slots, t->nbuckets - 1, d_out, slab);

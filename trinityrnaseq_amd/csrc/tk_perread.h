@@ -35,7 +35,7 @@ __device__ inline uint32_t wave_select(const uint32_t* vals, uint32_t n, uint32_
 
 // fastaToKmerCoverageStats for read r, one wavefront: window values staged
 // in `vals` (LDS, >= nwin entries), radix-select median, exact integer sums.
-// tmp: 64 x u64 LDS scratch of the cooperative lookup.  Lane 0 writes *out.
+// tmp: 128 x u64 LDS scratch of the cooperative lookup.  Lane 0 writes *out.
 __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int canonical,
                                   const TableView& t, const StatsParams& sp, uint32_t* vals,
                                   uint64_t* tmp, int lane, tk_read_stat* out) {
@@ -44,35 +44,46 @@ __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int ca
     const uint64_t wb = rv.word_off[r];
     uint32_t n = 0, vmax = 0;
     uint64_t sum = 0, sqlo = 0, sqhi = 0;
-    for (uint32_t base = 0; base < nwin; base += 64) {
-      const uint32_t p = base + lane;
-      bool has = false;
-      uint32_t v = 0;
-      uint64_t fwd = 0;
-      bool valid = false;
-      if (p < nwin) window_at(rv, wb, p, k, fwd, valid);
-      const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
-      const uint64_t c = coop_get64(t, key, valid, tmp, lane);
-      if (p < nwin) {
-        if (valid) {
-          v = (c != 0 && c >= sp.min_count) ? (c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c)
-                                            : sp.missing;
-          has = true;
-        } else if (sp.nwin_policy == TK_NWIN_AS_MISSING) {
-          v = sp.missing;
-          has = true;
+    // windows are taken 128 at a time (two per lane), values appended to
+    // vals in window order
+    for (uint32_t base = 0; base < nwin; base += 128) {
+      const uint32_t pa = base + lane, pb = base + 64 + lane;
+      uint64_t fa = 0, fb = 0;
+      bool va = false, vb = false;
+      if (pa < nwin) window_at(rv, wb, pa, k, fa, va);
+      if (pb < nwin) window_at(rv, wb, pb, k, fb, vb);
+      const uint64_t ka = va ? canon_key(fa, k, canonical) : 0;
+      const uint64_t kb = vb ? canon_key(fb, k, canonical) : 0;
+      uint64_t ca, cb;
+      coop_get128(t, ka, va, kb, vb, tmp, lane, ca, cb);
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        const uint32_t p = half ? pb : pa;
+        const bool valid = half ? vb : va;
+        const uint64_t c = half ? cb : ca;
+        bool has = false;
+        uint32_t v = 0;
+        if (p < nwin) {
+          if (valid) {
+            v = (c != 0 && c >= sp.min_count) ? (c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c)
+                                              : sp.missing;
+            has = true;
+          } else if (sp.nwin_policy == TK_NWIN_AS_MISSING) {
+            v = sp.missing;
+            has = true;
+          }
         }
+        const uint64_t m = __ballot(has);
+        if (has) {
+          vals[n + __popcll(m & ((1ull << lane) - 1))] = v;
+          sum += v;
+          const uint64_t sq = (uint64_t)v * v;
+          sqlo += sq;
+          sqhi += (sqlo < sq);
+          vmax = v > vmax ? v : vmax;
+        }
+        n += (uint32_t)__popcll(m);
       }
-      const uint64_t m = __ballot(has);
-      if (has) {
-        vals[n + __popcll(m & ((1ull << lane) - 1))] = v;
-        sum += v;
-        const uint64_t sq = (uint64_t)v * v;
-        sqlo += sq;
-        sqhi += (sqlo < sq);
-        vmax = v > vmax ? v : vmax;
-      }
-      n += (uint32_t)__popcll(m);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

@@ -57,6 +57,8 @@ struct tk_server {
   std::map<uint64_t, std::shared_ptr<Job>> jobs;
   uint64_t next_id = 1;
   uint64_t budget = 0, in_use = 0;
+  int holders = 0;        // jobs holding a stage-1 reservation
+  int stage2_waiting = 0; // of those, jobs waiting for their table reservation
   bool stop = false;
   int init_error = 0;
   std::string init_msg;
@@ -64,18 +66,28 @@ struct tk_server {
 
 namespace {
 
-// Wait until `bytes` fit the budget (a job larger than the whole budget runs
-// alone), then reserve them.
-void reserve(tk_server* s, uint64_t bytes) {
+// Wait until `bytes` fit the budget, then reserve them.  Stage 1 (reads)
+// admits a job; stage 2 (its table) is reserved while the job holds stage 1.
+// A job larger than the whole budget runs alone; when every job holding
+// memory is waiting for its stage-2 bytes, one of them goes ahead over
+// budget, so the holders can never wait on each other forever.
+void reserve(tk_server* s, uint64_t bytes, bool stage2) {
   std::unique_lock<std::mutex> lk(s->mu);
-  s->cv_mem.wait(lk, [&] { return s->in_use == 0 || s->in_use + bytes <= s->budget || s->stop; });
+  if (stage2) s->stage2_waiting++;
+  s->cv_mem.wait(lk, [&] {
+    return s->stop || s->in_use == 0 || s->in_use + bytes <= s->budget ||
+           (stage2 && s->stage2_waiting == s->holders);
+  });
+  if (stage2) s->stage2_waiting--;
+  else s->holders++;
   s->in_use += bytes;
 }
 
-void release(tk_server* s, uint64_t bytes) {
+void release(tk_server* s, uint64_t bytes, bool job_done) {
   {
     std::lock_guard<std::mutex> lk(s->mu);
     s->in_use -= bytes;
+    if (job_done) s->holders--;
   }
   s->cv_mem.notify_all();
 }
@@ -87,7 +99,7 @@ int run_job(tk_server* s, tk_ctx* ctx, Job& j) {
   // stage 1: reads (ASCII staging + packed codes + stats output)
   const uint64_t read_bytes = n_bases + n_bases * 3 / 8 + n_reads * (16 + sizeof(tk_read_stat)) + j.n_pairs;
   const auto t0 = Clock::now();
-  reserve(s, read_bytes);
+  reserve(s, read_bytes, false);
   const auto t1 = Clock::now();
   j.res.wait_s = secs(j.t_submit, t1);
   int rc = TK_OK;
@@ -102,9 +114,10 @@ int run_job(tk_server* s, tk_ctx* ctx, Job& j) {
     if ((rc = tk_estimate_distinct(ctx, reads, o.k, 1, &est))) break;
     j.res.distinct_estimate = est;
     const uint64_t expected = (uint64_t)(est * 1.05) + 4096;
-    // stage 2: the table (slots at load <= max_load, 8 bytes each)
-    table_bytes = (uint64_t)((double)expected / o.max_load * 8.0) + (64ull << 20);
-    reserve(s, table_bytes);
+    // stage 2: the table (8-byte slots at load <= max_load; the layout
+    // rounds up by at most 2x) plus its side table
+    table_bytes = (uint64_t)(2.0 * (double)expected / o.max_load * 8.0) + (1ull << 20);
+    reserve(s, table_bytes, true);
     if ((rc = tk_table_create(ctx, o.k, 1, expected, o.max_load, &table))) break;
     if ((rc = tk_count(ctx, table, reads))) break;
     if ((rc = tk_buffer_alloc(ctx, n_reads * sizeof(tk_read_stat), &d_stats))) break;
@@ -126,8 +139,8 @@ int run_job(tk_server* s, tk_ctx* ctx, Job& j) {
   tk_buffer_free(ctx, d_stats);
   tk_table_free(table);
   tk_reads_free(reads);
-  if (table_bytes) release(s, table_bytes);
-  release(s, read_bytes);
+  if (table_bytes) release(s, table_bytes, false);
+  release(s, read_bytes, true);
   const auto t2 = Clock::now();
   j.res.run_s = secs(t1, t2);
   j.res.latency_s = secs(j.t_submit, t2);
