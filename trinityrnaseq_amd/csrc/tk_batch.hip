@@ -192,9 +192,10 @@ __global__ __launch_bounds__(256) void k_batch_vote(ReadsView rv, const uint32_t
   const int wv = threadIdx.x >> 6;
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* hits = s_dyn + (uint64_t)wv * slab;
+  uint32_t* tmp = s_dyn + wpb * slab + wv * 64;
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads; r += (uint64_t)gridDim.x * wpb) {
     const CompLayout L = lay[read_comp[r]];
-    vote_read(rv, r, k, canonical, ct, L.ct_boff, L.ct_mask, hits, lane, out + r);
+    vote_read(rv, r, k, canonical, ct, L.ct_boff, L.ct_mask, hits, tmp, lane, out + r);
   }
 }
 
@@ -345,7 +346,7 @@ static int batch_enqueue(tk_ctx* c, tk_batch* b, const tk_stats_opts* o, tk_read
     if (cv.n_reads)
       hipLaunchKernelGGL(k_batch_ctable, dim3(grid(cv.n_reads, 4)), dim3(256), 0, c->stream, cv, b->d_contig_comp,
                          b->d_contig_local, b->d_lay, b->k, b->canonical, b->d_ct, c->d_flags);
-    hipLaunchKernelGGL(k_batch_vote, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb), (size_t)slab * 4 * wpb,
+    hipLaunchKernelGGL(k_batch_vote, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb), (size_t)slab * 4 * wpb + (size_t)wpb * 256,
                        c->stream, rv, b->d_read_comp, b->d_lay, b->k, b->canonical, b->d_ct, d_votes, slab);
   }
   hipError_t e = hipGetLastError();

@@ -404,47 +404,25 @@ constexpr int kHllRegs = 1 << kHllBits;
 
 // Windows whose hash has its top `sbits` bits zero (a 2^-sbits sample chosen
 // by key, so every occurrence of a sampled key is sampled) update the
-// registers; the estimate is scaled by 2^sbits.  One thread per window of a
-// block of 64 consecutive reads (as in k_count), block-private registers in
-// LDS merged with one atomicMax per register.
+// registers; the estimate is scaled by 2^sbits.  Wavefront per read,
+// block-private registers in LDS merged with one atomicMax per register.
 __global__ __launch_bounds__(256) void k_hll(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
                                              int canonical, uint32_t sbits,
                                              uint32_t* __restrict__ regs) {
   __shared__ uint32_t s_reg[kHllRegs];
-  __shared__ uint32_t s_off[65];
-  __shared__ uint64_t s_wb[64];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < kHllRegs; i += blockDim.x) s_reg[i] = 0;
-  for (uint64_t r0 = r_begin + (uint64_t)blockIdx.x * 64; r0 < r_end; r0 += (uint64_t)gridDim.x * 64) {
-    const int nr = (int)std::min<uint64_t>(64, r_end - r0);
-    __syncthreads();
-    if (tid < 64) {
-      uint32_t nw = 0;
-      if (tid < nr) {
-        const uint32_t L = rv.len[r0 + tid];
-        nw = L >= (uint32_t)k ? L - k + 1 : 0;
-        s_wb[tid] = rv.word_off[r0 + tid];
-      }
-      uint32_t x = nw;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (tid >= o) x += y;
-      }
-      s_off[tid + 1] = x;
-      if (tid == 0) s_off[0] = 0;
-    }
-    __syncthreads();
-    const uint32_t total = s_off[nr];
-    for (uint32_t w = tid; w < total; w += blockDim.x) {
-      int lo = 0, hi = nr;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_off[mid] <= w) lo = mid; else hi = mid;
-      }
+  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) s_reg[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint64_t wpb = blockDim.x >> 6;
+  for (uint64_t r = r_begin + (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); r < r_end;
+       r += (uint64_t)gridDim.x * wpb) {
+    const uint32_t L = rv.len[r];
+    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
+    const uint64_t wb = rv.word_off[r];
+    for (uint32_t p = lane; p < nwin; p += 64) {
       uint64_t fwd;
       bool valid;
-      window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
+      window_at(rv, wb, p, k, fwd, valid);
       if (!valid) continue;
       const uint64_t h0 = mix64(canon_key(fwd, k, canonical) ^ 0x5bd1e9955bd1e995ULL);
       if (sbits && (h0 >> (64 - sbits))) continue;
@@ -455,7 +433,7 @@ __global__ __launch_bounds__(256) void k_hll(ReadsView rv, uint64_t r_begin, uin
     }
   }
   __syncthreads();
-  for (int i = tid; i < kHllRegs; i += blockDim.x)
+  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x)
     if (s_reg[i]) atomicMax(&regs[i], s_reg[i]);
 }
 
@@ -1035,27 +1013,45 @@ __global__ void k_select_flags(const tk_read_stat* __restrict__ L, const tk_read
   }
 }
 
-__global__ void k_select(const tk_read_stat* __restrict__ L, const tk_read_stat* __restrict__ R,
-                         uint64_t n, const uint32_t* __restrict__ consumes,
-                         const uint32_t* __restrict__ idx, Lcg48Jump jump, uint64_t state0,
-                         int max_cov, uint8_t* __restrict__ sel,
-                         unsigned long long* __restrict__ nsel) {
+// drand48 state after `steps` draws from x: x <- A_j x + C_j for each set bit j.
+__device__ __forceinline__ uint64_t lcg48_jump(const Lcg48Jump& jump, uint64_t x, uint64_t steps) {
   const uint64_t M = (1ull << 48) - 1;
+  for (int j = 0; steps; j++, steps >>= 1)
+    if (steps & 1) x = (jump.a[j] * x + jump.c[j]) & M;
+  return x;
+}
+
+// One thread per pair.  The ranks `idx` of the rand() draws are ascending, so
+// each block jumps once to its first rank and every thread jumps the short
+// distance from there.
+__global__ __launch_bounds__(256) void k_select(const tk_read_stat* __restrict__ L,
+                                                const tk_read_stat* __restrict__ R, uint64_t n,
+                                                const uint32_t* __restrict__ consumes,
+                                                const uint32_t* __restrict__ idx, Lcg48Jump jump,
+                                                uint64_t state0, int max_cov, uint8_t* __restrict__ sel,
+                                                unsigned long long* __restrict__ nsel) {
+  __shared__ uint64_t s_base;
+  __shared__ uint32_t s_idx0;
   uint64_t cnt = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint8_t s = 0;
-    if (consumes[i]) {
-      uint64_t steps = (uint64_t)idx[i] + 1;
-      uint64_t x = state0;
-      for (int j = 0; steps; j++, steps >>= 1)
-        if (steps & 1) x = (jump.a[j] * x + jump.c[j]) & M;
-      const double r = (double)x * 0x1p-48;
-      const double med = ((double)L[i].median + (double)R[i].median) / 2.0;
-      s = (r <= (double)max_cov / med) ? 1 : 0;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += (uint64_t)gridDim.x * blockDim.x) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_idx0 = idx[i0];
+      s_base = lcg48_jump(jump, state0, idx[i0]);  // state before draw idx[i0]
     }
-    sel[i] = s;
-    cnt += s;
+    __syncthreads();
+    const uint64_t i = i0 + threadIdx.x;
+    if (i < n) {
+      uint8_t s = 0;
+      if (consumes[i]) {
+        const uint64_t x = lcg48_jump(jump, s_base, (uint64_t)(idx[i] - s_idx0) + 1);
+        const double r = (double)x * 0x1p-48;
+        const double med = ((double)L[i].median + (double)R[i].median) / 2.0;
+        s = (r <= (double)max_cov / med) ? 1 : 0;
+      }
+      sel[i] = s;
+      cnt += s;
+    }
   }
   cnt = wave_sum_u64(cnt);
   if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(nsel, (unsigned long long)cnt);
@@ -1122,9 +1118,10 @@ __global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical
   const int wv = threadIdx.x >> 6;
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* hits = s_dyn + (uint64_t)wv * slab;
+  uint32_t* tmp = s_dyn + wpb * slab + wv * 64;
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
        r += (uint64_t)gridDim.x * wpb)
-    vote_read(rv, r, k, canonical, slots, 0, bmask, hits, lane, out + r);
+    vote_read(rv, r, k, canonical, slots, 0, bmask, hits, tmp, lane, out + r);
 }
 
 static int grid_for_waves(tk_ctx* ctx, uint64_t n_items, int waves_per_block) {
@@ -1414,7 +1411,7 @@ static int hll_estimate(tk_ctx* c, const tk_reads* r, uint64_t rb, uint64_t re, 
   const uint32_t sbits = windows >= (64ull << 20) ? 4 : 0;
   TK_HIP(hipMemsetAsync(regs, 0, kHllRegs * 4, c->stream));
   if (re > rb) {
-    const uint64_t grid = std::min<uint64_t>(ceil_div(re - rb, 64), (uint64_t)std::max(c->num_cu, 1) * 16);
+    const int grid = grid_for_waves(c, re - rb, 4);
     TK_LAUNCH(c, "hll", grid, 256, 0, k_hll, reads_view(r), rb, re, k, canonical, sbits, regs);
   }
   std::vector<uint32_t> h(kHllRegs);
@@ -2083,7 +2080,7 @@ int tk_r2t_vote_reads_d(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t
   if (slab_layout(max_nwin, &slab, &wpb))
     return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16128 + t->k - 1);
   int grid = grid_for_waves(c, r->n_reads, wpb);
-  TK_LAUNCH(c, "vote", grid, wpb * 64, (size_t)slab * 4 * wpb, k_vote, reads_view(r), t->k,
+  TK_LAUNCH(c, "vote", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 256, k_vote, reads_view(r), t->k,
             t->canonical, t->d_slots, t->nbuckets - 1, d_out, slab);
   return TK_OK;
 }

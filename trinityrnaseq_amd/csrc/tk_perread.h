@@ -164,12 +164,59 @@ __device__ __forceinline__ uint32_t ctable_get(const uint64_t* slots, uint64_t b
   return 0xFFFFFFFFu;
 }
 
+// Bit g of the result is set when nibble g (lanes 4g..4g+3) of x is non-zero.
+__device__ __forceinline__ uint32_t nibble_any16(uint64_t x) {
+  x |= x >> 1;
+  x |= x >> 2;
+  x &= 0x1111111111111111ULL;
+  uint32_t m = 0;
+#pragma unroll
+  for (int g = 0; g < 16; g++) m |= (uint32_t)((x >> (4 * g)) & 1u) << g;
+  return m;
+}
+
+// Cooperative component-table lookup of the 64 keys held one per lane: in
+// round q the four lanes 4g..4g+3 read the four 16-byte slots of the bucket
+// of lane 16q+g's key (one 64-byte request per key, four loads in flight per
+// lane).  Returns the component of the calling lane's key (0xFFFFFFFF when
+// absent).  tmp: 64 x u32 LDS scratch.
+__device__ __forceinline__ uint32_t coop_cget64(const uint64_t* slots, uint64_t boff, uint64_t bmask,
+                                                uint64_t occ, uint64_t key, bool want, uint32_t* tmp,
+                                                int lane) {
+  const uint64_t tagged = key | occ;
+  const uint64_t b = boff + (mix64(key) & bmask);
+  const int g = lane >> 2, sl = lane & 3;
+  tmp[lane] = 0xFFFFFFFFu;
+  ulonglong2 sv[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint64_t bq = __shfl(b, q * 16 + g);
+    sv[q] = reinterpret_cast<const ulonglong2*>(slots + (bq << 3))[sl];
+  }
+  wave_sync();
+  uint64_t pending = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint64_t tq = __shfl(tagged, q * 16 + g);
+    const bool m = sv[q].x == tq;
+    if (m) tmp[q * 16 + g] = ~(uint32_t)sv[q].y;
+    const uint32_t gm = nibble_any16(__ballot(m));
+    const uint32_t ge = nibble_any16(__ballot(sv[q].x == 0));
+    pending |= (uint64_t)(~gm & ~ge & 0xFFFFu) << (16 * q);
+  }
+  wave_sync();
+  uint32_t c = tmp[lane];
+  if (want && ((pending >> lane) & 1ull)) c = ctable_get(slots, boff, bmask, occ, key);
+  wave_sync();
+  return want ? c : 0xFFFFFFFFu;
+}
+
 // ReadsToTranscripts vote of read r, one wavefront: component ids of the
 // hitting windows staged in `hits` (LDS), distinct ids walked in ascending
 // order so the first strictly largest count wins (ties -> smallest id).
 __device__ inline void vote_read(const ReadsView& rv, uint64_t r, int k, int canonical,
                                  const uint64_t* slots, uint64_t boff, uint64_t bmask,
-                                 uint32_t* hits, int lane, tk_r2t_vote* out) {
+                                 uint32_t* hits, uint32_t* tmp, int lane, tk_r2t_vote* out) {
   const uint64_t occ = 1ull << (2 * k);
     const uint32_t L = rv.len[r];
     const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
@@ -177,13 +224,11 @@ __device__ inline void vote_read(const ReadsView& rv, uint64_t r, int k, int can
     uint32_t n = 0;
     for (uint32_t base = 0; base < nwin; base += 64) {
       const uint32_t p = base + lane;
-      uint32_t c = 0xFFFFFFFFu;
-      if (p < nwin) {
-        uint64_t fwd;
-        bool valid;
-        window_at(rv, wb, p, k, fwd, valid);
-        if (valid) c = ctable_get(slots, boff, bmask, occ, canon_key(fwd, k, canonical));
-      }
+      uint64_t fwd = 0;
+      bool valid = false;
+      if (p < nwin) window_at(rv, wb, p, k, fwd, valid);
+      const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
+      const uint32_t c = coop_cget64(slots, boff, bmask, occ, key, valid, tmp, lane);
       const bool has = c != 0xFFFFFFFFu;
       const uint64_t m = __ballot(has);
       if (has) hits[n + __popcll(m & ((1ull << lane) - 1))] = c;
