@@ -62,6 +62,44 @@ def cpu_baseline(tx, n_pairs_sample: int, max_cov: int) -> dict:
             "count_s": t1 - t0, "stats_s": t2 - t1}
 
 
+def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict, steps: int) -> dict:
+    """Rates of random 64-byte line reads and random 8-byte atomics over a
+    buffer the size of the step's count table (tk_micro.hip), and the count
+    and scoring kernels' per-window rates as fractions of them."""
+    import ctypes
+    from trinityrnaseq_amd import api
+    lib = ctypes.CDLL(str(ROOT / "trinityrnaseq_amd" / "lib" / "libtk_micro.so"))
+    lib.tkm_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint64)]
+    lib.tkm_run.restype = ctypes.c_int
+    nbytes = max(1 << 30, table_bytes) // 4096 * 4096
+    buf = api.DeviceBuffer(ctx, nbytes)  # contents are irrelevant to the rates
+    rates, ops = {}, {}
+    for kind, name in ((1, "line64_read"), (2, "line128_read"), (3, "atomic_add"), (4, "atomic_add_ret")):
+        best = 0.0
+        for _ in range(3):
+            ms, done = ctypes.c_float(), ctypes.c_uint64()
+            rc = lib.tkm_run(kind, ctypes.c_void_p(buf.ptr), nbytes, wins_per_step, ctypes.byref(ms),
+                             ctypes.byref(done))
+            if rc:
+                raise RuntimeError(f"tkm_run({name}) failed: {rc}")
+            best = max(best, done.value / (ms.value / 1e3))
+        rates[name] = best
+        ops[name] = done.value
+    buf.free()
+    out = {"buffer_bytes": nbytes, "ops_per_launch": ops,
+           "line64_reads_per_s": rates["line64_read"],
+           "line128_reads_per_s": rates["line128_read"],
+           "atomics_per_s": rates["atomic_add"],
+           "returning_atomics_per_s": rates["atomic_add_ret"]}
+    for name, ref in (("count", "atomic_add_ret"), ("stats", "line64_read")):
+        if name in kern:
+            per_s = wins_per_step / (kern[name]["total_ms"] / 1e3 / steps)
+            out[f"{name}_windows_per_s"] = per_s
+            out[f"{name}_frac_of_ceiling"] = round(per_s / rates[ref], 4)
+    return out
+
+
 def make_components(n_comp: int, seed: int = 1003):
     """Chrysalis-style components: reads per component ~ power law (alpha 1.8)
     on [10, 200k]; 1-20 transcripts of 300-5000 nt; 150-nt reads at 1%
@@ -170,6 +208,27 @@ def bench_components(args) -> dict:
     return out
 
 
+def plain_job(ctx, seq: np.ndarray, off: np.ndarray, n: int, max_cov: int) -> int:
+    """One sample through the plain per-job API: upload + pack, size, count,
+    stats, select, selected flags back to the host, free everything."""
+    from trinityrnaseq_amd import api
+    reads = api.Reads(ctx, seq, off)
+    est = api.estimate_distinct(ctx, reads, K, True)
+    table = api.Table(ctx, K, True, int(est * 1.05) + 4096, 0.5)
+    table.count(reads)
+    d_stats = api.DeviceBuffer(ctx, 2 * n * api.READ_STAT_DTYPE.itemsize)
+    d_sel = api.DeviceBuffer(ctx, n + 64)
+    api.read_cov_stats_d(ctx, table, reads, d_stats.ptr, min_count=2)
+    max_nwin = max(0, reads.info()[2] - K + 1)
+    nsel = api.select_pairs_d(ctx, d_stats.ptr, d_stats.ptr + n * api.READ_STAT_DTYPE.itemsize, n, max_cov,
+                              1, 10000, max_nwin, d_sel.ptr)
+    sel = np.empty(n, np.uint8)
+    d_sel.download(sel)
+    for o in (d_sel, d_stats, table, reads):
+        o.free()
+    return nsel
+
+
 def bench_serving(args) -> dict:
     from trinityrnaseq_amd import api, synth
     rng = np.random.default_rng(2000)
@@ -179,34 +238,54 @@ def bench_serving(args) -> dict:
     samples = []
     for j, n in enumerate(sizes):
         tx = synth.large_transcriptome(2000 + j, 20_000)
-        buf = np.empty(2 * int(n) * READ_LEN, np.uint8)
+        # page-locked host memory, as a server reading FASTQ into DMA buffers holds its input
+        pinned = api.HostBuffer(2 * int(n) * READ_LEN)
+        buf = pinned.array
         synth.pe_reads(tx, int(n), READ_LEN, seed=2000 + j, out=buf)
         off = np.arange(2 * int(n) + 1, dtype=np.uint64) * np.uint64(READ_LEN)
-        samples.append((buf, off, int(n)))
+        samples.append((buf, off, int(n), pinned))
     total_pairs = int(sizes.sum())
     log(f"generated {args.samples} samples, {total_pairs} pairs in {time.perf_counter() - t:.1f}s")
-    # serial plain GPU form: one job at a time
-    srv1 = api.Server(slots=1, max_cov=args.max_cov)
+    # plain per-job GPU form, one sample at a time
+    ctx = api.Context(0)
+    plain_job(ctx, samples[0][0], samples[0][1], samples[0][2], args.max_cov)  # warm-up
     t = time.perf_counter()
-    serial_lat = []
-    for b, o, n in samples:
-        jid = srv1.submit(b, o, n)
-        res, _ = srv1.wait(jid)
-        serial_lat.append(res.latency_s)
-    serial_s = time.perf_counter() - t
+    plain_lat = []
+    plain_sel = []
+    for b, o, n, _ in samples:
+        t1 = time.perf_counter()
+        plain_sel.append(plain_job(ctx, b, o, n, args.max_cov))
+        plain_lat.append(time.perf_counter() - t1)
+    plain_s = time.perf_counter() - t
+    ctx.close()
+    log(f"plain serial: {plain_s:.3f}s")
+    # the server with one slot (same code path as served, no overlap)
+    srv1 = api.Server(slots=1, max_cov=args.max_cov, hbm_budget_bytes=int(args.hbm_budget_gb * 2**30))
+    srv1.wait(srv1.submit(samples[0][0], samples[0][1], samples[0][2]))
+    t = time.perf_counter()
+    for b, o, n, _ in samples:
+        srv1.wait(srv1.submit(b, o, n))
+    slot1_s = time.perf_counter() - t
     srv1.close()
+    log(f"server 1 slot: {slot1_s:.3f}s")
     # served: burst for capacity, then Poisson arrivals at 0.8 x capacity
     srv = api.Server(slots=args.slots, max_cov=args.max_cov, hbm_budget_bytes=int(args.hbm_budget_gb * 2**30))
+    srv.wait(srv.submit(samples[0][0], samples[0][1], samples[0][2]))
     t = time.perf_counter()
-    ids = [srv.submit(b, o, n) for b, o, n in samples]
+    ids = [srv.submit(b, o, n) for b, o, n, _ in samples]
+    served_sel = []
     for jid in ids:
-        srv.wait(jid)
+        res, _ = srv.wait(jid)
+        served_sel.append(res.n_selected)
     burst_s = time.perf_counter() - t
+    log(f"served burst: {burst_s:.3f}s")
+    if served_sel != plain_sel:
+        raise RuntimeError("served selections differ from the plain per-job ones")
     lam = 0.8 * args.samples / burst_s
     gaps = rng.exponential(1.0 / lam, size=args.samples)
     t0 = time.perf_counter()
     ids = []
-    for (b, o, n), g in zip(samples, gaps):
+    for (b, o, n, _), g in zip(samples, gaps):
         time.sleep(g)
         ids.append(srv.submit(b, o, n))
     lat = []
@@ -215,22 +294,27 @@ def bench_serving(args) -> dict:
         lat.append(res.latency_s)
     poisson_s = time.perf_counter() - t0
     srv.close()
+    for s_ in samples:
+        s_[3].free()
     lat = sorted(lat)
     q = lambda v, x: v[min(len(v) - 1, int(round(x * (len(v) - 1))))]
     return {
         "metric": METRIC + " [served stream]", "value": 2 * total_pairs / burst_s, "unit": "reads/s",
-        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(burst_s * 1000, 3),
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(burst_s * 1000, 3),
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "u64",
-        "data": "synthetic (seeded samples generated in-process)",
+        "data": "synthetic (seeded samples generated in-process, in page-locked host memory)",
         "config": {"workload": f"serving: {args.samples} samples, read pairs log-uniform "
                                f"[{lo:.0f}, {hi:.0f}] (scale {args.serve_scale} of 0.5M-30M), 2x150 PE, "
                                f"count + coverage stats + max_cov={args.max_cov} selection per sample",
                    "slots": args.slots, "hbm_budget_gb": args.hbm_budget_gb, "total_pairs": total_pairs},
-        "serial_plain_reads_per_s": 2 * total_pairs / serial_s,
-        "served_over_serial": serial_s / burst_s,
+        "serial_plain_reads_per_s": 2 * total_pairs / plain_s,
+        "served_over_serial": plain_s / burst_s,
+        "server_1slot_reads_per_s": 2 * total_pairs / slot1_s,
+        "served_over_1slot": slot1_s / burst_s,
+        "selections_match_plain": True,
         "poisson": {"lambda_per_s": lam, "p50_s": q(lat, 0.5), "p99_s": q(lat, 0.99),
                     "reads_per_s": 2 * total_pairs / poisson_s},
-        "serial_latency_p50_s": q(sorted(serial_lat), 0.5), "serial_latency_p99_s": q(sorted(serial_lat), 0.99),
+        "serial_latency_p50_s": q(sorted(plain_lat), 0.5), "serial_latency_p99_s": q(sorted(plain_lat), 0.99),
     }
 
 
@@ -243,6 +327,8 @@ def main() -> int:
     ap.add_argument("--max-cov", type=int, default=50)
     ap.add_argument("--cpu-pairs", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ceiling", action="store_true",
+                    help="skip the random-access ceiling microbenchmarks after the timed steps")
     ap.add_argument("--copy-ref-gb", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GB (HBM counter reference)")
     ap.add_argument("--workload", choices=["large_pe", "components", "serving"], default="large_pe")
@@ -347,6 +433,9 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                     "other": {n: round(v["achieved_gbs"], 1) for n, v in roof.items()}}
+
+    if roofline is not None and not args.no_ceiling:
+        roofline["ceiling"] = random_access_ceiling(ctx, info["capacity"] * 8, wins_per_step, kern, args.steps)
 
     cpu = None
     if not args.no_cpu:

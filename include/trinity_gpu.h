@@ -87,6 +87,10 @@ int tk_memcpy_d2h(tk_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
 /* Device-to-device copy on the context stream (also a known-size traffic
  * reference when reading HBM byte counters). */
 int tk_memcpy_d2d(tk_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
+/* Page-locked host memory: reads uploaded from it (tk_reads_from_host,
+ * tk_server_submit) move by DMA without a staging copy. */
+int tk_host_alloc(uint64_t bytes, void** h_ptr);
+int tk_host_free(void* h_ptr);
 
 /* ---------------- reads (device-resident, 2-bit packed) ---------------- */
 /*

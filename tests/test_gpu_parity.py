@@ -43,15 +43,21 @@ def _count(api, ctx, seqs_or_pair, k=25, canonical=True, expected=None, max_load
     return r, t, seq, off
 
 
-@pytest.fixture(params=["direct", "regions"])
+@pytest.fixture(params=["direct", "regions", "ordered"])
 def layout(request, monkeypatch):
-    """Count through the direct kernel, or through the region-partitioned
-    passes (1 MB regions so that small tables span many regions)."""
+    """Count through the direct kernel, through the region-partitioned
+    passes (1 MB regions so that small tables span many regions), or through
+    the direct kernel taking reads in the locality order (forced on small
+    inputs)."""
     if request.param == "direct":
         monkeypatch.setenv("TK_PARTITION", "0")
-    else:
+    elif request.param == "regions":
         monkeypatch.setenv("TK_PARTITION", "1")
         monkeypatch.setenv("TK_REGION_MB", "1")
+    else:
+        monkeypatch.setenv("TK_PARTITION", "0")
+        monkeypatch.setenv("TK_ORDER", "1")
+        monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
     return request.param
 
 
@@ -147,6 +153,24 @@ def test_read_stats_parity(api, ctx, small_se, median_policy, nwin_policy, monke
         got = api.read_cov_stats(ctx, t, r, min_count, 1, nwin_policy, median_policy)
         want = O.read_stats(seq, off, ok, oc, 25, True, min_count, 1, nwin_policy, median_policy)
         assert np.array_equal(got, want.astype(got.dtype))
+
+
+def test_locality_order_changes_nothing(api, ctx, small_se, monkeypatch):
+    """The table passes take large read sets in minimizer order; tables and
+    per-read stats are identical to input order and to the oracle."""
+    say("locality_order")
+    seq, off = small_se
+    ok, oc = O.count(seq, off, 25, True)
+    want = O.read_stats(seq, off, ok, oc, 25, True, 2, 1, 0, 0)
+    for flag in ("0", "stats", "1"):
+        monkeypatch.setenv("TK_ORDER", flag)
+        r, t, _, _ = _count(api, ctx, (seq, off))
+        gk, gc = t.dump()
+        assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+        got = api.read_cov_stats(ctx, t, r, 2, 1, 0, 0)
+        assert np.array_equal(got, want.astype(got.dtype))
+        t.free()
+        r.free()
 
 
 def test_read_stats_long_reads(api, ctx):

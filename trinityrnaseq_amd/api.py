@@ -42,6 +42,8 @@ SIGNATURES = {
     "tk_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_uint64]),
     "tk_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_uint64]),
     "tk_memcpy_d2d": (C.c_int, [_P, _P, _P, C.c_uint64]),
+    "tk_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_P)]),
+    "tk_host_free": (C.c_int, [_P]),
     "tk_reads_from_host": (C.c_int, [_P, C.c_char_p, _P, C.c_uint64, C.POINTER(_P)]),
     "tk_reads_from_device_d": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(_P)]),
     "tk_reads_info": (C.c_int, [_P, _U64P, _U64P, _U32P]),
@@ -205,6 +207,29 @@ class DeviceBuffer:
             pass
 
 
+class HostBuffer:
+    """Page-locked host memory (tk_host_alloc), viewed as a uint8 array."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().tk_host_alloc(self.nbytes, C.byref(p)), "tk_host_alloc")
+        self.ptr = p.value or 0
+        self.array = np.ctypeslib.as_array((C.c_uint8 * max(self.nbytes, 1)).from_address(self.ptr))[: self.nbytes]
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().tk_host_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def pack_sequences(seqs) -> tuple[bytes, np.ndarray]:
     """List of str/bytes -> (concatenated bytes, uint64 offsets[n+1])."""
     bs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
@@ -215,11 +240,17 @@ def pack_sequences(seqs) -> tuple[bytes, np.ndarray]:
 
 
 class Reads:
-    def __init__(self, ctx: Context, seq: bytes, offsets: np.ndarray):
+    def __init__(self, ctx: Context, seq, offsets: np.ndarray):
+        """seq: bytes, or a uint8 array (pinned host memory uploads without staging)."""
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        if isinstance(seq, np.ndarray):
+            seq = np.ascontiguousarray(seq, dtype=np.uint8)
+            sp = C.cast(C.c_void_p(seq.ctypes.data), C.c_char_p) if seq.size else None
+        else:
+            sp = seq
         self.h = C.c_void_p()
         self._ctx = ctx
-        _check(lib().tk_reads_from_host(ctx.h, seq, _ptr(offsets), len(offsets) - 1, C.byref(self.h)),
+        _check(lib().tk_reads_from_host(ctx.h, sp, _ptr(offsets), len(offsets) - 1, C.byref(self.h)),
                "tk_reads_from_host")
 
     @classmethod

@@ -3,6 +3,7 @@
 Outputs (git-ignored, shipped with the source tree):
   trinityrnaseq_amd/lib/libtrinity_gpu.so   HIP kernels + C ABI (include/trinity_gpu.h)
   trinityrnaseq_amd/lib/libtk_synth.so      seeded synthetic read generator (bench/tests)
+  trinityrnaseq_amd/lib/libtk_micro.so      random-access ceiling microbenchmarks (bench)
   trinityrnaseq_amd/bin/*                   CLI tools keeping the reference command lines
 """
 from __future__ import annotations
@@ -94,6 +95,17 @@ def build_synth(force: bool = True) -> Path:
     return out
 
 
+def build_micro(force: bool = True) -> Path:
+    LIB.mkdir(exist_ok=True)
+    src = CSRC / "tk_micro.hip"
+    out = LIB / "libtk_micro.so"
+    if force or not _newer(out, [src]):
+        tmp = LIB / "libtk_micro.so.tmp"
+        _run([HIPCC, *HIP_FLAGS, "-shared", str(src), "-o", str(tmp)])
+        os.replace(tmp, out)
+    return out
+
+
 def build_tools(force: bool = True) -> list[Path]:
     BIN.mkdir(exist_ok=True)
     common = [CSRC / c for c in TOOL_COMMON]
@@ -117,6 +129,7 @@ def build_tools(force: bool = True) -> list[Path]:
 def build_all(force: bool = True) -> None:
     build_library(force)
     build_synth(force)
+    build_micro(force)
     build_tools(force)
 
 

@@ -312,6 +312,7 @@ static ReadsView rview(const tk_reads* r) {
   v.word_off = r->d_word_off;
   v.len = r->d_len;
   v.n_reads = r->n_reads;
+  v.order = nullptr;
   return v;
 }
 

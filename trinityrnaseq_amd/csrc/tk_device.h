@@ -58,7 +58,21 @@ struct ReadsView {
   const uint64_t* word_off;
   const uint32_t* len;
   uint64_t n_reads;
+  const uint32_t* order = nullptr;  // processing order of the table passes (nullptr = input order)
 };
+
+// Read processed at position i of the table passes.
+__device__ __forceinline__ uint64_t read_at(const ReadsView& rv, uint64_t i) {
+  return rv.order ? rv.order[i] : i;
+}
+
+// Workgroup b of a launch runs on XCD b % 8 (round-robin dispatch).  For a
+// grid that is a multiple of 8 this maps b to a logical block such that each
+// XCD takes one contiguous run of logical blocks, so work adjacent in the
+// read order shares that XCD's L2.
+__device__ __forceinline__ uint64_t xcd_block(uint64_t b, uint64_t grid) {
+  return (b & 7) * (grid >> 3) + (b >> 3);
+}
 
 // Forward k-mer starting at base `pos` of the read whose first word is wbase.
 __device__ __forceinline__ void window_at(const ReadsView& rv, uint64_t wbase,
@@ -352,6 +366,81 @@ __device__ __forceinline__ void coop_get128(const TableView& t, uint64_t keyA, b
   cB = fb;
 }
 
+// Bit g of the result is set when nibble g (lanes 4g..4g+3) of x is non-zero.
+__device__ __forceinline__ uint32_t nibble_bits(uint64_t x) {
+  x |= x >> 1;
+  x |= x >> 2;
+  x &= 0x1111111111111111ULL;
+  x = (x | (x >> 3)) & 0x0303030303030303ULL;
+  x = (x | (x >> 6)) & 0x000F000F000F000FULL;
+  x = (x | (x >> 12)) & 0x000000FF000000FFULL;
+  return (uint32_t)((x | (x >> 24)) & 0xFFFFu);
+}
+
+// Cooperative lookup of 64*NK keys, NK per lane (key i of a lane is key
+// number 64*i + lane), four lanes per key.  In round q the lanes 4g..4g+3
+// read the 64-byte home bucket of key 16q+g, sixteen bytes (two slots) each:
+// one coalesced request per key, as with eight lanes, but half the load,
+// shuffle and compare instructions.  All 4*NK loads of a lane are issued
+// before any is inspected.  Whether the bucket matched or held an empty slot
+// reaches the key's own lane through the round's ballots; the matching lane
+// passes the raw count field through `val` (64*NK x u64, LDS).  A key whose
+// home bucket is full without a match continues with a full probe.  out[i]
+// is the count of key i (0 = absent or pruned); lanes with want[i] = false
+// take part in the rounds and get 0.
+template <int NK>
+__device__ __forceinline__ void coop4_get(const TableView& t, const uint64_t (&key)[NK],
+                                          const bool (&want)[NK], uint64_t* val, int lane,
+                                          uint64_t (&out)[NK]) {
+  constexpr int R = 4 * NK;
+  const int g = lane >> 2, sl = lane & 3;
+  uint64_t b[NK], tg[NK];
+#pragma unroll
+  for (int i = 0; i < NK; i++) {
+    b[i] = bucket0(t, key[i]);
+    tg[i] = key[i] | t.occ;
+  }
+  ulonglong2 sv[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const uint64_t bq = __shfl(b[q >> 2], (q & 3) * 16 + g);
+    sv[q] = reinterpret_cast<const ulonglong2*>(t.slots + (bq << 3))[sl];
+  }
+  uint64_t mbs[R], ebs[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const uint64_t tq = __shfl(tg[q >> 2], (q & 3) * 16 + g);
+    const bool m0 = (sv[q].x & t.kmo) == tq;
+    const bool m1 = (sv[q].y & t.kmo) == tq;
+    if (m0 | m1) val[16 * q + g] = (m0 ? sv[q].x : sv[q].y) >> t.cshift;
+    mbs[q] = __ballot(m0 | m1);
+    ebs[q] = __ballot((sv[q].x == 0) | (sv[q].y == 0));
+  }
+  wave_sync();
+  const int qq = lane >> 4, sh = (lane & 15) * 4;
+#pragma unroll
+  for (int i = 0; i < NK; i++) {
+    const uint64_t mq = qq == 0 ? mbs[4 * i] : qq == 1 ? mbs[4 * i + 1] : qq == 2 ? mbs[4 * i + 2] : mbs[4 * i + 3];
+    const uint64_t eq = qq == 0 ? ebs[4 * i] : qq == 1 ? ebs[4 * i + 1] : qq == 2 ? ebs[4 * i + 2] : ebs[4 * i + 3];
+    const bool hit = (mq >> sh) & 0xFull;
+    const bool empty = (eq >> sh) & 0xFull;
+    uint64_t f = 0;
+    if (want[i]) {
+      if (hit) {
+        f = val[64 * i + lane];
+        if (f == t.cmax) {
+          const uint64_t* o = ovf_find(t, tg[i]);
+          f = o ? *o : f;
+        }
+      } else if (!empty) {
+        f = table_get(t, key[i]);
+      }
+    }
+    out[i] = f;
+  }
+  wave_sync();
+}
+
 // Cooperative +1 of the 64 keys held one per lane (want = lane has a key).
 // Same bucket/round layout as coop_get64.  In each group the lane holding
 // the key adds 1; otherwise the lowest empty slot of the snapshot is claimed
@@ -424,6 +513,87 @@ __device__ __forceinline__ uint32_t coop_add64(const TableView& t, uint64_t key,
   spilled = false;
   if ((todo >> lane) & 1ull) {
     const uint32_t r = table_add(t, key, 1, err);
+    if (r == 2u) spilled = true;
+    else claimed += r;
+  }
+  return claimed;
+}
+
+// Cooperative +1 of the 64 keys held one per lane, four lanes per key (the
+// round layout of coop4_get: in round q the lanes 4g..4g+3 hold the sixteen
+// bytes 16*sl.. of the home bucket of key 16q+g, lane 16q+g's key).  In each
+// group the lane holding the key adds 1 to it; otherwise the lowest empty
+// slot of the snapshot is claimed with CAS.  Same protocol and guarantees as
+// coop_add64 (no key inserted twice; all rounds' atomics in flight before any
+// result is inspected; unresolved keys finished by their own lane).
+// `c` (1 <= c <= cmax) is the number of occurrences the lane adds.
+__device__ __forceinline__ uint32_t coop4_add(const TableView& t, uint64_t key, bool want, int lane,
+                                              uint32_t* err, bool& spilled, uint64_t c = 1) {
+  const uint64_t tagged = key | t.occ;
+  const uint64_t b = bucket0(t, key);
+  const int g = lane >> 2, sl = lane & 3;
+  const uint64_t one = c << t.cshift;
+  uint64_t todo = __ballot(want);
+  ulonglong2 sv[4];
+  uint64_t bq[4], tq[4], old[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    bq[q] = __shfl(b, q * 16 + g);
+    sv[q] = reinterpret_cast<const ulonglong2*>(t.slots + (bq[q] << 3))[sl];
+  }
+  uint32_t kind = 0;  // per round, 3 bits: 1 = add issued, 2 = CAS issued; bit 2 = second slot
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    tq[q] = __shfl(tagged, q * 16 + g);
+    old[q] = 0;
+    const bool act = (todo >> (q * 16 + g)) & 1ull;
+    const bool m0 = act && (sv[q].x & t.kmo) == tq[q];
+    const bool m1 = act && (sv[q].y & t.kmo) == tq[q];
+    const bool e0 = act && sv[q].x == 0;
+    const bool e1 = act && sv[q].y == 0;
+    const uint32_t mb = (uint32_t)((__ballot(m0 | m1) >> (4 * g)) & 0xFu);
+    const uint32_t eb = (uint32_t)((__ballot(e0 | e1) >> (4 * g)) & 0xFu);
+    const uint64_t oneq = __shfl(one, q * 16 + g);
+    if (m0 | m1) {
+      const int j = m0 ? 0 : 1;
+      old[q] = atomicAdd((unsigned long long*)(t.slots + (bq[q] << 3) + 2 * sl + j), (unsigned long long)oneq);
+      kind |= (1u | ((uint32_t)j << 2)) << (3 * q);
+    } else if (act && !mb && eb && sl == __builtin_ctz(eb)) {
+      const int j = e0 ? 0 : 1;
+      old[q] = atomicCAS((unsigned long long*)(t.slots + (bq[q] << 3) + 2 * sl + j), 0ULL,
+                         (unsigned long long)(tq[q] | oneq));
+      kind |= (2u | ((uint32_t)j << 2)) << (3 * q);
+    }
+  }
+  uint32_t claimed = 0;
+  uint64_t carry = 0;  // bit w: the +1 of lane w's key wrapped its count field
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint32_t kq = (kind >> (3 * q)) & 3u;
+    const int j = (kind >> (3 * q + 2)) & 1u;
+    bool done = false, wrap = false;
+    const uint64_t cq = __shfl(c, q * 16 + g);
+    if (kq == 1u) {
+      wrap = (old[q] >> t.cshift) + cq > t.cmax;
+      done = true;
+    } else if (kq == 2u) {
+      if (old[q] == 0) {
+        claimed++;
+        done = true;
+      } else if ((old[q] & t.kmo) == tq[q]) {
+        const uint64_t o2 = atomicAdd((unsigned long long*)(t.slots + (bq[q] << 3) + 2 * sl + j),
+                                      (unsigned long long)(cq << t.cshift));
+        wrap = (o2 >> t.cshift) + cq > t.cmax;
+        done = true;
+      }
+    }
+    todo &= ~((uint64_t)nibble_bits(__ballot(done)) << (16 * q));
+    carry |= (uint64_t)nibble_bits(__ballot(wrap)) << (16 * q);
+  }
+  if ((carry >> lane) & 1ull) ovf_add(t, tagged, t.cmax + 1, err);
+  spilled = false;
+  if ((todo >> lane) & 1ull) {
+    const uint32_t r = table_add(t, key, c, err);
     if (r == 2u) spilled = true;
     else claimed += r;
   }

@@ -27,6 +27,12 @@ int hip_fail(hipError_t e, const char* what);
 extern std::atomic<uint64_t> g_dev_bytes;
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p, size_t bytes);
+// Per-thread device block cache: between cache_begin and cache_end, blocks
+// this thread frees (up to limit_bytes in total) are kept for its later
+// allocations.  Only for a thread that runs all its work on one stream and
+// frees every object it creates on that thread.
+void cache_begin(size_t limit_bytes);
+void cache_end();
 
 struct PendingEvent {
   std::string name;
@@ -64,6 +70,8 @@ struct tk_reads {
   uint64_t* d_codes = nullptr;     // n_words + 1
   uint32_t* d_nmask = nullptr;     // n_words + 1
   std::map<int, uint64_t> windows; // k -> total windows
+  uint32_t* d_order = nullptr;     // locality order of the table passes (n_reads)
+  bool order_done = false;         // d_order built, or found not worth building
   uint64_t bytes = 0;
 };
 

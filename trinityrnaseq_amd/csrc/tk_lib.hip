@@ -14,7 +14,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 
 #include "tk_internal.h"
 #include "tk_perread.h"
@@ -39,16 +41,71 @@ int hip_fail(hipError_t e, const char* what) {
               hipGetErrorString(e));
 }
 
+// Device blocks of one host thread that issues all its work on one stream.
+// A freed block goes back to the thread's free list instead of hipFree
+// (which waits for the whole device), and the next allocation of a similar
+// size takes it: stream order makes the reuse safe without any wait.
+struct BlockCache {
+  std::multimap<size_t, void*> free_blocks;  // capacity -> block
+  std::unordered_map<void*, size_t> owned;   // every block of this cache -> capacity
+  size_t cached = 0;                         // bytes on the free list
+  size_t limit = 0;
+};
+static thread_local BlockCache* t_cache = nullptr;
+
+static size_t cache_round(size_t bytes) {
+  const size_t g = bytes >= (64u << 20) ? (size_t)(2u << 20) : (size_t)4096;
+  return (bytes + g - 1) / g * g;
+}
+
+void cache_begin(size_t limit_bytes) {
+  if (!t_cache) t_cache = new BlockCache();
+  t_cache->limit = limit_bytes;
+}
+
+void cache_end() {
+  BlockCache* c = t_cache;
+  if (!c) return;
+  t_cache = nullptr;
+  for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+  // blocks still handed out are released by their own dev_free (plain hipFree)
+  delete c;
+}
+
 int dev_alloc(void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;
-  hipError_t e = hipMalloc(p, bytes);
+  size_t cap = bytes;
+  if (BlockCache* c = t_cache) {
+    auto it = c->free_blocks.lower_bound(bytes);
+    if (it != c->free_blocks.end() && it->first <= 2 * bytes + (1u << 20)) {
+      *p = it->second;
+      c->cached -= it->first;
+      c->free_blocks.erase(it);
+      g_dev_bytes += bytes;
+      return TK_OK;
+    }
+    cap = cache_round(bytes);
+  }
+  hipError_t e = hipMalloc(p, cap);
+  if (e != hipSuccess && t_cache && !t_cache->free_blocks.empty()) {
+    // give the cached blocks back and try once more
+    (void)hipGetLastError();
+    for (auto& kv : t_cache->free_blocks) {
+      (void)hipFree(kv.second);
+      t_cache->owned.erase(kv.second);
+    }
+    t_cache->free_blocks.clear();
+    t_cache->cached = 0;
+    e = hipMalloc(p, cap);
+  }
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
     return fail(e == hipErrorOutOfMemory ? TK_ERR_OOM : TK_ERR_HIP,
                 "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
   }
+  if (t_cache) t_cache->owned[*p] = cap;
   g_dev_bytes += bytes;
   return TK_OK;
 }
@@ -56,8 +113,28 @@ int dev_alloc(void** p, size_t bytes) {
 void dev_free(void* p, size_t bytes) {
   if (!p) return;
   if (bytes == 0) bytes = 256;
-  (void)hipFree(p);
   g_dev_bytes -= bytes;
+  if (BlockCache* c = t_cache) {
+    auto it = c->owned.find(p);
+    if (it != c->owned.end()) {
+      const size_t cap = it->second;
+      if (cap <= c->limit) {
+        c->free_blocks.emplace(cap, p);
+        c->cached += cap;
+        // over the limit: release the largest free blocks
+        while (c->cached > c->limit && !c->free_blocks.empty()) {
+          auto last = std::prev(c->free_blocks.end());
+          (void)hipFree(last->second);
+          c->cached -= last->first;
+          c->owned.erase(last->second);
+          c->free_blocks.erase(last);
+        }
+        return;
+      }
+      c->owned.erase(it);
+    }
+  }
+  (void)hipFree(p);
 }
 
 int ensure_scratch(tk_ctx* ctx, int slot, size_t bytes, void** p) {
@@ -151,6 +228,7 @@ static ReadsView reads_view(const tk_reads* r) {
   v.word_off = r->d_word_off;
   v.len = r->d_len;
   v.n_reads = r->n_reads;
+  v.order = r->d_order;
   return v;
 }
 
@@ -378,6 +456,102 @@ static int reads_build(tk_ctx* ctx, const char* d_seq, const uint64_t* d_off, ui
   return TK_OK;
 }
 
+// Sort key of read r for the locality order of the table passes: the
+// smallest mixed hash over the read's canonical 16-mers (an N base reads as
+// A: the key only orders reads).  Reads that share it overlap one locus of
+// the transcriptome, so taken together their k-mers hit the same table lines
+// while those are still in L2 / Infinity Cache, instead of one HBM line per
+// window.  Reads shorter than 16 bases sort last.
+__global__ __launch_bounds__(256) void k_minimizer(ReadsView rv, uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ idx) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rv.n_reads;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t L = rv.len[r];
+    const uint64_t wb = rv.word_off[r];
+    uint32_t fwd = 0, rc = 0, best = 0xFFFFFFFFu;
+    for (uint32_t w = 0; 32 * w < L; w++) {
+      const uint64_t code = rv.codes[wb + w];
+      const uint32_t nb = L - 32 * w < 32u ? L - 32 * w : 32u;
+      for (uint32_t j = 0; j < nb; j++) {
+        const uint32_t b = (uint32_t)(code >> (62 - 2 * j)) & 3u;
+        fwd = (fwd << 2) | b;
+        rc = (rc >> 2) | ((3u - b) << 30);
+        if (32 * w + j >= 15) {
+          uint32_t x = fwd < rc ? fwd : rc;
+          x *= 0x9E3779B1u;
+          x ^= x >> 16;
+          x *= 0x85EBCA6Bu;
+          x ^= x >> 13;
+          best = x < best ? x : best;
+        }
+      }
+    }
+    keys[r] = best;
+    idx[r] = (uint32_t)r;
+  }
+}
+
+// TK_COOP8=1 counts with eight lanes (8 bytes each) per key instead of four
+// lanes of 16 bytes.
+static bool coop8_enabled() {
+  const char* e = getenv("TK_COOP8");
+  return e && e[0] == '1';
+}
+
+// TK_AGG=0 counts ordered reads without the block-level aggregation.
+static bool agg_enabled() {
+  const char* e = getenv("TK_AGG");
+  return !(e && e[0] == '0');
+}
+
+// Which table passes take reads in the locality order: TK_ORDER=0 none,
+// TK_ORDER=stats the scoring pass only, TK_ORDER=1 count and scoring.
+enum { kOrderNone = 0, kOrderStats = 1, kOrderAll = 2 };
+static int order_mode() {
+  const char* e = getenv("TK_ORDER");
+  if (!e) return kOrderStats;
+  if (e[0] == '0') return kOrderNone;
+  if (e[0] == 's') return kOrderStats;
+  return kOrderAll;
+}
+
+// Read sets below this many reads stay in input order (TK_ORDER_MIN_READS).
+static uint64_t order_min_reads() {
+  const char* e = getenv("TK_ORDER_MIN_READS");
+  return e ? strtoull(e, nullptr, 10) : (1ull << 16);
+}
+
+// Build the locality order of r once (read set large enough to be out of
+// cache); results never depend on it, only the memory access pattern does.
+static int reads_order(tk_ctx* ctx, const tk_reads* rc_) {
+  tk_reads* r = const_cast<tk_reads*>(rc_);
+  if (r->order_done) return TK_OK;
+  if (order_mode() == kOrderNone || r->n_reads < order_min_reads() || r->n_reads > 0x7FFFFFFFull) {
+    r->order_done = true;
+    return TK_OK;
+  }
+  const uint64_t n = r->n_reads;
+  uint64_t* keys = nullptr;
+  uint32_t* order = nullptr;
+  int rc = dev_alloc((void**)&keys, n * 8);
+  if (rc) return rc;
+  if ((rc = dev_alloc((void**)&order, n * 4))) { dev_free(keys, n * 8); return rc; }
+  const uint64_t grid = std::min<uint64_t>(ceil_div(n, 256), (uint64_t)std::max(ctx->num_cu, 1) * 16);
+  hipEvent_t a;
+  timing_begin(ctx, &a);
+  hipLaunchKernelGGL(k_minimizer, dim3(grid), dim3(256), 0, ctx->stream, reads_view(r), keys, order);
+  timing_end(ctx, "minimizer", a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) rc = hip_fail(e, "k_minimizer");
+  if (!rc) rc = sort_pairs_u64_u32(ctx, keys, order, n, 32);
+  dev_free(keys, n * 8);
+  if (rc) { dev_free(order, n * 4); return rc; }
+  r->d_order = order;
+  r->bytes += n * 4;
+  r->order_done = true;
+  return TK_OK;
+}
+
 static int reads_windows(tk_ctx* ctx, const tk_reads* rc_, int k, uint64_t* total) {
   tk_reads* r = const_cast<tk_reads*>(rc_);
   auto it = r->windows.find(k);
@@ -444,22 +618,24 @@ constexpr int kReadsPerBlock = 64;
 
 // One thread per k-window of a block of 64 consecutive reads.  flags[0] gets
 // error bits, flags[2..3] (u64) the number of newly claimed slots.
+template <int LANES>
 __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
                                                uint64_t r_end, int k, int canonical,
                                                TableView t, uint32_t* __restrict__ flags) {
   __shared__ uint32_t s_off[kReadsPerBlock + 1];
   __shared__ uint64_t s_wb[kReadsPerBlock];
   __shared__ uint32_t s_new;
-  const uint64_t r0 = r_begin + (uint64_t)blockIdx.x * kReadsPerBlock;
+  const uint64_t r0 = r_begin + xcd_block(blockIdx.x, gridDim.x) * kReadsPerBlock;
   if (r0 >= r_end) return;
   const int nr = (int)std::min<uint64_t>(kReadsPerBlock, r_end - r0);
   const int tid = threadIdx.x;
   if (tid < kReadsPerBlock) {
     uint32_t nw = 0;
     if (tid < nr) {
-      const uint32_t L = rv.len[r0 + tid];
+      const uint64_t rr = read_at(rv, r0 + tid);
+      const uint32_t L = rv.len[rr];
       nw = L >= (uint32_t)k ? L - k + 1 : 0;
-      s_wb[tid] = rv.word_off[r0 + tid];
+      s_wb[tid] = rv.word_off[rr];
     }
     uint32_t x = nw;
 #pragma unroll
@@ -489,11 +665,146 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
     }
     const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
     bool lost = false;
-    mynew += coop_add64(t, key, valid, tid & 63, &flags[0], lost);
+    if (LANES == 4)
+      mynew += coop4_add(t, key, valid, tid & 63, &flags[0], lost);
+    else
+      mynew += coop_add64(t, key, valid, tid & 63, &flags[0], lost);
     if (lost) atomicOr(&flags[0], 1u);
   }
   mynew = wave_sum_u32(mynew);
   if ((tid & 63) == 0 && mynew) atomicAdd(&s_new, mynew);
+  __syncthreads();
+  if (tid == 0 && s_new)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)s_new);
+}
+
+// Block-aggregated count for reads taken in the locality order, where the
+// 64 reads of a block largely overlap one locus.  The block's windows go
+// kAggTile at a time into an LDS hash table (tagged key -> occurrences);
+// each distinct key then reaches the global table once, with its number of
+// occurrences, through the cooperative four-lane add.  Needs cmax >=
+// kAggTile (k <= 26) so that one add never carries more than once.
+constexpr int kAggSlots = 2048;
+constexpr int kAggTile = 1024;
+
+__global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
+                                                   int canonical, TableView t, uint32_t* __restrict__ flags) {
+  __shared__ uint64_t s_key[kAggSlots];  // tagged key, 0 = empty
+  __shared__ uint32_t s_cnt[kAggSlots];
+  __shared__ uint32_t s_off[kReadsPerBlock + 1];
+  __shared__ uint64_t s_wb[kReadsPerBlock];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_new;
+  const uint64_t r0 = r_begin + xcd_block(blockIdx.x, gridDim.x) * kReadsPerBlock;
+  if (r0 >= r_end) return;
+  const int nr = (int)std::min<uint64_t>(kReadsPerBlock, r_end - r0);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid < kReadsPerBlock) {
+    uint32_t nw = 0;
+    if (tid < nr) {
+      const uint64_t rr = read_at(rv, r0 + tid);
+      const uint32_t L = rv.len[rr];
+      nw = L >= (uint32_t)k ? L - k + 1 : 0;
+      s_wb[tid] = rv.word_off[rr];
+    }
+    uint32_t x = nw;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o);
+      if (tid >= o) x += y;
+    }
+    s_off[tid + 1] = x;
+    if (tid == 0) { s_off[0] = 0; s_new = 0; }
+  }
+  for (int i = tid; i < kAggSlots; i += blockDim.x) {
+    s_key[i] = 0;
+    s_cnt[i] = 0;
+  }
+  __syncthreads();
+  const uint32_t total = s_off[nr];
+  uint32_t mynew = 0;
+  for (uint32_t tile = 0; tile < total; tile += kAggTile) {
+    // 1. occurrences of the tile's windows into the LDS table (load <= 1/2)
+    const uint32_t tend = total - tile < (uint32_t)kAggTile ? total : tile + kAggTile;
+    for (uint32_t w = tile + tid; w < tend; w += blockDim.x) {
+      int lo = 0, hi = nr;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_off[mid] <= w) lo = mid; else hi = mid;
+      }
+      uint64_t fwd;
+      bool valid;
+      window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
+      if (!valid) continue;
+      const uint64_t tagged = canon_key(fwd, k, canonical) | t.occ;
+      uint32_t h = (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 53) & (kAggSlots - 1);
+      while (true) {
+        uint64_t cur = s_key[h];
+        if (cur == 0) {
+          cur = atomicCAS((unsigned long long*)&s_key[h], 0ULL, (unsigned long long)tagged);
+          if (cur == 0) cur = tagged;
+        }
+        if (cur == tagged) {
+          atomicAdd(&s_cnt[h], 1u);
+          break;
+        }
+        h = (h + 1) & (kAggSlots - 1);
+      }
+    }
+    __syncthreads();
+    // 2. compact the occupied entries to the front (thread t owns 8 entries)
+    constexpr int kPer = kAggSlots / 256;
+    uint64_t ek[kPer];
+    uint32_t ec[kPer];
+    uint32_t nocc = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      ek[j] = s_key[tid * kPer + j];
+      ec[j] = s_cnt[tid * kPer + j];
+      nocc += ek[j] != 0;
+    }
+    uint32_t incl = nocc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    uint32_t base = incl - nocc, ndist = 0;
+    for (int w2 = 0; w2 < 4; w2++) {
+      const uint32_t ws = s_wsum[w2];
+      if (w2 < wv) base += ws;
+      ndist += ws;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (ek[j]) {
+        s_key[base] = ek[j];
+        s_cnt[base] = ec[j];
+        base++;
+      }
+    }
+    __syncthreads();
+    // 3. one global add per distinct key (wave-uniform trip count)
+    for (uint32_t e0 = (uint32_t)wv * 64; e0 < ndist; e0 += blockDim.x) {
+      const uint32_t e = e0 + lane;
+      const bool want = e < ndist;
+      const uint64_t key = want ? (s_key[e] & ~t.occ) : 0;
+      const uint64_t c = want ? s_cnt[e] : 1;
+      bool lost = false;
+      mynew += coop4_add(t, key, want, lane, &flags[0], lost, c);
+      if (lost) atomicOr(&flags[0], 1u);
+    }
+    __syncthreads();
+    for (int i = tid; i < kAggSlots; i += blockDim.x) {
+      s_key[i] = 0;
+      s_cnt[i] = 0;
+    }
+    __syncthreads();
+  }
+  mynew = wave_sum_u32(mynew);
+  if (lane == 0 && mynew) atomicAdd(&s_new, mynew);
   __syncthreads();
   if (tid == 0 && s_new)
     atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)s_new);
@@ -989,9 +1300,17 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* vals = s_dyn + (uint64_t)wv * slab;
   uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * kLookupScratch;
-  for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
-       r += (uint64_t)gridDim.x * wpb)
+  // XCD x (= blockIdx % 8) takes positions [x*n/8, (x+1)*n/8) of the read
+  // order; its waves stride through that range together (gridDim % 8 == 0)
+  const uint64_t per_xcd = (rv.n_reads + 7) >> 3;
+  const uint64_t x = blockIdx.x & 7;
+  const uint64_t lo = x * per_xcd;
+  const uint64_t hi = lo + per_xcd < rv.n_reads ? lo + per_xcd : rv.n_reads;
+  for (uint64_t i = lo + (uint64_t)(blockIdx.x >> 3) * wpb + wv; i < hi;
+       i += (uint64_t)(gridDim.x >> 3) * wpb) {
+    const uint64_t r = read_at(rv, i);
     stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
+  }
 }
 
 // =====================================================================
@@ -1229,6 +1548,23 @@ int tk_device_bytes_in_use(uint64_t* bytes) {
 static std::mutex g_buf_mu;
 static std::map<void*, uint64_t> g_bufs;
 
+int tk_host_alloc(uint64_t bytes, void** h) {
+  if (!h) return fail(TK_ERR_ARG, "NULL argument");
+  *h = nullptr;
+  hipError_t e = hipHostMalloc(h, bytes ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *h = nullptr;
+    return hip_fail(e, "hipHostMalloc");
+  }
+  return TK_OK;
+}
+
+int tk_host_free(void* h) {
+  if (!h) return TK_OK;
+  TK_HIP(hipHostFree(h));
+  return TK_OK;
+}
+
 int tk_buffer_alloc(tk_ctx* c, uint64_t bytes, void** d) {
   if (!c || !d) return fail(TK_ERR_ARG, "NULL argument");
   if (set_device(c->device)) return TK_ERR_HIP;
@@ -1351,6 +1687,7 @@ int tk_reads_free(tk_reads* r) {
   dev_free(r->d_word_off, (r->n_reads + 1) * 8);
   if (r->d_codes) dev_free(r->d_codes, (r->n_words + 1) * 8);
   if (r->d_nmask) dev_free(r->d_nmask, (r->n_words + 1) * 4);
+  if (r->d_order) dev_free(r->d_order, r->n_reads * 4);
   delete r;
   return TK_OK;
 }
@@ -1438,10 +1775,11 @@ static bool compact_enabled() {
   return e && e[0] == '1';
 }
 
-// TK_PARTITION=0 keeps large tables on the direct (unpartitioned) count.
+// TK_PARTITION=1 counts large tables in region-partitioned passes (off by
+// default: measured slower than the direct count on the 20M-pair workload).
 static bool partition_enabled() {
   const char* e = getenv("TK_PARTITION");
-  return !(e && e[0] == '0');
+  return e && e[0] == '1';
 }
 
 __global__ void k_add_keys(const uint64_t* __restrict__ keys, uint64_t n, TableView t,
@@ -1606,6 +1944,13 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
   const uint64_t cbits = 63 - 2 * t->k;
   if ((rc = table_ensure_ovf(c, t, t->ovf_bound + (total >> cbits) + 1))) return rc;
   t->ovf_bound += (total >> cbits) + 1;
+  ReadsView rview = reads_view(r);
+  if (order_mode() == kOrderAll) {
+    if ((rc = reads_order(c, r))) return rc;
+    rview.order = r->d_order;
+  } else {
+    rview.order = nullptr;
+  }
   uint64_t rb = 0;
   while (rb < r->n_reads) {
     const uint32_t nparts = (uint32_t)(t->nb >> t->rbits);
@@ -1632,12 +1977,19 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
       continue;
     }
     nreads = std::min<uint64_t>(nreads, r->n_reads - rb);
-    nreads = std::min<uint64_t>(nreads, (uint64_t)0x7FFFFFFF * kReadsPerBlock);
+    nreads = std::min<uint64_t>(nreads, (uint64_t)0x7FFFFFF0 * kReadsPerBlock);
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 16, c->stream));
-    uint64_t grid = ceil_div(nreads, kReadsPerBlock);
+    const uint64_t grid = (ceil_div(nreads, kReadsPerBlock) + 7) / 8 * 8;
     TableView v = table_view(t);
-    TK_LAUNCH(c, "count", grid, 256, 0, k_count, reads_view(r), rb, rb + nreads, t->k,
-              t->canonical, v, c->d_flags);
+    if (rview.order && agg_enabled() && 63 - 2 * t->k >= 11)
+      TK_LAUNCH(c, "count", grid, 256, 0, k_count_agg, rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags);
+    else if (coop8_enabled())
+      TK_LAUNCH(c, "count", grid, 256, 0, k_count<8>, rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags);
+    else
+      TK_LAUNCH(c, "count", grid, 256, 0, k_count<4>, rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags);
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 16, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table probe limit reached (flags=%u)", c->h_flags[0]);
@@ -1916,7 +2268,8 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
   tk_table* use = t;
   if ((rc = scoring_table(c, t, o->min_count, &use))) return rc;
   StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
-  int grid = grid_for_waves(c, r->n_reads, wpb);
+  if ((rc = reads_order(c, r))) return rc;
+  const int grid = (grid_for_waves(c, r->n_reads, wpb) + 7) / 8 * 8;
   TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, k_stats,
             reads_view(r), t->k,
             t->canonical, table_view(use), sp, d_out, slab);

@@ -33,17 +33,105 @@ __device__ inline uint32_t wave_select(const uint32_t* vals, uint32_t n, uint32_
   return prefix;
 }
 
-// fastaToKmerCoverageStats for read r, one wavefront: window values staged
-// in `vals` (LDS, >= nwin entries), radix-select median, exact integer sums.
-// tmp: 128 x u64 LDS scratch of the cooperative lookup.  Lane 0 writes *out.
+// Value of one window for the coverage statistics: `has` = the window takes
+// part (fastaToKmerCoverageStats.cpp skips or counts invalid windows by
+// policy), v = its count, or `missing` below min_count.
+__device__ __forceinline__ void window_value(const StatsParams& sp, bool in_read, bool valid, uint64_t c,
+                                             bool& has, uint32_t& v) {
+  has = false;
+  v = 0;
+  if (!in_read) return;
+  if (valid) {
+    v = (c != 0 && c >= sp.min_count) ? (c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c) : sp.missing;
+    has = true;
+  } else if (sp.nwin_policy == TK_NWIN_AS_MISSING) {
+    v = sp.missing;
+    has = true;
+  }
+}
+
+// kth smallest (0-based) of the values held two per lane (ha/va, hb_/vb; a
+// value takes part when its flag is set), radix select over bits hb..0.
+__device__ __forceinline__ uint32_t wave_select2(bool ha, uint32_t va, bool hb_, uint32_t vb, uint32_t kth,
+                                                 int hb) {
+  uint32_t prefix = 0, pmask = 0;
+  for (int bit = hb; bit >= 0; --bit) {
+    const bool za = ha && (va & pmask) == prefix && !((va >> bit) & 1u);
+    const bool zb = hb_ && (vb & pmask) == prefix && !((vb >> bit) & 1u);
+    const uint32_t c = (uint32_t)__popcll(__ballot(za)) + (uint32_t)__popcll(__ballot(zb));
+    if (kth >= c) {
+      kth -= c;
+      prefix |= 1u << bit;
+    }
+    pmask |= 1u << bit;
+  }
+  return prefix;
+}
+
+__device__ __forceinline__ void sq_add(uint64_t v, uint64_t& lo, uint64_t& hi) {
+  const uint64_t sq = v * v;
+  lo += sq;
+  hi += (lo < sq);
+}
+
+__device__ __forceinline__ void wave_sum_u128(uint64_t& lo, uint64_t& hi) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t ol = __shfl_xor(lo, o);
+    const uint64_t oh = __shfl_xor(hi, o);
+    const uint64_t nl = lo + ol;
+    hi = hi + oh + (nl < lo);
+    lo = nl;
+  }
+}
+
+// fastaToKmerCoverageStats for read r, one wavefront: exact integer sums and
+// the median by radix select.  Reads of up to 128 windows keep their values
+// in registers (two per lane); longer reads stage them in `vals` (LDS, >=
+// nwin entries) 128 windows at a time.  tmp: 128 x u64 LDS scratch of the
+// cooperative lookup.  Lane 0 writes *out.
 __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int canonical,
                                   const TableView& t, const StatsParams& sp, uint32_t* vals,
                                   uint64_t* tmp, int lane, tk_read_stat* out) {
-    const uint32_t L = rv.len[r];
-    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
-    const uint64_t wb = rv.word_off[r];
-    uint32_t n = 0, vmax = 0;
-    uint64_t sum = 0, sqlo = 0, sqhi = 0;
+  const uint32_t L = rv.len[r];
+  const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
+  const uint64_t wb = rv.word_off[r];
+  uint32_t n = 0, vmax = 0, med = 0;
+  uint64_t sum = 0, sqlo = 0, sqhi = 0;
+  if (nwin <= 128) {
+    const uint32_t pa = lane, pb = 64 + lane;
+    uint64_t fa = 0, fb = 0;
+    bool va = false, vb = false;
+    if (pa < nwin) window_at(rv, wb, pa, k, fa, va);
+    if (pb < nwin) window_at(rv, wb, pb, k, fb, vb);
+    const uint64_t keys[2] = {va ? canon_key(fa, k, canonical) : 0, vb ? canon_key(fb, k, canonical) : 0};
+    const bool want[2] = {va, vb};
+    uint64_t c[2];
+    coop4_get<2>(t, keys, want, tmp, lane, c);
+    bool ha, hb_;
+    uint32_t xa, xb;
+    window_value(sp, pa < nwin, va, c[0], ha, xa);
+    window_value(sp, pb < nwin, vb, c[1], hb_, xb);
+    n = (uint32_t)__popcll(__ballot(ha)) + (uint32_t)__popcll(__ballot(hb_));
+    sum = (uint64_t)xa + xb;
+    sq_add(xa, sqlo, sqhi);
+    sq_add(xb, sqlo, sqhi);
+    sum = wave_sum_u64(sum);
+    wave_sum_u128(sqlo, sqhi);
+    if (n) {
+      vmax = wave_max_u32(xa > xb ? xa : xb);
+      const int hb = 31 - __clz(vmax | 1u);
+      if (sp.median_policy == TK_MEDIAN_LOWER) {
+        med = wave_select2(ha, xa, hb_, xb, (n - 1) / 2, hb);
+      } else if (sp.median_policy == TK_MEDIAN_MEAN_FLOOR && !(n & 1)) {
+        const uint64_t a = wave_select2(ha, xa, hb_, xb, n / 2 - 1, hb);
+        const uint64_t b = wave_select2(ha, xa, hb_, xb, n / 2, hb);
+        med = (uint32_t)((a + b) / 2);
+      } else {
+        med = wave_select2(ha, xa, hb_, xb, n / 2, hb);
+      }
+    }
+  } else {
     // windows are taken 128 at a time (two per lane), values appended to
     // vals in window order
     for (uint32_t base = 0; base < nwin; base += 128) {
@@ -52,52 +140,28 @@ __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int ca
       bool va = false, vb = false;
       if (pa < nwin) window_at(rv, wb, pa, k, fa, va);
       if (pb < nwin) window_at(rv, wb, pb, k, fb, vb);
-      const uint64_t ka = va ? canon_key(fa, k, canonical) : 0;
-      const uint64_t kb = vb ? canon_key(fb, k, canonical) : 0;
-      uint64_t ca, cb;
-      coop_get128(t, ka, va, kb, vb, tmp, lane, ca, cb);
+      const uint64_t keys[2] = {va ? canon_key(fa, k, canonical) : 0, vb ? canon_key(fb, k, canonical) : 0};
+      const bool want[2] = {va, vb};
+      uint64_t c[2];
+      coop4_get<2>(t, keys, want, tmp, lane, c);
 #pragma unroll
       for (int half = 0; half < 2; half++) {
-        const uint32_t p = half ? pb : pa;
-        const bool valid = half ? vb : va;
-        const uint64_t c = half ? cb : ca;
-        bool has = false;
-        uint32_t v = 0;
-        if (p < nwin) {
-          if (valid) {
-            v = (c != 0 && c >= sp.min_count) ? (c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c)
-                                              : sp.missing;
-            has = true;
-          } else if (sp.nwin_policy == TK_NWIN_AS_MISSING) {
-            v = sp.missing;
-            has = true;
-          }
-        }
+        bool has;
+        uint32_t v;
+        window_value(sp, (half ? pb : pa) < nwin, half ? vb : va, c[half], has, v);
         const uint64_t m = __ballot(has);
         if (has) {
           vals[n + __popcll(m & ((1ull << lane) - 1))] = v;
           sum += v;
-          const uint64_t sq = (uint64_t)v * v;
-          sqlo += sq;
-          sqhi += (sqlo < sq);
+          sq_add(v, sqlo, sqhi);
           vmax = v > vmax ? v : vmax;
         }
         n += (uint32_t)__popcll(m);
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
     sum = wave_sum_u64(sum);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint64_t ol = __shfl_xor(sqlo, o);
-      const uint64_t oh = __shfl_xor(sqhi, o);
-      const uint64_t nl = sqlo + ol;
-      sqhi = sqhi + oh + (nl < sqlo);
-      sqlo = nl;
-    }
-    uint32_t med = 0;
+    wave_sum_u128(sqlo, sqhi);
     if (n) {
       vmax = wave_max_u32(vmax);
       const int hb = 31 - __clz(vmax | 1u);
@@ -111,15 +175,16 @@ __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int ca
         med = wave_select(vals, n, n / 2, hb, lane);
       }
     }
-    if (lane == 0) {
-      tk_read_stat st;
-      st.median = med;
-      st.nwin = n;
-      st.sum = sum;
-      st.sumsq_lo = sqlo;
-      st.sumsq_hi = sqhi;
-      *out = st;
-    }
+  }
+  if (lane == 0) {
+    tk_read_stat st;
+    st.median = med;
+    st.nwin = n;
+    st.sum = sum;
+    st.sumsq_lo = sqlo;
+    st.sumsq_hi = sqhi;
+    *out = st;
+  }
   wave_sync();
 }
 
@@ -164,17 +229,6 @@ __device__ __forceinline__ uint32_t ctable_get(const uint64_t* slots, uint64_t b
   return 0xFFFFFFFFu;
 }
 
-// Bit g of the result is set when nibble g (lanes 4g..4g+3) of x is non-zero.
-__device__ __forceinline__ uint32_t nibble_any16(uint64_t x) {
-  x |= x >> 1;
-  x |= x >> 2;
-  x &= 0x1111111111111111ULL;
-  uint32_t m = 0;
-#pragma unroll
-  for (int g = 0; g < 16; g++) m |= (uint32_t)((x >> (4 * g)) & 1u) << g;
-  return m;
-}
-
 // Cooperative component-table lookup of the 64 keys held one per lane: in
 // round q the four lanes 4g..4g+3 read the four 16-byte slots of the bucket
 // of lane 16q+g's key (one 64-byte request per key, four loads in flight per
@@ -200,8 +254,8 @@ __device__ __forceinline__ uint32_t coop_cget64(const uint64_t* slots, uint64_t 
     const uint64_t tq = __shfl(tagged, q * 16 + g);
     const bool m = sv[q].x == tq;
     if (m) tmp[q * 16 + g] = ~(uint32_t)sv[q].y;
-    const uint32_t gm = nibble_any16(__ballot(m));
-    const uint32_t ge = nibble_any16(__ballot(sv[q].x == 0));
+    const uint32_t gm = nibble_bits(__ballot(m));
+    const uint32_t ge = nibble_bits(__ballot(sv[q].x == 0));
     pending |= (uint64_t)(~gm & ~ge & 0xFFFFu) << (16 * q);
   }
   wave_sync();

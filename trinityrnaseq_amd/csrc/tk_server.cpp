@@ -11,7 +11,10 @@
 //   * admission by device footprint in two stages: the packed reads (known
 //     from the input size) before upload, the count table once its HyperLogLog
 //     estimate is known; a job waits until its bytes fit the budget;
-//   * results: selected pair flags (host), distinct estimate, per-stage times.
+//   * results: selected pair flags (host), distinct estimate, per-stage times;
+//   * each worker keeps the device blocks its jobs free (up to budget/(2*slots)
+//     bytes) and reuses them, so steady-state jobs make no hipMalloc/hipFree
+//     calls, which would otherwise wait for every worker's kernels.
 #include <stdint.h>
 #include <string.h>
 
@@ -26,6 +29,13 @@
 #include <vector>
 
 #include "../../include/trinity_gpu.h"
+
+namespace tk {
+// Per-thread device block cache (tk_lib.hip): a worker frees and reallocates
+// the same few block sizes job after job without hipFree/hipMalloc.
+void cache_begin(size_t limit_bytes);
+void cache_end();
+}  // namespace tk
 
 namespace {
 
@@ -149,6 +159,9 @@ int run_job(tk_server* s, tk_ctx* ctx, Job& j) {
 }
 
 void worker(tk_server* s, int idx) {
+  // Every object of a job is created and freed on this thread and its
+  // context's stream, which is what the block cache requires.
+  tk::cache_begin(s->budget / (2 * (uint64_t)s->opts.slots));
   tk_ctx* ctx = nullptr;
   int rc = tk_init(s->opts.device, &ctx);
   if (rc) {
@@ -182,6 +195,7 @@ void worker(tk_server* s, int idx) {
     s->cv_done.notify_all();
   }
   if (ctx) tk_ctx_free(ctx);
+  tk::cache_end();
 }
 
 }  // namespace
