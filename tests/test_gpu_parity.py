@@ -301,6 +301,31 @@ def test_reads_from_device_unaligned(api, ctx, shift):
     assert np.array_equal(st, O.read_stats(seq, off, ok, oc, 25, True, 1, 1, 0, 0).astype(st.dtype))
 
 
+def test_pack_every_byte_value(api, ctx):
+    """Every byte value 0..255 at every position of a 32-base word: only
+    A/C/G/T/a/c/g/t are bases, anything else breaks the windows over it."""
+    say("pack_every_byte_value")
+    rng = np.random.default_rng(2024)
+    seqs = []
+    for v in range(256):
+        for pos in (0, 7, 31, 32, 45):
+            b = bytearray(rng.choice(np.frombuffer(b"ACGTacgt", np.uint8), 80).tobytes())
+            b[pos] = v
+            seqs.append(bytes(b))
+    off = np.zeros(len(seqs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(s) for s in seqs])
+    seq = b"".join(seqs)
+    for k in (5, 25):
+        r = api.Reads(ctx, seq, off)
+        t = api.Table(ctx, k, True, 1 << 16)
+        t.count(r)
+        gk, gc = t.dump()
+        ok, oc = O.count(seq, off, k, True)
+        assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+        st = api.read_cov_stats(ctx, t, r)
+        assert np.array_equal(st, O.read_stats(seq, off, ok, oc, k, True, 1, 1, 0, 0).astype(st.dtype))
+
+
 def _components(seed, n_comp, max_reads=400):
     """Small Chrysalis-like components: transcripts, reads drawn from them
     (both strands, 1% errors), and contig-like pieces of the transcripts."""

@@ -307,11 +307,32 @@ __global__ void k_block_starts(const uint64_t* __restrict__ word_off, uint64_t n
   }
 }
 
+// Bytes of x that are zero get 0x80, the others 0 (exact per byte).
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t v) {
+  const uint32_t t = (v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+  return ~(t | v) & 0x80808080u;
+}
+
+// Four ASCII bases (byte 0 first) -> 8 code bits (first base in the top two;
+// 0 for an invalid base) and 4 mask bits (first base in bit 3; set for
+// anything but A/C/G/T/a/c/g/t).
+// Codes: ((ch >> 1) ^ (ch >> 2)) & 3 maps A/a 0, C/c 1, G/g 2, T/t 3.
+__device__ __forceinline__ void pack4(uint32_t x, uint32_t& code8, uint32_t& mask4) {
+  const uint32_t u = x & 0xDFDFDFDFu;
+  const uint32_t ok = zero_bytes(u ^ 0x41414141u) | zero_bytes(u ^ 0x43434343u) |
+                      zero_bytes(u ^ 0x47474747u) | zero_bytes(u ^ 0x54545454u);
+  const uint32_t y = ((x >> 1) ^ (x >> 2)) & ((ok >> 7) * 3u);  // invalid bases code 0
+  code8 = ((y & 3u) << 6) | (((y >> 8) & 3u) << 4) | (((y >> 16) & 3u) << 2) | ((y >> 24) & 3u);
+  const uint32_t bad = ~ok & 0x80808080u;
+  mask4 = ((bad >> 4) & 8u) | ((bad >> 13) & 4u) | ((bad >> 22) & 2u) | (bad >> 31);
+}
+
 // One thread per output word: 32 bases -> 64-bit code word + 32-bit mask.
-// A block's 256 words cover at most 8 KiB of contiguous ASCII, which the
-// block first stages in LDS with coalesced aligned 4-byte loads; the word
-// offsets of its reads are staged too, so each thread finds its read with a
-// short LDS binary search.
+// The thread reads its bases straight from the ASCII with aligned 4-byte
+// loads (only those holding at least one of its bytes, so nothing outside
+// the buffer is touched) and converts four at a time with SWAR byte tests.
+// The word offsets of the block's reads are staged in LDS so each thread
+// finds its read with a short binary search.
 constexpr int kPackReadsLds = 512;
 
 __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
@@ -322,7 +343,6 @@ __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
                                               uint64_t n_reads, uint64_t n_words, uint64_t n_bytes,
                                               uint64_t* __restrict__ codes,
                                               uint32_t* __restrict__ nmask) {
-  __shared__ uint32_t s_txt[2056];
   __shared__ uint64_t s_woff[kPackReadsLds + 1];
   const uint64_t w0 = (uint64_t)blockIdx.x * blockDim.x;
   if (w0 >= n_words) return;
@@ -335,23 +355,6 @@ __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
   const bool lds = rl - rf + 2 <= kPackReadsLds + 1;
   if (lds)
     for (uint64_t i = threadIdx.x; i < rl - rf + 2; i += blockDim.x) s_woff[i] = word_off[rf + i];
-  const uint64_t b_lo = off[rf] + (w0 - word_off[rf]) * 32;
-  const uint64_t b_hi = std::min<uint64_t>(off[rl] + (wl - word_off[rl]) * 32 + 32, off[rl + 1]);
-  // a0: b_lo rounded down to a 4-byte aligned address (may precede seq)
-  const int64_t mis = (int64_t)(reinterpret_cast<uintptr_t>(seq) & 3);
-  const int64_t a0 = (((int64_t)b_lo + mis) & ~(int64_t)3) - mis;
-  const uint32_t n4 = (uint32_t)(((int64_t)b_hi - a0 + 3) >> 2);
-  for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x) {
-    const int64_t a = a0 + 4 * (int64_t)i;
-    uint32_t v = 0;
-    if (a >= 0 && (uint64_t)a + 4 <= n_bytes) {
-      v = *reinterpret_cast<const uint32_t*>(seq + a);
-    } else {
-      for (int j = 0; j < 4; j++)
-        if (a + j >= 0 && (uint64_t)(a + j) < n_bytes) v |= (uint32_t)(uint8_t)seq[a + j] << (8 * j);
-    }
-    s_txt[i] = v;
-  }
   __syncthreads();
   const uint64_t w = w0 + threadIdx.x;
   if (w > wl) return;
@@ -365,18 +368,30 @@ __global__ __launch_bounds__(256) void k_pack(const char* __restrict__ seq,
   const uint64_t j = w - (lds ? s_woff[r - rf] : word_off[r]);
   const uint32_t L = len[r];
   const uint64_t b0 = j * 32;
-  const uint8_t* src = reinterpret_cast<const uint8_t*>(s_txt) + ((int64_t)(off[r] + b0) - a0);
   const uint32_t nb = (uint32_t)std::min<uint64_t>(32, L - b0);
+  const uintptr_t p = reinterpret_cast<uintptr_t>(seq) + off[r] + b0;
+  const uintptr_t pa = p & ~(uintptr_t)3;
+  const uint32_t sh = (uint32_t)(p & 3);
+  const uintptr_t pend = p + nb;  // <= seq + n_bytes
+  uint32_t wv[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++)
+    wv[i] = (pa + 4 * i < pend) ? *reinterpret_cast<const uint32_t*>(pa + 4 * i) : 0u;
   uint64_t code = 0;
   uint32_t mask = 0;
-  for (uint32_t i = 0; i < 32; i++) {
-    uint32_t bad = 1, c = 0;
-    if (i < nb) c = base_code(src[i], bad);
-    code = (code << 2) | c;
-    mask = (mask << 1) | bad;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    uint32_t d = __builtin_amdgcn_alignbyte(wv[i + 1], wv[i], sh);
+    const int keep = (int)nb - 4 * i;  // bytes of this dword inside the read
+    if (keep < 4) d = keep <= 0 ? 0u : (d & ((1u << (8 * keep)) - 1u));
+    uint32_t c8, m4;
+    pack4(d, c8, m4);
+    code |= (uint64_t)c8 << (56 - 8 * i);
+    mask |= m4 << (28 - 4 * i);
   }
   codes[w] = code;
   nmask[w] = mask;
+  (void)n_bytes;
 }
 
 __global__ void k_windows(const uint32_t* __restrict__ len, uint64_t n, int k,
@@ -505,11 +520,12 @@ static bool agg_enabled() {
 }
 
 // Which table passes take reads in the locality order: TK_ORDER=0 none,
-// TK_ORDER=stats the scoring pass only, TK_ORDER=1 count and scoring.
+// TK_ORDER=stats the scoring pass only, TK_ORDER=1 (default) count and
+// scoring.
 enum { kOrderNone = 0, kOrderStats = 1, kOrderAll = 2 };
 static int order_mode() {
   const char* e = getenv("TK_ORDER");
-  if (!e) return kOrderStats;
+  if (!e) return kOrderAll;
   if (e[0] == '0') return kOrderNone;
   if (e[0] == 's') return kOrderStats;
   return kOrderAll;
@@ -523,10 +539,28 @@ static uint64_t order_min_reads() {
 
 // Build the locality order of r once (read set large enough to be out of
 // cache); results never depend on it, only the memory access pattern does.
+static bool order_wanted(const tk_reads* r) {
+  return !r->order_done && order_mode() != kOrderNone && r->n_reads >= order_min_reads() &&
+         r->n_reads <= 0x7FFFFFFFull;
+}
+
+// Sort the per-read keys into r's locality order (takes ownership of both
+// buffers: keys is freed, order becomes r->d_order).
+static int order_finish(tk_ctx* ctx, tk_reads* r, uint64_t* keys, uint32_t* order) {
+  const uint64_t n = r->n_reads;
+  int rc = sort_pairs_u64_u32(ctx, keys, order, n, 32);
+  dev_free(keys, n * 8);
+  if (rc) { dev_free(order, n * 4); return rc; }
+  r->d_order = order;
+  r->bytes += n * 4;
+  r->order_done = true;
+  return TK_OK;
+}
+
 static int reads_order(tk_ctx* ctx, const tk_reads* rc_) {
   tk_reads* r = const_cast<tk_reads*>(rc_);
   if (r->order_done) return TK_OK;
-  if (order_mode() == kOrderNone || r->n_reads < order_min_reads() || r->n_reads > 0x7FFFFFFFull) {
+  if (!order_wanted(r)) {
     r->order_done = true;
     return TK_OK;
   }
@@ -542,14 +576,12 @@ static int reads_order(tk_ctx* ctx, const tk_reads* rc_) {
   hipLaunchKernelGGL(k_minimizer, dim3(grid), dim3(256), 0, ctx->stream, reads_view(r), keys, order);
   timing_end(ctx, "minimizer", a);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) rc = hip_fail(e, "k_minimizer");
-  if (!rc) rc = sort_pairs_u64_u32(ctx, keys, order, n, 32);
-  dev_free(keys, n * 8);
-  if (rc) { dev_free(order, n * 4); return rc; }
-  r->d_order = order;
-  r->bytes += n * 4;
-  r->order_done = true;
-  return TK_OK;
+  if (e != hipSuccess) {
+    dev_free(keys, n * 8);
+    dev_free(order, n * 4);
+    return hip_fail(e, "k_minimizer");
+  }
+  return order_finish(ctx, r, keys, order);
 }
 
 static int reads_windows(tk_ctx* ctx, const tk_reads* rc_, int k, uint64_t* total) {
@@ -576,34 +608,65 @@ static int reads_windows(tk_ctx* ctx, const tk_reads* rc_, int k, uint64_t* tota
 constexpr int kHllBits = 12;
 constexpr int kHllRegs = 1 << kHllBits;
 
-// Windows whose hash has its top `sbits` bits zero (a 2^-sbits sample chosen
-// by key, so every occurrence of a sampled key is sampled) update the
-// registers; the estimate is scaled by 2^sbits.  Wavefront per read,
+// HyperLogLog registers of the canonical k-mers.  Windows whose hash has its
+// top `sbits` bits zero (a 2^-sbits sample chosen by key, so every
+// occurrence of a sampled key is sampled) update the registers; the estimate
+// is scaled by 2^sbits.  Thread per read with rolling forward and
+// reverse-complement words (a window ends at base i once `run` valid bases
+// end there), so each base is read once instead of once per window;
 // block-private registers in LDS merged with one atomicMax per register.
+// When `keys`/`idx` are given, the same pass writes each read's locality
+// sort key exactly as k_minimizer does.
 __global__ __launch_bounds__(256) void k_hll(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
-                                             int canonical, uint32_t sbits,
-                                             uint32_t* __restrict__ regs) {
+                                                  int canonical, uint32_t sbits, uint32_t* __restrict__ regs,
+                                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
   __shared__ uint32_t s_reg[kHllRegs];
   for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) s_reg[i] = 0;
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const uint64_t wpb = blockDim.x >> 6;
-  for (uint64_t r = r_begin + (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); r < r_end;
-       r += (uint64_t)gridDim.x * wpb) {
+  const uint64_t kmask = (1ull << (2 * k)) - 1;  // k <= 31
+  const int rsh = 2 * k - 2;
+  for (uint64_t r = r_begin + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < r_end;
+       r += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t L = rv.len[r];
-    const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
     const uint64_t wb = rv.word_off[r];
-    for (uint32_t p = lane; p < nwin; p += 64) {
-      uint64_t fwd;
-      bool valid;
-      window_at(rv, wb, p, k, fwd, valid);
-      if (!valid) continue;
-      const uint64_t h0 = mix64(canon_key(fwd, k, canonical) ^ 0x5bd1e9955bd1e995ULL);
-      if (sbits && (h0 >> (64 - sbits))) continue;
-      const uint64_t h = h0 << sbits;
-      const uint32_t idx = (uint32_t)(h >> (64 - kHllBits));
-      const uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1 + sbits));
-      atomicMax(&s_reg[idx], (uint32_t)__clzll(rest) + 1);
+    uint64_t fwd = 0, rc = 0;
+    uint32_t run = 0, f16 = 0, r16 = 0, best = 0xFFFFFFFFu;
+    for (uint32_t w = 0; 32 * w < L; w++) {
+      const uint64_t code = rv.codes[wb + w];
+      const uint32_t msk = rv.nmask[wb + w];
+      const uint32_t nb = L - 32 * w < 32u ? L - 32 * w : 32u;
+      for (uint32_t j = 0; j < nb; j++) {
+        const uint32_t b = (uint32_t)(code >> (62 - 2 * j)) & 3u;
+        fwd = ((fwd << 2) | b) & kmask;
+        rc = (rc >> 2) | ((uint64_t)(3u - b) << rsh);
+        run = ((msk >> (31 - j)) & 1u) ? 0u : run + 1;
+        if (run >= (uint32_t)k) {
+          const uint64_t key = (canonical && rc < fwd) ? rc : fwd;
+          const uint64_t h0 = mix64(key ^ 0x5bd1e9955bd1e995ULL);
+          if (!(sbits && (h0 >> (64 - sbits)))) {
+            const uint64_t h = h0 << sbits;
+            const uint32_t ix = (uint32_t)(h >> (64 - kHllBits));
+            const uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1 + sbits));
+            atomicMax(&s_reg[ix], (uint32_t)__clzll(rest) + 1);
+          }
+        }
+        if (keys) {
+          f16 = (f16 << 2) | b;
+          r16 = (r16 >> 2) | ((3u - b) << 30);
+          if (32 * w + j >= 15) {
+            uint32_t x = f16 < r16 ? f16 : r16;
+            x *= 0x9E3779B1u;
+            x ^= x >> 16;
+            x *= 0x85EBCA6Bu;
+            x ^= x >> 13;
+            best = x < best ? x : best;
+          }
+        }
+      }
+    }
+    if (keys) {
+      keys[r] = best;
+      idx[r] = (uint32_t)r;
     }
   }
   __syncthreads();
@@ -1747,10 +1810,28 @@ static int hll_estimate(tk_ctx* c, const tk_reads* r, uint64_t rb, uint64_t re, 
   uint32_t* regs = (uint32_t*)p;
   const uint32_t sbits = windows >= (64ull << 20) ? 4 : 0;
   TK_HIP(hipMemsetAsync(regs, 0, kHllRegs * 4, c->stream));
-  if (re > rb) {
-    const int grid = grid_for_waves(c, re - rb, 4);
-    TK_LAUNCH(c, "hll", grid, 256, 0, k_hll, reads_view(r), rb, re, k, canonical, sbits, regs);
+  // a whole-read-set estimate also yields the locality sort keys
+  tk_reads* rw = const_cast<tk_reads*>(r);
+  uint64_t* keys = nullptr;
+  uint32_t* order = nullptr;
+  if (rb == 0 && re == r->n_reads && order_wanted(r)) {
+    if ((rc = dev_alloc((void**)&keys, r->n_reads * 8))) return rc;
+    if ((rc = dev_alloc((void**)&order, r->n_reads * 4))) { dev_free(keys, r->n_reads * 8); return rc; }
   }
+  if (re > rb) {
+    const uint64_t grid = std::min<uint64_t>(ceil_div(re - rb, 256), (uint64_t)std::max(c->num_cu, 1) * 16);
+    hipEvent_t a;
+    timing_begin(c, &a);
+    hipLaunchKernelGGL(k_hll, dim3(grid), dim3(256), 0, c->stream, reads_view(r), rb, re, k, canonical,
+                       sbits, regs, keys, order);
+    timing_end(c, "hll", a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      if (keys) { dev_free(keys, r->n_reads * 8); dev_free(order, r->n_reads * 4); }
+      return hip_fail(e, "k_hll");
+    }
+  }
+  if (keys && (rc = order_finish(c, rw, keys, order))) return rc;
   std::vector<uint32_t> h(kHllRegs);
   TK_HIP(hipMemcpyAsync(h.data(), regs, kHllRegs * 4, hipMemcpyDeviceToHost, c->stream));
   TK_HIP(hipStreamSynchronize(c->stream));
