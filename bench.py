@@ -36,6 +36,11 @@ READ_LEN = 150
 # stats: 0.375 B packed base + 64 B bucket line read; per read 12 B in + 32 B stats out
 COUNT_B_PER_WIN, COUNT_B_PER_READ = 128.375, 12.0
 STATS_B_PER_WIN, STATS_B_PER_READ = 64.375, 44.0
+# Memory-side bytes per window actually moved (FETCH_SIZE + WRITE_SIZE of one step over its
+# 5.04 G windows, counters calibrated against kernels of the same access shape), from
+# profiles/large_pe_20M_pairs_count_stats.md.  The counters cannot be read inside this
+# process, so the profiled figures are carried here for roofline.traffic.
+TRAFFIC_B_PER_WIN = {"count": (139.1 + 94.2) / 5.04, "stats": (248.7 + 1.3) / 5.04}
 
 
 def log(*a):
@@ -72,22 +77,36 @@ def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict,
     lib.tkm_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                             ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint64)]
     lib.tkm_run.restype = ctypes.c_int
+    lib.tkm_xcc_map.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
     nbytes = max(1 << 30, table_bytes) // 4096 * 4096
     buf = api.DeviceBuffer(ctx, nbytes)  # contents are irrelevant to the rates
     rates, ops = {}, {}
-    for kind, name in ((1, "line64_read"), (2, "line128_read"), (3, "atomic_add"), (4, "atomic_add_ret")):
+    for kind, name, nb in ((1, "line64_read", nbytes), (2, "line128_read", nbytes), (3, "atomic_add", nbytes),
+                           (4, "atomic_add_ret", nbytes), (1, "line64_read_128MB", 128 << 20),
+                           (1, "line64_read_2MB", 2 << 20), (4, "atomic_add_ret_128MB", 128 << 20)):
         best = 0.0
         for _ in range(3):
             ms, done = ctypes.c_float(), ctypes.c_uint64()
-            rc = lib.tkm_run(kind, ctypes.c_void_p(buf.ptr), nbytes, wins_per_step, ctypes.byref(ms),
+            rc = lib.tkm_run(kind, ctypes.c_void_p(buf.ptr), nb, wins_per_step, ctypes.byref(ms),
                              ctypes.byref(done))
             if rc:
                 raise RuntimeError(f"tkm_run({name}) failed: {rc}")
             best = max(best, done.value / (ms.value / 1e3))
         rates[name] = best
         ops[name] = done.value
+    # which XCD each workgroup of a launch runs on (b % 8 expected)
+    nblk = 4096
+    if lib.tkm_xcc_map(ctypes.c_void_p(buf.ptr), nblk):
+        raise RuntimeError("tkm_xcc_map failed")
+    xcc = np.empty(nblk, np.uint32)
+    buf.download(xcc)
     buf.free()
     out = {"buffer_bytes": nbytes, "ops_per_launch": ops,
+           "xcc_of_block_mod8_match": float(np.mean(xcc == (np.arange(nblk) % 8))),
+           "xcc_ids": sorted(set(int(v) for v in xcc)),
+           "line64_reads_per_s_128MB": rates["line64_read_128MB"],
+           "line64_reads_per_s_2MB": rates["line64_read_2MB"],
+           "returning_atomics_per_s_128MB": rates["atomic_add_ret_128MB"],
            "line64_reads_per_s": rates["line64_read"],
            "line128_reads_per_s": rates["line128_read"],
            "atomics_per_s": rates["atomic_add"],
@@ -208,6 +227,98 @@ def bench_components(args) -> dict:
     return out
 
 
+def bench_stress(args) -> dict:
+    """Large single sample: 150M 2x150 PE reads from 150k transcripts with
+    heavy expression skew (lognormal sigma 2.5), 1.5% substitutions.  One step:
+    pack, size, count, Inchworm prune (min count 2, entropy >= 1.5) on a
+    copy of the counts, coverage stats (min_count 2), and ReadsToTranscripts
+    against 400k contig-like sequences."""
+    from trinityrnaseq_amd import api, synth
+    n_pairs = int(args.stress_pairs)
+    n_reads = 2 * n_pairs
+    t = time.perf_counter()
+    tx = synth.Transcriptome(1005, 150_000, 1, 1200.0, 0.6, 200, 15000, share_frac=0.10,
+                             expr_sigma=2.5)
+    host = np.empty(n_reads * READ_LEN, np.uint8)
+    synth.pe_reads(tx, n_pairs, READ_LEN, seed=1005, sub_rate=0.015, out=host)
+    off = np.arange(n_reads + 1, dtype=np.uint64) * np.uint64(READ_LEN)
+    # 400k contig-like pieces: three overlapping thirds of each transcript, the
+    # first 400k of them; component = transcript index
+    pieces, comps = [], []
+    for i in range(tx.n):
+        sq = tx.seq(i)
+        L = len(sq)
+        for j in range(3):
+            a, b = j * L // 3, min(L, (j + 2) * L // 3)
+            if b - a >= K:
+                pieces.append(sq[a:b])
+                comps.append(i)
+        if len(pieces) >= 400_000:
+            break
+    pieces, comps = pieces[:400_000], np.array(comps[:400_000], np.uint32)
+    cseq, coff = api.pack_sequences(pieces)
+    log(f"generated {n_reads} reads and {len(pieces)} contigs in {time.perf_counter() - t:.1f}s")
+    ctx = api.Context(0)
+    d_seq = api.DeviceBuffer(ctx, host.nbytes + 64)
+    d_off = api.DeviceBuffer(ctx, off.nbytes)
+    d_seq.upload(host)
+    d_off.upload(off)
+    del host
+    d_stats = api.DeviceBuffer(ctx, n_reads * api.READ_STAT_DTYPE.itemsize)
+    keep = api.entropy_keep_table(K, 1.5)
+    contigs = api.Reads(ctx, cseq, coff)
+    wins = n_reads * (READ_LEN - K + 1)
+
+    def step():
+        reads = api.Reads.from_device(ctx, d_seq.ptr, d_off.ptr, n_reads)
+        est = api.estimate_distinct(ctx, reads, K, True)
+        table = api.Table(ctx, K, True, int(est * 1.05) + 4096, 0.5)
+        table.count(reads)
+        api.read_cov_stats_d(ctx, table, reads, d_stats.ptr, min_count=2)
+        ctx.sync()
+        kept = table.prune(2, keep)  # Inchworm's dictionary, after the stats used the full counts
+        ct = api.CTable(ctx, contigs, comps, K, True)
+        votes = ct.vote(reads)
+        info = table.info()
+        ct.free()
+        table.free()
+        reads.free()
+        return est, kept, info, int((votes["comp"] != 0xFFFFFFFF).sum())
+
+    est, kept, info, assigned = step()  # warm-up
+    log(f"warmup: est_distinct={est:.3e} kept={kept} capacity={info['capacity']} assigned={assigned}")
+    ctx.timing(True)
+    ctx.timing_reset()
+    lat = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        est, kept, info, assigned = step()
+        lat.append(time.perf_counter() - t)
+        log(f"step {i}: {lat[-1]:.3f}s")
+    kern = {}
+    for name in ["pack", "hll", "sort_pairs", "count", "stats", "prune", "ctable_build", "vote"]:
+        tot, n = ctx.kernel_time(name)
+        if n:
+            kern[name] = {"total_ms": round(tot, 3), "launches": n, "avg_ms": round(tot / n, 4)}
+    ms = 1000 * sum(lat) / len(lat)
+    for b in (d_seq, d_off, d_stats):
+        b.free()
+    contigs.free()
+    ctx.close()
+    return {
+        "metric": METRIC + " [stress]", "value": wins / (ms / 1000.0), "unit": "k-mers/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded transcriptome + reads generated in-process)",
+        "config": {"workload": f"stress: {n_pairs} 2x150 PE reads, 150k transcripts (expression lognormal "
+                               f"sigma 2.5), 1.5% error, k=25 canonical: count + prune (min 2, entropy 1.5) "
+                               f"+ coverage stats + R2T vs {len(pieces)} contigs",
+                   "pairs": n_pairs, "contigs": len(pieces)},
+        "reads_per_s": n_reads / (ms / 1000.0), "kernels": kern, "distinct_estimate": est,
+        "kept_after_prune": kept, "table_capacity": info["capacity"], "reads_assigned": assigned,
+    }
+
+
 def plain_job(ctx, seq: np.ndarray, off: np.ndarray, n: int, max_cov: int) -> int:
     """One sample through the plain per-job API: upload + pack, size, count,
     stats, select, selected flags back to the host, free everything."""
@@ -238,9 +349,10 @@ def bench_serving(args) -> dict:
     samples = []
     for j, n in enumerate(sizes):
         tx = synth.large_transcriptome(2000 + j, 20_000)
-        # page-locked host memory, as a server reading FASTQ into DMA buffers holds its input
-        pinned = api.HostBuffer(2 * int(n) * READ_LEN)
-        buf = pinned.array
+        # ordinary (pageable) host memory, as a client holds parsed reads; --serve-pinned puts
+        # the samples in page-locked memory instead
+        pinned = api.HostBuffer(2 * int(n) * READ_LEN) if args.serve_pinned else None
+        buf = pinned.array if pinned is not None else np.empty(2 * int(n) * READ_LEN, np.uint8)
         synth.pe_reads(tx, int(n), READ_LEN, seed=2000 + j, out=buf)
         off = np.arange(2 * int(n) + 1, dtype=np.uint64) * np.uint64(READ_LEN)
         samples.append((buf, off, int(n), pinned))
@@ -295,14 +407,16 @@ def bench_serving(args) -> dict:
     poisson_s = time.perf_counter() - t0
     srv.close()
     for s_ in samples:
-        s_[3].free()
+        if s_[3] is not None:
+            s_[3].free()
     lat = sorted(lat)
     q = lambda v, x: v[min(len(v) - 1, int(round(x * (len(v) - 1))))]
     return {
         "metric": METRIC + " [served stream]", "value": 2 * total_pairs / burst_s, "unit": "reads/s",
         "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(burst_s * 1000, 3),
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "u64",
-        "data": "synthetic (seeded samples generated in-process, in page-locked host memory)",
+        "data": "synthetic (seeded samples generated in-process, in "
+                + ("page-locked" if args.serve_pinned else "pageable") + " host memory)",
         "config": {"workload": f"serving: {args.samples} samples, read pairs log-uniform "
                                f"[{lo:.0f}, {hi:.0f}] (scale {args.serve_scale} of 0.5M-30M), 2x150 PE, "
                                f"count + coverage stats + max_cov={args.max_cov} selection per sample",
@@ -331,12 +445,15 @@ def main() -> int:
                     help="skip the random-access ceiling microbenchmarks after the timed steps")
     ap.add_argument("--copy-ref-gb", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GB (HBM counter reference)")
-    ap.add_argument("--workload", choices=["large_pe", "components", "serving"], default="large_pe")
+    ap.add_argument("--workload", choices=["large_pe", "components", "serving", "stress"], default="large_pe")
+    ap.add_argument("--stress-pairs", type=float, default=150e6)
     ap.add_argument("--components", type=int, default=10_000)
     ap.add_argument("--plain-components", type=int, default=1000)
     ap.add_argument("--samples", type=int, default=64)
     ap.add_argument("--serve-scale", type=float, default=0.1)
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--serve-pinned", action="store_true",
+                    help="serving: hold the samples in page-locked host memory")
     ap.add_argument("--hbm-budget-gb", type=float, default=96.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -348,6 +465,9 @@ def main() -> int:
         return 0
     if args.workload == "serving":
         print(json.dumps(bench_serving(args)), flush=True)
+        return 0
+    if args.workload == "stress":
+        print(json.dumps(bench_stress(args)), flush=True)
         return 0
 
     from trinityrnaseq_amd import api, synth
@@ -430,8 +550,15 @@ def main() -> int:
     roofline = None
     if dom:
         a = roof[dom]["achieved_gbs"]
+        launches = kern[dom]["launches"] / args.steps
+        traffic = (TRAFFIC_B_PER_WIN[dom] * wins_per_step / launches
+                   if dom in TRAFFIC_B_PER_WIN and n_pairs == 20_000_000 else None)
         roofline = {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                    "frac": round(a / HBM_PEAK_GBS, 4),
+                    "traffic": round(traffic) if traffic else None,
+                    "traffic_source": "profiles/large_pe_20M_pairs_count_stats.md (FETCH_SIZE + WRITE_SIZE)",
+                    "algorithmic_bytes_per_launch": round(roof[dom]["bytes_per_step"] / launches),
+                    "kernel": dom,
                     "other": {n: round(v["achieved_gbs"], 1) for n, v in roof.items()}}
 
     if roofline is not None and not args.no_ceiling:

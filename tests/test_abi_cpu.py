@@ -53,3 +53,13 @@ def test_no_device_reports_error_not_crash():
     assert rc == -2
     assert lib.tk_last_error()
     assert lib.tk_abi_version() == 1
+
+
+def test_entropy_keep_table_matches_oracle():
+    """The host-side composition table handed to tk_table_prune equals the
+    oracle's restatement of Inchworm's entropy filter."""
+    import numpy as np
+    from oracle import oracle as O
+    from trinityrnaseq_amd import api
+    for k, e in [(25, 1.5), (25, 1.0), (25, 0.0), (5, 1.0), (31, 1.7)]:
+        assert np.array_equal(api.entropy_keep_table(k, e), O.entropy_keep_table(k, e))

@@ -301,6 +301,37 @@ def test_reads_from_device_unaligned(api, ctx, shift):
     assert np.array_equal(st, O.read_stats(seq, off, ok, oc, 25, True, 1, 1, 0, 0).astype(st.dtype))
 
 
+def test_upload_pageable_pinned_device(api, ctx):
+    """Reads larger than the staging buffers (pageable source: staged in
+    chunks through pinned buffers) pack exactly as from page-locked memory
+    and from device memory."""
+    say("upload_pageable_pinned_device")
+    from trinityrnaseq_amd import synth
+    tx = synth.small_transcriptome(1001)
+    seq, off = synth.se_reads(tx, 330_001, 151, seed=77)  # ~50 MB: three and a bit 16 MB chunks
+    ok, oc = O.count(seq.tobytes(), off, 25, True)
+    pin = api.HostBuffer(seq.nbytes)
+    pin.array[:] = seq
+    dseq = api.DeviceBuffer(ctx, seq.nbytes)
+    dseq.upload(seq)
+    doff = api.DeviceBuffer(ctx, off.nbytes)
+    doff.upload(off)
+    for how in ("pageable", "pinned", "device"):
+        if how == "pageable":
+            r = api.Reads(ctx, seq, off)
+        elif how == "pinned":
+            r = api.Reads(ctx, pin.array, off)
+        else:
+            r = api.Reads.from_device(ctx, dseq.ptr, doff.ptr, len(off) - 1)
+        t = api.Table(ctx, 25, True, len(ok) + 1024)
+        t.count(r)
+        gk, gc = t.dump()
+        assert np.array_equal(gk, ok) and np.array_equal(gc, oc), how
+        t.free()
+        r.free()
+    pin.free()
+
+
 def test_pack_every_byte_value(api, ctx):
     """Every byte value 0..255 at every position of a 32-base word: only
     A/C/G/T/a/c/g/t are bases, anything else breaks the windows over it."""

@@ -230,6 +230,25 @@ class HostBuffer:
             pass
 
 
+def entropy_keep_table(k: int, min_entropy: float) -> np.ndarray:
+    """Inchworm's entropy filter as a composition table for Table.prune:
+    keep[(a*(k+1)+c)*(k+1)+g] = 1 when the Shannon entropy (bits) of a k-mer
+    with a A's, c C's, g G's and k-a-c-g T's is >= min_entropy (the rule the
+    inchworm CLI applies for --min_any_entropy)."""
+    K1 = k + 1
+    keep = np.zeros(K1 * K1 * K1, np.uint8)
+    for a in range(K1):
+        for c in range(K1 - a):
+            for g in range(K1 - a - c):
+                h = 0.0
+                for n in (a, c, g, k - a - c - g):
+                    if n > 0:
+                        p = n / k
+                        h -= p * np.log2(p)
+                keep[(a * K1 + c) * K1 + g] = h >= min_entropy
+    return keep
+
+
 def pack_sequences(seqs) -> tuple[bytes, np.ndarray]:
     """List of str/bytes -> (concatenated bytes, uint64 offsets[n+1])."""
     bs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]

@@ -57,6 +57,8 @@ struct tk_ctx {
   uint32_t* h_flags = nullptr;     // pinned mirror
   tk::Scratch scratch[4];          // reusable device scratch buffers
   int num_cu = 0;
+  void* stage[2] = {nullptr, nullptr};        // pinned staging for pageable uploads
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 struct tk_reads {

@@ -513,11 +513,6 @@ static bool coop8_enabled() {
   return e && e[0] == '1';
 }
 
-// TK_AGG=0 counts ordered reads without the block-level aggregation.
-static bool agg_enabled() {
-  const char* e = getenv("TK_AGG");
-  return !(e && e[0] == '0');
-}
 
 // Which table passes take reads in the locality order: TK_ORDER=0 none,
 // TK_ORDER=stats the scoring pass only, TK_ORDER=1 (default) count and
@@ -742,22 +737,25 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
 }
 
 // Block-aggregated count for reads taken in the locality order, where the
-// 64 reads of a block largely overlap one locus.  The block's windows go
-// kAggTile at a time into an LDS hash table (tagged key -> occurrences);
-// each distinct key then reaches the global table once, with its number of
-// occurrences, through the cooperative four-lane add.  Needs cmax >=
-// kAggTile (k <= 26) so that one add never carries more than once.
-constexpr int kAggSlots = 2048;
-constexpr int kAggTile = 1024;
-
+// 64 reads of a block largely overlap one locus.  Every device-scope atomic
+// on the table is a round trip to the memory side (the eight XCDs' L2s are
+// not coherent with each other), so the count pass is bound by the number
+// of atomics, not by bytes: the block first merges its windows, SLOTS/2 at a
+// time, in an LDS hash table of words `tagged key | occurrences << (2k+1)`
+// (the global slot format), and each distinct key then reaches the global
+// table once, with its number of occurrences, through the cooperative
+// four-lane add.  Each wavefront compacts and flushes its own quarter of
+// the LDS table.  Needs cmax >= SLOTS/2 (k <= 25 for SLOTS <= 8192), so
+// neither the LDS word nor one global add carries more than once.
+template <int SLOTS>
 __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
                                                    int canonical, TableView t, uint32_t* __restrict__ flags) {
-  __shared__ uint64_t s_key[kAggSlots];  // tagged key, 0 = empty
-  __shared__ uint32_t s_cnt[kAggSlots];
+  extern __shared__ uint64_t s_tab[];  // SLOTS words, 0 = empty
   __shared__ uint32_t s_off[kReadsPerBlock + 1];
   __shared__ uint64_t s_wb[kReadsPerBlock];
-  __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_new;
+  constexpr uint32_t kTile = SLOTS / 2;   // windows per merge: LDS load <= 1/2
+  constexpr uint32_t kQuarter = SLOTS / 4;
   const uint64_t r0 = r_begin + xcd_block(blockIdx.x, gridDim.x) * kReadsPerBlock;
   if (r0 >= r_end) return;
   const int nr = (int)std::min<uint64_t>(kReadsPerBlock, r_end - r0);
@@ -779,16 +777,15 @@ __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begi
     s_off[tid + 1] = x;
     if (tid == 0) { s_off[0] = 0; s_new = 0; }
   }
-  for (int i = tid; i < kAggSlots; i += blockDim.x) {
-    s_key[i] = 0;
-    s_cnt[i] = 0;
-  }
+  for (int i = tid; i < SLOTS; i += blockDim.x) s_tab[i] = 0;
   __syncthreads();
   const uint32_t total = s_off[nr];
+  const uint64_t one = 1ull << t.cshift;
+  uint64_t* quarter = s_tab + (uint32_t)wv * kQuarter;
   uint32_t mynew = 0;
-  for (uint32_t tile = 0; tile < total; tile += kAggTile) {
-    // 1. occurrences of the tile's windows into the LDS table (load <= 1/2)
-    const uint32_t tend = total - tile < (uint32_t)kAggTile ? total : tile + kAggTile;
+  for (uint32_t tile = 0; tile < total; tile += kTile) {
+    // 1. the tile's occurrences into the LDS table
+    const uint32_t tend = total - tile < kTile ? total : tile + kTile;
     for (uint32_t w = tile + tid; w < tend; w += blockDim.x) {
       int lo = 0, hi = nr;
       while (hi - lo > 1) {
@@ -800,70 +797,42 @@ __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begi
       window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
       if (!valid) continue;
       const uint64_t tagged = canon_key(fwd, k, canonical) | t.occ;
-      uint32_t h = (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 53) & (kAggSlots - 1);
+      uint32_t h = (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 40) & (SLOTS - 1);
       while (true) {
-        uint64_t cur = s_key[h];
-        if (cur == 0) {
-          cur = atomicCAS((unsigned long long*)&s_key[h], 0ULL, (unsigned long long)tagged);
-          if (cur == 0) cur = tagged;
+        uint64_t v = s_tab[h];
+        if (v == 0) {
+          v = atomicCAS((unsigned long long*)&s_tab[h], 0ULL, (unsigned long long)(tagged | one));
+          if (v == 0) break;
         }
-        if (cur == tagged) {
-          atomicAdd(&s_cnt[h], 1u);
+        if ((v & t.kmo) == tagged) {
+          atomicAdd((unsigned long long*)&s_tab[h], (unsigned long long)one);
           break;
         }
-        h = (h + 1) & (kAggSlots - 1);
+        h = (h + 1) & (SLOTS - 1);
       }
     }
     __syncthreads();
-    // 2. compact the occupied entries to the front (thread t owns 8 entries)
-    constexpr int kPer = kAggSlots / 256;
-    uint64_t ek[kPer];
-    uint32_t ec[kPer];
-    uint32_t nocc = 0;
-#pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      ek[j] = s_key[tid * kPer + j];
-      ec[j] = s_cnt[tid * kPer + j];
-      nocc += ek[j] != 0;
+    // 2. each wavefront compacts its quarter in place (a wavefront's LDS
+    //    reads of a batch precede its writes, and writes land at or below
+    //    the batch being read)...
+    uint32_t nd = 0;
+    for (uint32_t i0 = 0; i0 < kQuarter; i0 += 64) {
+      const uint64_t v = quarter[i0 + lane];
+      const uint64_t m = __ballot(v != 0);
+      if (v) quarter[nd + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = v;
+      nd += (uint32_t)__popcll(m);
     }
-    uint32_t incl = nocc;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(incl, o);
-      if (lane >= o) incl += y;
-    }
-    if (lane == 63) s_wsum[wv] = incl;
-    __syncthreads();
-    uint32_t base = incl - nocc, ndist = 0;
-    for (int w2 = 0; w2 < 4; w2++) {
-      const uint32_t ws = s_wsum[w2];
-      if (w2 < wv) base += ws;
-      ndist += ws;
-    }
-#pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      if (ek[j]) {
-        s_key[base] = ek[j];
-        s_cnt[base] = ec[j];
-        base++;
-      }
-    }
-    __syncthreads();
-    // 3. one global add per distinct key (wave-uniform trip count)
-    for (uint32_t e0 = (uint32_t)wv * 64; e0 < ndist; e0 += blockDim.x) {
-      const uint32_t e = e0 + lane;
-      const bool want = e < ndist;
-      const uint64_t key = want ? (s_key[e] & ~t.occ) : 0;
-      const uint64_t c = want ? s_cnt[e] : 1;
+    wave_sync();
+    // 3. ...and adds its distinct keys to the global table, 64 at a time
+    for (uint32_t e0 = 0; e0 < nd; e0 += 64) {
+      const bool want = e0 + lane < nd;
+      const uint64_t v = want ? quarter[e0 + lane] : 0;
       bool lost = false;
-      mynew += coop4_add(t, key, want, lane, &flags[0], lost, c);
+      mynew += coop4_add(t, v & (t.occ - 1), want, lane, &flags[0], lost, want ? v >> t.cshift : 1);
       if (lost) atomicOr(&flags[0], 1u);
     }
-    __syncthreads();
-    for (int i = tid; i < kAggSlots; i += blockDim.x) {
-      s_key[i] = 0;
-      s_cnt[i] = 0;
-    }
+    wave_sync();
+    for (uint32_t i = lane; i < kQuarter; i += 64) quarter[i] = 0;
     __syncthreads();
   }
   mynew = wave_sum_u32(mynew);
@@ -1363,8 +1332,12 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* vals = s_dyn + (uint64_t)wv * slab;
   uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * kLookupScratch;
-  // XCD x (= blockIdx % 8) takes positions [x*n/8, (x+1)*n/8) of the read
-  // order; its waves stride through that range together (gridDim % 8 == 0)
+  // XCD x (= blockIdx % 8, the dispatch order; checked on the device with
+  // the XCC_ID register) takes positions [x*n/8, (x+1)*n/8) of the read
+  // order, and its waves stride through that range together (gridDim % 8 ==
+  // 0), so the reads in flight on one XCD are neighbours in the order and
+  // share its L2.  A shared work counter per XCD was measured 2.8x slower:
+  // same-address atomics serialize at the memory side.
   const uint64_t per_xcd = (rv.n_reads + 7) >> 3;
   const uint64_t x = blockIdx.x & 7;
   const uint64_t lo = x * per_xcd;
@@ -1565,6 +1538,10 @@ int tk_ctx_free(tk_ctx* c) {
   for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   for (auto& s : c->scratch) dev_free(s.p, s.bytes);
+  for (int i = 0; i < 2; i++) {
+    if (c->stage[i]) (void)hipHostFree(c->stage[i]);
+    if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
+  }
   dev_free(c->d_flags, 256);
   (void)hipHostFree(c->h_flags);
   (void)hipStreamDestroy(c->stream);
@@ -1655,10 +1632,13 @@ int tk_buffer_free(tk_ctx* c, void* d) {
   return TK_OK;
 }
 
+static int upload_h2d(tk_ctx* c, void* dst, const void* src, uint64_t bytes);
+
 int tk_memcpy_h2d(tk_ctx* c, void* dst, const void* src, uint64_t bytes) {
   if (!c || (bytes && (!dst || !src))) return fail(TK_ERR_ARG, "NULL argument");
   if (set_device(c->device)) return TK_ERR_HIP;
-  if (bytes) TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  int rc = upload_h2d(c, dst, src, bytes);
+  if (rc) return rc;
   TK_HIP(hipStreamSynchronize(c->stream));
   return TK_OK;
 }
@@ -1683,6 +1663,37 @@ int tk_memcpy_d2d(tk_ctx* c, void* dst, const void* src, uint64_t bytes) {
 }
 
 // ---------------- reads ----------------
+constexpr uint64_t kStageBytes = 16ull << 20;
+
+// Host-to-device copy on the context's stream.  A page-locked source moves
+// by one DMA.  A pageable source is staged through the context's two pinned
+// buffers, chunk by chunk: the host copy of one chunk overlaps the DMA of
+// the other, and threads with contexts of their own stage in parallel.
+static int upload_h2d(tk_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!bytes) return TK_OK;
+  hipPointerAttribute_t attr;
+  bool pinned = hipPointerGetAttributes(&attr, src) == hipSuccess && attr.type == hipMemoryTypeHost;
+  (void)hipGetLastError();
+  if (pinned || bytes <= kStageBytes) {
+    TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return TK_OK;
+  }
+  for (int i = 0; i < 2; i++) {
+    if (!c->stage[i]) TK_HIP(hipHostMalloc(&c->stage[i], kStageBytes, hipHostMallocDefault));
+    if (!c->stage_ev[i]) TK_HIP(hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
+  }
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  for (uint64_t off = 0, i = 0; off < bytes; off += kStageBytes, i ^= 1) {
+    const uint64_t n = std::min<uint64_t>(kStageBytes, bytes - off);
+    TK_HIP(hipEventSynchronize(c->stage_ev[i]));  // the DMA that last read this buffer is done
+    memcpy(c->stage[i], s + off, n);
+    TK_HIP(hipMemcpyAsync(d + off, c->stage[i], n, hipMemcpyHostToDevice, c->stream));
+    TK_HIP(hipEventRecord(c->stage_ev[i], c->stream));
+  }
+  return TK_OK;
+}
+
 int tk_reads_from_host(tk_ctx* c, const char* seq, const uint64_t* offsets, uint64_t n,
                        tk_reads** out) {
   if (!c || !offsets || !out || (!seq && n && offsets[n] > offsets[0]))
@@ -1699,10 +1710,8 @@ int tk_reads_from_host(tk_ctx* c, const char* seq, const uint64_t* offsets, uint
   if (rc) return rc;
   rc = dev_alloc((void**)&d_off, (n + 1) * 8);
   if (rc) { dev_free(d_seq, nb + 64); return rc; }
-  hipError_t e = hipSuccess;
-  if (nb) e = hipMemcpyAsync(d_seq, seq, nb, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_off, offsets, (n + 1) * 8, hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) rc = hip_fail(e, "upload reads");
+  rc = upload_h2d(c, d_seq, seq, nb);
+  if (!rc) rc = upload_h2d(c, d_off, offsets, (n + 1) * 8);
   if (!rc) rc = reads_build(c, d_seq, d_off, n, nb, out);
   (void)hipStreamSynchronize(c->stream);
   dev_free(d_seq, nb + 64);
@@ -2012,6 +2021,29 @@ static int count_partitioned(tk_ctx* c, tk_table* t, const tk_reads* r, uint64_t
   return rc;
 }
 
+// LDS words of the block-level aggregation of the ordered count (0 = none).
+// Default 4096 (32 KB, windows merged 2048 at a time): four blocks per CU,
+// which the kernel's registers allow anyway; the 64 KB table merges more per
+// pass but halves the blocks per CU and measured slower.  TK_AGG=0/8192
+// overrides.
+static int agg_slots(tk_ctx* c) {
+  const char* e = getenv("TK_AGG");
+  if (e && e[0] == '0') return 0;
+  if (!e || atoi(e) != 8192) return 4096;
+  static int big = -1;
+  if (big < 0) {
+    int maxlds = 0;
+    (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    big = 0;
+    if (maxlds >= 8192 * 8 + 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) == hipSuccess)
+      big = 1;
+    (void)hipGetLastError();
+  }
+  return big ? 8192 : 4096;
+}
+
 int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
   if (!c || !t || !r) return fail(TK_ERR_ARG, "NULL argument");
   if (t->device != c->device || r->device != c->device) return fail(TK_ERR_ARG, "device mismatch");
@@ -2062,8 +2094,12 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 16, c->stream));
     const uint64_t grid = (ceil_div(nreads, kReadsPerBlock) + 7) / 8 * 8;
     TableView v = table_view(t);
-    if (rview.order && agg_enabled() && 63 - 2 * t->k >= 11)
-      TK_LAUNCH(c, "count", grid, 256, 0, k_count_agg, rview, rb, rb + nreads, t->k,
+    const int agg = rview.order && 63 - 2 * t->k >= 13 ? agg_slots(c) : 0;
+    if (agg == 8192)
+      TK_LAUNCH(c, "count", grid, 256, 8192 * 8, k_count_agg<8192>, rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags);
+    else if (agg == 4096)
+      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, k_count_agg<4096>, rview, rb, rb + nreads, t->k,
                 t->canonical, v, c->d_flags);
     else if (coop8_enabled())
       TK_LAUNCH(c, "count", grid, 256, 0, k_count<8>, rview, rb, rb + nreads, t->k,

@@ -19,6 +19,8 @@
 extern "C" {
 enum { TKM_GATHER8 = 0, TKM_GATHER64 = 1, TKM_GATHER128 = 2, TKM_ATOMIC = 3, TKM_ATOMIC_RET = 4 };
 int tkm_run(int kind, void* buf, uint64_t nbytes, uint64_t n_ops, float* ms, uint64_t* ops_done);
+// XCC (XCD) that each of `nblocks` workgroups ran on, into d_out (device, u32).
+int tkm_xcc_map(uint32_t* d_out, uint32_t nblocks);
 }
 
 namespace {
@@ -80,7 +82,18 @@ __global__ __launch_bounds__(256) void k_atomic(uint64_t* __restrict__ buf, uint
   sink[tid] = acc;
 }
 
+__global__ void k_xcc_map(uint32_t* __restrict__ out) {
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu;  // XCC_ID[3:0]
+}
+
 }  // namespace
+
+extern "C" int tkm_xcc_map(uint32_t* d_out, uint32_t nblocks) {
+  if (!d_out || nblocks == 0) return 1;
+  hipLaunchKernelGGL(k_xcc_map, dim3(nblocks), dim3(64), 0, 0, d_out);
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 4;
+}
 
 extern "C" int tkm_run(int kind, void* buf, uint64_t nbytes, uint64_t n_ops, float* ms, uint64_t* ops_done) {
   if (!buf || !ms || !ops_done || kind < 0 || kind > TKM_ATOMIC_RET || nbytes < 4096 || n_ops == 0) return 1;
