@@ -246,9 +246,12 @@ def test_select_pairs_parity(api, ctx):
         assert ns == int(want.sum())
 
 
+@pytest.mark.parametrize("ordered", [False, True])
 @pytest.mark.parametrize("canonical", [True, False])
-def test_r2t_parity(api, ctx, canonical):
-    say("r2t canonical", canonical)
+def test_r2t_parity(api, ctx, canonical, ordered, monkeypatch):
+    say("r2t canonical", canonical, "ordered", ordered)
+    if ordered:  # reads voted in the locality order even though the set is small
+        monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
     from trinityrnaseq_amd import synth
     tx = synth.large_transcriptome(77, 600)
     # bundles: components of 1-5 transcripts joined by X

@@ -1474,9 +1474,16 @@ __global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical
   const uint64_t wpb = blockDim.x >> 6;
   uint32_t* hits = s_dyn + (uint64_t)wv * slab;
   uint32_t* tmp = s_dyn + wpb * slab + wv * 64;
-  for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads;
-       r += (uint64_t)gridDim.x * wpb)
+  // reads in the locality order, one contiguous range of it per XCD (as k_stats)
+  const uint64_t per_xcd = (rv.n_reads + 7) >> 3;
+  const uint64_t x = blockIdx.x & 7;
+  const uint64_t lo = x * per_xcd;
+  const uint64_t hi = lo + per_xcd < rv.n_reads ? lo + per_xcd : rv.n_reads;
+  for (uint64_t i = lo + (uint64_t)(blockIdx.x >> 3) * wpb + wv; i < hi;
+       i += (uint64_t)(gridDim.x >> 3) * wpb) {
+    const uint64_t r = read_at(rv, i);
     vote_read(rv, r, k, canonical, slots, 0, bmask, hits, tmp, lane, out + r);
+  }
 }
 
 static int grid_for_waves(tk_ctx* ctx, uint64_t n_items, int waves_per_block) {
@@ -2549,7 +2556,9 @@ int tk_r2t_vote_reads_d(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t
   int wpb = 0;
   if (slab_layout(max_nwin, &slab, &wpb))
     return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16128 + t->k - 1);
-  int grid = grid_for_waves(c, r->n_reads, wpb);
+  int rc = reads_order(c, r);
+  if (rc) return rc;
+  const int grid = (grid_for_waves(c, r->n_reads, wpb) + 7) / 8 * 8;
   TK_LAUNCH(c, "vote", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 256, k_vote, reads_view(r), t->k,
             t->canonical, t->d_slots, t->nbuckets - 1, d_out, slab);
   return TK_OK;
