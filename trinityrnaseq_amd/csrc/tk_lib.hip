@@ -740,21 +740,23 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
 // 64 reads of a block largely overlap one locus.  Every device-scope atomic
 // on the table is a round trip to the memory side (the eight XCDs' L2s are
 // not coherent with each other), so the count pass is bound by the number
-// of atomics, not by bytes: the block first merges its windows, SLOTS/2 at a
-// time, in an LDS hash table of words `tagged key | occurrences << (2k+1)`
-// (the global slot format), and each distinct key then reaches the global
-// table once, with its number of occurrences, through the cooperative
-// four-lane add.  Each wavefront compacts and flushes its own quarter of
-// the LDS table.  Needs cmax >= SLOTS/2 (k <= 25 for SLOTS <= 8192), so
-// neither the LDS word nor one global add carries more than once.
+// of atomics, not by bytes: the block first merges its windows, 256 per
+// round, in an LDS hash table of words `tagged key | occurrences << (2k+1)`
+// (the global slot format), and only when the table could pass 3/4 load
+// (or a count kMaxRun) does each distinct key reach the global table, once,
+// with its number of occurrences, through the cooperative four-lane add.
+// Each wavefront compacts and flushes its own quarter of the LDS table.
+// Needs cmax >= kMaxRun (k <= 25), so neither the LDS word nor one global
+// add carries more than once.
 template <int SLOTS>
 __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
                                                    int canonical, TableView t, uint32_t* __restrict__ flags) {
   extern __shared__ uint64_t s_tab[];  // SLOTS words, 0 = empty
   __shared__ uint32_t s_off[kReadsPerBlock + 1];
   __shared__ uint64_t s_wb[kReadsPerBlock];
-  __shared__ uint32_t s_new;
-  constexpr uint32_t kTile = SLOTS / 2;   // windows per merge: LDS load <= 1/2
+  __shared__ uint32_t s_new, s_distinct;
+  constexpr uint32_t kMaxDistinct = SLOTS / 4 * 3;  // LDS load <= 3/4
+  constexpr uint32_t kMaxRun = 8064;                // occurrences per merge <= cmax (k <= 25)
   constexpr uint32_t kQuarter = SLOTS / 4;
   const uint64_t r0 = r_begin + xcd_block(blockIdx.x, gridDim.x) * kReadsPerBlock;
   if (r0 >= r_end) return;
@@ -778,15 +780,17 @@ __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begi
     if (tid == 0) { s_off[0] = 0; s_new = 0; }
   }
   for (int i = tid; i < SLOTS; i += blockDim.x) s_tab[i] = 0;
+  if (tid == 0) s_distinct = 0;
   __syncthreads();
   const uint32_t total = s_off[nr];
   const uint64_t one = 1ull << t.cshift;
   uint64_t* quarter = s_tab + (uint32_t)wv * kQuarter;
   uint32_t mynew = 0;
-  for (uint32_t tile = 0; tile < total; tile += kTile) {
-    // 1. the tile's occurrences into the LDS table
-    const uint32_t tend = total - tile < kTile ? total : tile + kTile;
-    for (uint32_t w = tile + tid; w < tend; w += blockDim.x) {
+  uint32_t since = 0;  // windows merged since the last flush (block-uniform)
+  for (uint32_t w0 = 0; w0 < total; w0 += 256) {
+    // 1. 256 windows into the LDS table
+    const uint32_t w = w0 + tid;
+    if (w < total) {
       int lo = 0, hi = nr;
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -795,45 +799,58 @@ __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begi
       uint64_t fwd;
       bool valid;
       window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
-      if (!valid) continue;
-      const uint64_t tagged = canon_key(fwd, k, canonical) | t.occ;
-      uint32_t h = (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 40) & (SLOTS - 1);
-      while (true) {
-        uint64_t v = s_tab[h];
-        if (v == 0) {
-          v = atomicCAS((unsigned long long*)&s_tab[h], 0ULL, (unsigned long long)(tagged | one));
-          if (v == 0) break;
+      if (valid) {
+        const uint64_t tagged = canon_key(fwd, k, canonical) | t.occ;
+        uint32_t h = (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 40) & (SLOTS - 1);
+        while (true) {
+          uint64_t v = s_tab[h];
+          if (v == 0) {
+            v = atomicCAS((unsigned long long*)&s_tab[h], 0ULL, (unsigned long long)(tagged | one));
+            if (v == 0) {
+              atomicAdd(&s_distinct, 1u);
+              break;
+            }
+          }
+          if ((v & t.kmo) == tagged) {
+            atomicAdd((unsigned long long*)&s_tab[h], (unsigned long long)one);
+            break;
+          }
+          h = (h + 1) & (SLOTS - 1);
         }
-        if ((v & t.kmo) == tagged) {
-          atomicAdd((unsigned long long*)&s_tab[h], (unsigned long long)one);
-          break;
-        }
-        h = (h + 1) & (SLOTS - 1);
       }
     }
+    since += 256;
     __syncthreads();
-    // 2. each wavefront compacts its quarter in place (a wavefront's LDS
-    //    reads of a batch precede its writes, and writes land at or below
-    //    the batch being read)...
-    uint32_t nd = 0;
-    for (uint32_t i0 = 0; i0 < kQuarter; i0 += 64) {
-      const uint64_t v = quarter[i0 + lane];
-      const uint64_t m = __ballot(v != 0);
-      if (v) quarter[nd + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = v;
-      nd += (uint32_t)__popcll(m);
+    // Flush when another round could take the table past 3/4 load or a
+    // count past kMaxRun occurrences, and after the last round.
+    const uint32_t distinct = s_distinct;
+    if (w0 + 256 >= total || distinct + 256 > kMaxDistinct || since + 256 > kMaxRun) {
+      __syncthreads();  // every thread has read s_distinct
+      if (tid == 0) s_distinct = 0;
+      // 2. each wavefront compacts its quarter in place (a wavefront's LDS
+      //    reads of a batch precede its writes, and writes land at or below
+      //    the batch being read)...
+      uint32_t nd = 0;
+      for (uint32_t i0 = 0; i0 < kQuarter; i0 += 64) {
+        const uint64_t v = quarter[i0 + lane];
+        const uint64_t m = __ballot(v != 0);
+        if (v) quarter[nd + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = v;
+        nd += (uint32_t)__popcll(m);
+      }
+      wave_sync();
+      // 3. ...and adds its distinct keys to the global table, 64 at a time
+      for (uint32_t e0 = 0; e0 < nd; e0 += 64) {
+        const bool want = e0 + lane < nd;
+        const uint64_t v = want ? quarter[e0 + lane] : 0;
+        bool lost = false;
+        mynew += coop4_add(t, v & (t.occ - 1), want, lane, &flags[0], lost, want ? v >> t.cshift : 1);
+        if (lost) atomicOr(&flags[0], 1u);
+      }
+      wave_sync();
+      for (uint32_t i = lane; i < kQuarter; i += 64) quarter[i] = 0;
+      __syncthreads();
+      since = 0;
     }
-    wave_sync();
-    // 3. ...and adds its distinct keys to the global table, 64 at a time
-    for (uint32_t e0 = 0; e0 < nd; e0 += 64) {
-      const bool want = e0 + lane < nd;
-      const uint64_t v = want ? quarter[e0 + lane] : 0;
-      bool lost = false;
-      mynew += coop4_add(t, v & (t.occ - 1), want, lane, &flags[0], lost, want ? v >> t.cshift : 1);
-      if (lost) atomicOr(&flags[0], 1u);
-    }
-    wave_sync();
-    for (uint32_t i = lane; i < kQuarter; i += 64) quarter[i] = 0;
-    __syncthreads();
   }
   mynew = wave_sum_u32(mynew);
   if (lane == 0 && mynew) atomicAdd(&s_new, mynew);
@@ -1325,7 +1342,7 @@ __global__ void k_compact(TableView src, uint64_t n_slots, uint64_t min_count, T
 // =====================================================================
 __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonical, TableView t,
                                                StatsParams sp, tk_read_stat* __restrict__ out,
-                                               uint32_t slab) {
+                                               uint32_t slab, uint32_t chunk) {
   extern __shared__ uint32_t s_dyn[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -1334,18 +1351,32 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
   uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * kLookupScratch;
   // XCD x (= blockIdx % 8, the dispatch order; checked on the device with
   // the XCC_ID register) takes positions [x*n/8, (x+1)*n/8) of the read
-  // order, and its waves stride through that range together (gridDim % 8 ==
-  // 0), so the reads in flight on one XCD are neighbours in the order and
-  // share its L2.  A shared work counter per XCD was measured 2.8x slower:
-  // same-address atomics serialize at the memory side.
+  // order (gridDim % 8 == 0).  chunk == 0: its waves stride through that
+  // range together, so the reads in flight on one XCD are neighbours.
+  // chunk > 0: each wave scores `chunk` consecutive reads one after the
+  // other, so reads of one locus reuse the lines their predecessor brought
+  // into L2 instead of missing on them at the same time.  A shared work
+  // counter per XCD was measured 2.8x slower: same-address atomics
+  // serialize at the memory side.
   const uint64_t per_xcd = (rv.n_reads + 7) >> 3;
   const uint64_t x = blockIdx.x & 7;
   const uint64_t lo = x * per_xcd;
   const uint64_t hi = lo + per_xcd < rv.n_reads ? lo + per_xcd : rv.n_reads;
-  for (uint64_t i = lo + (uint64_t)(blockIdx.x >> 3) * wpb + wv; i < hi;
-       i += (uint64_t)(gridDim.x >> 3) * wpb) {
-    const uint64_t r = read_at(rv, i);
-    stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
+  const uint64_t gw = (uint64_t)(blockIdx.x >> 3) * wpb + wv;  // wave of this XCD
+  const uint64_t nw = (uint64_t)(gridDim.x >> 3) * wpb;        // waves per XCD
+  if (chunk == 0) {
+    for (uint64_t i = lo + gw; i < hi; i += nw) {
+      const uint64_t r = read_at(rv, i);
+      stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
+    }
+  } else {
+    for (uint64_t c0 = lo + gw * chunk; c0 < hi; c0 += nw * chunk) {
+      const uint64_t ce = c0 + chunk < hi ? c0 + chunk : hi;
+      for (uint64_t i = c0; i < ce; i++) {
+        const uint64_t r = read_at(rv, i);
+        stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
+      }
+    }
   }
 }
 
@@ -1484,6 +1515,16 @@ __global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical
     const uint64_t r = read_at(rv, i);
     vote_read(rv, r, k, canonical, slots, 0, bmask, hits, tmp, lane, out + r);
   }
+}
+
+// Grid of the per-read table passes (k_stats, k_vote): at most TK_READ_BLOCKS_PER_CU
+// (default 32; 0 = one block per wpb reads) blocks per CU.
+static int per_read_grid(tk_ctx* ctx, uint64_t n_reads, int wpb) {
+  const char* e = getenv("TK_READ_BLOCKS_PER_CU");
+  const uint64_t per_cu = e ? strtoull(e, nullptr, 10) : 32;
+  const uint64_t want = ceil_div(n_reads, (uint64_t)wpb);
+  const uint64_t cap = per_cu ? (uint64_t)std::max(ctx->num_cu, 1) * per_cu : (uint64_t)0x7FFFFFF0;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
 }
 
 static int grid_for_waves(tk_ctx* ctx, uint64_t n_items, int waves_per_block) {
@@ -2393,10 +2434,12 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
   if ((rc = scoring_table(c, t, o->min_count, &use))) return rc;
   StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
   if ((rc = reads_order(c, r))) return rc;
-  const int grid = (grid_for_waves(c, r->n_reads, wpb) + 7) / 8 * 8;
+  const int grid = (per_read_grid(c, r->n_reads, wpb) + 7) / 8 * 8;
+  const char* ce = getenv("TK_STATS_CHUNK");
+  const uint32_t chunk = ce ? (uint32_t)strtoul(ce, nullptr, 10) : 0;
   TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, k_stats,
             reads_view(r), t->k,
-            t->canonical, table_view(use), sp, d_out, slab);
+            t->canonical, table_view(use), sp, d_out, slab, chunk);
   return TK_OK;
 }
 
@@ -2558,7 +2601,7 @@ int tk_r2t_vote_reads_d(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t
     return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16128 + t->k - 1);
   int rc = reads_order(c, r);
   if (rc) return rc;
-  const int grid = (grid_for_waves(c, r->n_reads, wpb) + 7) / 8 * 8;
+  const int grid = (per_read_grid(c, r->n_reads, wpb) + 7) / 8 * 8;
   TK_LAUNCH(c, "vote", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 256, k_vote, reads_view(r), t->k,
             t->canonical, t->d_slots, t->nbuckets - 1, d_out, slab);
   return TK_OK;
