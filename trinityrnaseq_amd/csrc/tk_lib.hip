@@ -471,37 +471,55 @@ static int reads_build(tk_ctx* ctx, const char* d_seq, const uint64_t* d_off, ui
   return TK_OK;
 }
 
-// Sort key of read r for the locality order of the table passes: the
-// smallest mixed hash over the read's canonical 16-mers (an N base reads as
-// A: the key only orders reads).  Reads that share it overlap one locus of
-// the transcriptome, so taken together their k-mers hit the same table lines
-// while those are still in L2 / Infinity Cache, instead of one HBM line per
-// window.  Reads shorter than 16 bases sort last.
+// Locality sort key of a read, fed its bases in order: bits 39..8 hold the
+// smallest mixed hash over its canonical 16-mers (a minimizer; an N base
+// reads as A, the key only orders reads), bits 7..0 the read's start relative
+// to that 16-mer on the 16-mer's canonical strand.  Reads that share the
+// minimizer overlap one locus of the transcriptome and sort by where they
+// start along it, so neighbours in the order share most k-mers: their table
+// lines are then reused from L2, and their windows merge before counting.
+// Reads shorter than 16 bases sort last.
+struct MinKey {
+  uint32_t f = 0, r = 0, best = 0xFFFFFFFFu, bpos = 0;
+  bool bfwd = true;
+  __device__ __forceinline__ void push(uint32_t b, uint32_t pos) {
+    f = (f << 2) | b;
+    r = (r >> 2) | ((3u - b) << 30);
+    if (pos >= 15) {
+      const bool fw = f <= r;
+      uint32_t x = fw ? f : r;
+      x *= 0x9E3779B1u;
+      x ^= x >> 16;
+      x *= 0x85EBCA6Bu;
+      x ^= x >> 13;
+      if (x < best) {
+        best = x;
+        bpos = pos - 15;
+        bfwd = fw;
+      }
+    }
+  }
+  __device__ __forceinline__ uint64_t key(uint32_t L) const {
+    uint32_t rel = bfwd ? bpos : L - 16 - bpos;
+    rel = rel > 255u ? 255u : rel;
+    return ((uint64_t)best << 8) | (255u - rel);
+  }
+};
+constexpr int kOrderKeyBits = 40;
+
 __global__ __launch_bounds__(256) void k_minimizer(ReadsView rv, uint64_t* __restrict__ keys,
                                                    uint32_t* __restrict__ idx) {
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rv.n_reads;
        r += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t L = rv.len[r];
     const uint64_t wb = rv.word_off[r];
-    uint32_t fwd = 0, rc = 0, best = 0xFFFFFFFFu;
+    MinKey mk;
     for (uint32_t w = 0; 32 * w < L; w++) {
       const uint64_t code = rv.codes[wb + w];
       const uint32_t nb = L - 32 * w < 32u ? L - 32 * w : 32u;
-      for (uint32_t j = 0; j < nb; j++) {
-        const uint32_t b = (uint32_t)(code >> (62 - 2 * j)) & 3u;
-        fwd = (fwd << 2) | b;
-        rc = (rc >> 2) | ((3u - b) << 30);
-        if (32 * w + j >= 15) {
-          uint32_t x = fwd < rc ? fwd : rc;
-          x *= 0x9E3779B1u;
-          x ^= x >> 16;
-          x *= 0x85EBCA6Bu;
-          x ^= x >> 13;
-          best = x < best ? x : best;
-        }
-      }
+      for (uint32_t j = 0; j < nb; j++) mk.push((uint32_t)(code >> (62 - 2 * j)) & 3u, 32 * w + j);
     }
-    keys[r] = best;
+    keys[r] = mk.key(L);
     idx[r] = (uint32_t)r;
   }
 }
@@ -543,7 +561,7 @@ static bool order_wanted(const tk_reads* r) {
 // buffers: keys is freed, order becomes r->d_order).
 static int order_finish(tk_ctx* ctx, tk_reads* r, uint64_t* keys, uint32_t* order) {
   const uint64_t n = r->n_reads;
-  int rc = sort_pairs_u64_u32(ctx, keys, order, n, 32);
+  int rc = sort_pairs_u64_u32(ctx, keys, order, n, kOrderKeyBits);
   dev_free(keys, n * 8);
   if (rc) { dev_free(order, n * 4); return rc; }
   r->d_order = order;
@@ -625,7 +643,8 @@ __global__ __launch_bounds__(256) void k_hll(ReadsView rv, uint64_t r_begin, uin
     const uint32_t L = rv.len[r];
     const uint64_t wb = rv.word_off[r];
     uint64_t fwd = 0, rc = 0;
-    uint32_t run = 0, f16 = 0, r16 = 0, best = 0xFFFFFFFFu;
+    uint32_t run = 0;
+    MinKey mk;
     for (uint32_t w = 0; 32 * w < L; w++) {
       const uint64_t code = rv.codes[wb + w];
       const uint32_t msk = rv.nmask[wb + w];
@@ -645,22 +664,11 @@ __global__ __launch_bounds__(256) void k_hll(ReadsView rv, uint64_t r_begin, uin
             atomicMax(&s_reg[ix], (uint32_t)__clzll(rest) + 1);
           }
         }
-        if (keys) {
-          f16 = (f16 << 2) | b;
-          r16 = (r16 >> 2) | ((3u - b) << 30);
-          if (32 * w + j >= 15) {
-            uint32_t x = f16 < r16 ? f16 : r16;
-            x *= 0x9E3779B1u;
-            x ^= x >> 16;
-            x *= 0x85EBCA6Bu;
-            x ^= x >> 13;
-            best = x < best ? x : best;
-          }
-        }
+        if (keys) mk.push(b, 32 * w + j);
       }
     }
     if (keys) {
-      keys[r] = best;
+      keys[r] = mk.key(L);
       idx[r] = (uint32_t)r;
     }
   }
@@ -1518,10 +1526,11 @@ __global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical
 }
 
 // Grid of the per-read table passes (k_stats, k_vote): at most TK_READ_BLOCKS_PER_CU
-// (default 32; 0 = one block per wpb reads) blocks per CU.
+// (default 128, the fastest of 4..128 and unbounded; 0 = one block per wpb
+// reads) blocks per CU.
 static int per_read_grid(tk_ctx* ctx, uint64_t n_reads, int wpb) {
   const char* e = getenv("TK_READ_BLOCKS_PER_CU");
-  const uint64_t per_cu = e ? strtoull(e, nullptr, 10) : 32;
+  const uint64_t per_cu = e ? strtoull(e, nullptr, 10) : 128;
   const uint64_t want = ceil_div(n_reads, (uint64_t)wpb);
   const uint64_t cap = per_cu ? (uint64_t)std::max(ctx->num_cu, 1) * per_cu : (uint64_t)0x7FFFFFF0;
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
