@@ -173,6 +173,29 @@ def test_locality_order_changes_nothing(api, ctx, small_se, monkeypatch):
         r.free()
 
 
+@pytest.mark.parametrize("cache", ["2048", "4096"])
+def test_read_stats_cached_policies(api, ctx, small_se, cache, monkeypatch):
+    """The ordered scoring pass with the block count cache (reads of at most
+    128 windows), every median/N-window policy and min_count, against the
+    oracle; N-rich and short reads mixed in."""
+    say("read_stats_cached", cache)
+    monkeypatch.setenv("TK_STATS_CACHE", cache)
+    monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
+    seq0, off0 = small_se
+    extra = edge_reads() + random_reads(5, 400, 0, 150, n_rate=0.05)
+    eseq, eoff = concat(extra)
+    seq = seq0 + eseq
+    off = np.concatenate([off0, off0[-1] + eoff[1:]])
+    r, t, _, _ = _count(api, ctx, (seq, off))
+    ok, oc = O.count(seq, off, 25, True)
+    for median_policy in (0, 1, 2):
+        for nwin_policy in (0, 1):
+            for min_count in (1, 2, 3):
+                got = api.read_cov_stats(ctx, t, r, min_count, 1, nwin_policy, median_policy)
+                want = O.read_stats(seq, off, ok, oc, 25, True, min_count, 1, nwin_policy, median_policy)
+                assert np.array_equal(got, want.astype(got.dtype)), (median_policy, nwin_policy, min_count)
+
+
 def test_read_stats_long_reads(api, ctx):
     say("read_stats_long_reads")
     seqs = random_reads(77, 40, 1000, 9000, n_rate=0.001) + repetitive_reads(78, 200)

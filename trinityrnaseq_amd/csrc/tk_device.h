@@ -387,7 +387,7 @@ __device__ __forceinline__ uint32_t nibble_bits(uint64_t x) {
 // passes the raw count field through `val` (64*NK x u64, LDS).  A key whose
 // home bucket is full without a match continues with a full probe.  out[i]
 // is the count of key i (0 = absent or pruned); lanes with want[i] = false
-// take part in the rounds and get 0.
+// take part in the rounds (their bucket is not read) and get 0.
 template <int NK>
 __device__ __forceinline__ void coop4_get(const TableView& t, const uint64_t (&key)[NK],
                                           const bool (&want)[NK], uint64_t* val, int lane,
@@ -404,7 +404,8 @@ __device__ __forceinline__ void coop4_get(const TableView& t, const uint64_t (&k
 #pragma unroll
   for (int q = 0; q < R; q++) {
     const uint64_t bq = __shfl(b[q >> 2], (q & 3) * 16 + g);
-    sv[q] = reinterpret_cast<const ulonglong2*>(t.slots + (bq << 3))[sl];
+    const int wq = __shfl((int)want[q >> 2], (q & 3) * 16 + g);
+    sv[q] = wq ? reinterpret_cast<const ulonglong2*>(t.slots + (bq << 3))[sl] : make_ulonglong2(0, 0);
   }
   uint64_t mbs[R], ebs[R];
 #pragma unroll

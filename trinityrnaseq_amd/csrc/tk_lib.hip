@@ -1388,6 +1388,36 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
   }
 }
 
+// k_stats for ordered reads of at most 128 windows each: every block takes
+// kCacheChunk consecutive reads of the order (four at a time, one per wave)
+// and keeps the counts it looked up in an LDS cache of CACHE slots shared by
+// its waves (CountCache), so a locus' k-mers are fetched from the table about
+// once per block instead of once per read.  Blocks map to XCDs as xcd_block.
+constexpr uint32_t kCacheChunk = 64;
+
+template <int CACHE>
+__global__ __launch_bounds__(256) void k_stats_cached(ReadsView rv, int k, int canonical, TableView t,
+                                                      StatsParams sp, tk_read_stat* __restrict__ out) {
+  extern __shared__ uint64_t s_lds[];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const uint32_t wpb = blockDim.x >> 6;
+  uint64_t* tmp = s_lds + CACHE + (uint32_t)wv * kLookupScratch;
+  CountCache cc{s_lds, reinterpret_cast<uint32_t*>(s_lds + CACHE + wpb * kLookupScratch), CACHE - 1};
+  for (int i = threadIdx.x; i < CACHE; i += blockDim.x) {
+    cc.keys[i] = 0;
+    cc.vals[i] = kNotReady;
+  }
+  __syncthreads();
+  const uint64_t i0 = xcd_block(blockIdx.x, gridDim.x) * kCacheChunk;
+  for (uint32_t j = wv; j < kCacheChunk; j += wpb) {
+    const uint64_t i = i0 + j;
+    if (i >= rv.n_reads) break;
+    const uint64_t r = read_at(rv, i);
+    stats_read_cached(rv, r, k, canonical, t, sp, cc, tmp, lane, out + r);
+  }
+}
+
 // =====================================================================
 // Pair selection (nbkc_merge_left_right_stats.pl + nbkc_normalize.pl)
 // =====================================================================
@@ -2423,6 +2453,18 @@ static int scoring_table(tk_ctx* c, tk_table* t, uint32_t min_count, tk_table** 
   return TK_OK;
 }
 
+// LDS count-cache slots of the ordered scoring pass: TK_STATS_CACHE = 0
+// (default), 2048 or 4096.  Off by default: on the 20M-pair workload the
+// cached pass took 116 ms (2048) and 156 ms (4096) against 78 ms without,
+// the LDS per block cutting the waves in flight more than the cache cuts
+// table reads.
+static int stats_cache_slots() {
+  const char* e = getenv("TK_STATS_CACHE");
+  if (!e) return 0;
+  const int v = atoi(e);
+  return v == 2048 ? 2048 : v == 0 ? 0 : 4096;
+}
+
 int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stats_opts* o,
                         tk_read_stat* d_out) {
   if (!c || !t || !r || !o || (r->n_reads && !d_out)) return fail(TK_ERR_ARG, "NULL argument");
@@ -2444,6 +2486,19 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
   StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
   if ((rc = reads_order(c, r))) return rc;
   const int grid = (per_read_grid(c, r->n_reads, wpb) + 7) / 8 * 8;
+  const int cache = stats_cache_slots();
+  if (cache && r->d_order && max_nwin <= 128) {
+    const uint64_t g = (ceil_div(r->n_reads, kCacheChunk) + 7) / 8 * 8;
+    if (g > 0x7FFFFFF0ull) return fail(TK_ERR_ARG, "too many reads for one scoring pass");
+    const size_t lds = (size_t)cache * 12 + 4 * 8 * kLookupScratch;
+    if (cache == 2048)
+      TK_LAUNCH(c, "stats", g, 256, lds, k_stats_cached<2048>, reads_view(r), t->k, t->canonical,
+                table_view(use), sp, d_out);
+    else
+      TK_LAUNCH(c, "stats", g, 256, lds, k_stats_cached<4096>, reads_view(r), t->k, t->canonical,
+                table_view(use), sp, d_out);
+    return TK_OK;
+  }
   const char* ce = getenv("TK_STATS_CHUNK");
   const uint32_t chunk = ce ? (uint32_t)strtoul(ce, nullptr, 10) : 0;
   TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, k_stats,

@@ -188,6 +188,116 @@ __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int ca
   wave_sync();
 }
 
+// Block-shared LDS cache of table counts (tagged key -> count), for reads
+// taken in the locality order: a read's windows are mostly its
+// predecessors', so the block looks a key up in HBM once and its later reads
+// find it here.  A slot's key is claimed by CAS and its value written after;
+// a reader that finds the key before the value (kNotReady) treats it as a
+// miss, so a lookup never returns a wrong count.  Slots are never reused.
+constexpr uint32_t kNotReady = 0xFFFFFFFFu;
+constexpr int kCacheProbe = 16;
+
+struct CountCache {
+  uint64_t* keys;   // 0 = empty
+  uint32_t* vals;   // kNotReady until written
+  uint32_t mask;    // slots - 1
+};
+
+__device__ __forceinline__ uint32_t cache_home(uint64_t tagged, uint32_t mask) {
+  return (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 40) & mask;
+}
+
+__device__ __forceinline__ bool cache_get(const CountCache& c, uint64_t tagged, uint32_t& v) {
+  uint32_t h = cache_home(tagged, c.mask);
+  for (int p = 0; p < kCacheProbe; p++) {
+    const uint64_t k = c.keys[h];
+    if (k == tagged) {
+      v = c.vals[h];
+      return v != kNotReady;
+    }
+    if (k == 0) return false;
+    h = (h + 1) & c.mask;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void cache_put(const CountCache& c, uint64_t tagged, uint32_t v) {
+  uint32_t h = cache_home(tagged, c.mask);
+  for (int p = 0; p < kCacheProbe; p++) {
+    uint64_t k = c.keys[h];
+    if (k == 0) {
+      k = atomicCAS((unsigned long long*)&c.keys[h], 0ULL, (unsigned long long)tagged);
+      if (k == 0) {
+        c.vals[h] = v;
+        return;
+      }
+    }
+    if (k == tagged) return;
+    h = (h + 1) & c.mask;
+  }
+}
+
+// stats_read for a read of at most 128 windows, with the block's count
+// cache in front of the table: only keys the cache does not hold are looked
+// up (coop4_get reads no bucket for the others), and those are added to it.
+__device__ inline void stats_read_cached(const ReadsView& rv, uint64_t r, int k, int canonical,
+                                         const TableView& t, const StatsParams& sp, const CountCache& cc,
+                                         uint64_t* tmp, int lane, tk_read_stat* out) {
+  const uint32_t L = rv.len[r];
+  const uint32_t nwin = L >= (uint32_t)k ? L - k + 1 : 0;
+  const uint64_t wb = rv.word_off[r];
+  const uint32_t pa = lane, pb = 64 + lane;
+  uint64_t fa = 0, fb = 0;
+  bool va = false, vb = false;
+  if (pa < nwin) window_at(rv, wb, pa, k, fa, va);
+  if (pb < nwin) window_at(rv, wb, pb, k, fb, vb);
+  const uint64_t keys[2] = {va ? canon_key(fa, k, canonical) : 0, vb ? canon_key(fb, k, canonical) : 0};
+  uint32_t cva = 0, cvb = 0;
+  const bool hita = va && cache_get(cc, keys[0] | t.occ, cva);
+  const bool hitb = vb && cache_get(cc, keys[1] | t.occ, cvb);
+  const bool want[2] = {va && !hita, vb && !hitb};
+  uint64_t c[2];
+  coop4_get<2>(t, keys, want, tmp, lane, c);
+  if (want[0] && c[0] < kNotReady) cache_put(cc, keys[0] | t.occ, (uint32_t)c[0]);
+  if (want[1] && c[1] < kNotReady) cache_put(cc, keys[1] | t.occ, (uint32_t)c[1]);
+  if (hita) c[0] = cva;
+  if (hitb) c[1] = cvb;
+  bool ha, hb_;
+  uint32_t xa, xb;
+  window_value(sp, pa < nwin, va, c[0], ha, xa);
+  window_value(sp, pb < nwin, vb, c[1], hb_, xb);
+  const uint32_t n = (uint32_t)__popcll(__ballot(ha)) + (uint32_t)__popcll(__ballot(hb_));
+  uint64_t sum = (uint64_t)xa + xb, sqlo = 0, sqhi = 0;
+  sq_add(xa, sqlo, sqhi);
+  sq_add(xb, sqlo, sqhi);
+  sum = wave_sum_u64(sum);
+  wave_sum_u128(sqlo, sqhi);
+  uint32_t med = 0;
+  if (n) {
+    const uint32_t vmax = wave_max_u32(xa > xb ? xa : xb);
+    const int hb = 31 - __clz(vmax | 1u);
+    if (sp.median_policy == TK_MEDIAN_LOWER) {
+      med = wave_select2(ha, xa, hb_, xb, (n - 1) / 2, hb);
+    } else if (sp.median_policy == TK_MEDIAN_MEAN_FLOOR && !(n & 1)) {
+      const uint64_t a = wave_select2(ha, xa, hb_, xb, n / 2 - 1, hb);
+      const uint64_t b = wave_select2(ha, xa, hb_, xb, n / 2, hb);
+      med = (uint32_t)((a + b) / 2);
+    } else {
+      med = wave_select2(ha, xa, hb_, xb, n / 2, hb);
+    }
+  }
+  if (lane == 0) {
+    tk_read_stat st;
+    st.median = med;
+    st.nwin = n;
+    st.sum = sum;
+    st.sumsq_lo = sqlo;
+    st.sumsq_hi = sqhi;
+    *out = st;
+  }
+  wave_sync();
+}
+
 // Component table: 64-byte buckets of four 16-byte slots (tagged key, ~comp);
 // buckets [boff, boff + bmask] belong to one table.
 __device__ inline void ctable_add(uint64_t* slots, uint64_t boff, uint64_t bmask, uint64_t occ,
