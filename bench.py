@@ -83,7 +83,8 @@ def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict,
     rates, ops = {}, {}
     for kind, name, nb in ((1, "line64_read", nbytes), (2, "line128_read", nbytes), (3, "atomic_add", nbytes),
                            (4, "atomic_add_ret", nbytes), (1, "line64_read_128MB", 128 << 20),
-                           (1, "line64_read_2MB", 2 << 20), (4, "atomic_add_ret_128MB", 128 << 20)):
+                           (1, "line64_read_2MB", 2 << 20), (4, "atomic_add_ret_128MB", 128 << 20),
+                           (1, "line64_read_1GB", 1 << 30), (1, "line64_read_4GB", min(nbytes, 4 << 30))):
         best = 0.0
         for _ in range(3):
             ms, done = ctypes.c_float(), ctypes.c_uint64()
@@ -107,6 +108,8 @@ def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict,
            "line64_reads_per_s_128MB": rates["line64_read_128MB"],
            "line64_reads_per_s_2MB": rates["line64_read_2MB"],
            "returning_atomics_per_s_128MB": rates["atomic_add_ret_128MB"],
+           "line64_reads_per_s_1GB": rates["line64_read_1GB"],
+           "line64_reads_per_s_4GB": rates["line64_read_4GB"],
            "line64_reads_per_s": rates["line64_read"],
            "line128_reads_per_s": rates["line128_read"],
            "atomics_per_s": rates["atomic_add"],
@@ -499,6 +502,8 @@ def main() -> int:
         a_.free()
         b_.free()
 
+    exact = {"want": True}
+
     def job():
         reads = api.Reads.from_device(ctx, d_seq.ptr, d_off.ptr, n_reads)
         est = api.estimate_distinct(ctx, reads, K, True)
@@ -509,6 +514,8 @@ def main() -> int:
                                   args.max_cov, 1, 10000, READ_LEN - K + 1, d_sel.ptr)
         ctx.sync()
         info = table.info()
+        if exact["want"]:  # warm-up only, never inside the timed steps
+            exact["distinct"] = table.dump_size()
         table.free()
         reads.free()
         return nsel, est, info
@@ -518,6 +525,7 @@ def main() -> int:
         nsel, est, info = job()
         log(f"warmup {i}: {time.perf_counter() - t:.3f}s selected={nsel} est_distinct={est:.3e} "
             f"capacity={info['capacity']}")
+    exact["want"] = False
     ctx.timing(True)
     ctx.timing_reset()
     lat = []
@@ -582,7 +590,7 @@ def main() -> int:
         "reads_per_s": n_reads / (ms / 1000.0),
         "latency_s": {"p50": p(0.5), "p99": p(0.99), "min": lat_sorted[0], "max": lat_sorted[-1]},
         "kernels": kern, "roofline": roofline, "cpu_baseline": cpu,
-        "selected_pairs": nsel, "distinct_estimate": est,
+        "selected_pairs": nsel, "distinct_estimate": est, "distinct_exact": exact.get("distinct"),
     }
     print(json.dumps(out), flush=True)
     for b in (d_seq, d_off, d_stats, d_sel):

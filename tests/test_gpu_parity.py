@@ -137,6 +137,26 @@ def test_hll_estimate(api, ctx, small_se):
     assert abs(est - len(ok)) / len(ok) < 0.08
 
 
+def test_hll_estimate_sampled(api, ctx):
+    """Above 64M windows the estimate samples 1/16 of the keys by hash; it
+    stays within 4% of the exact distinct count (and the table built from it
+    counts exactly)."""
+    say("hll_estimate_sampled")
+    from trinityrnaseq_amd import synth
+    tx = synth.large_transcriptome(1003, 5000)
+    seq, off = synth.se_reads(tx, 560_000, 150, seed=9, sub_rate=0.01)
+    seq = seq.tobytes()
+    r = api.Reads(ctx, seq, off)
+    assert r.windows(25) >= (64 << 20)
+    est = api.estimate_distinct(ctx, r, 25, True)
+    ok, oc = O.count(seq, off, 25, True)
+    assert abs(est - len(ok)) / len(ok) < 0.04, (est, len(ok))
+    t = api.Table(ctx, 25, True, int(est * 1.05) + 4096, 0.5)
+    t.count(r)
+    gk, gc = t.dump()
+    assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+
+
 @pytest.mark.parametrize("median_policy", [0, 1, 2])
 @pytest.mark.parametrize("nwin_policy", [0, 1])
 def test_read_stats_parity(api, ctx, small_se, median_policy, nwin_policy, monkeypatch):

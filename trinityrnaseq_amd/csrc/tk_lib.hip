@@ -624,7 +624,8 @@ constexpr int kHllRegs = 1 << kHllBits;
 // HyperLogLog registers of the canonical k-mers.  Windows whose hash has its
 // top `sbits` bits zero (a 2^-sbits sample chosen by key, so every
 // occurrence of a sampled key is sampled) update the registers; the estimate
-// is scaled by 2^sbits.  Thread per read with rolling forward and
+// is scaled by 2^sbits.  (Sampling by a multiplicative hash of the key
+// instead was no faster and biased the estimate by +5.5%.)  Thread per read with rolling forward and
 // reverse-complement words (a window ends at base i once `run` valid bases
 // end there), so each base is read once instead of once per window;
 // block-private registers in LDS merged with one atomicMax per register.
