@@ -1349,9 +1349,9 @@ __global__ void k_compact(TableView src, uint64_t n_slots, uint64_t min_count, T
 // =====================================================================
 // Per-read coverage statistics: one wavefront per read
 // =====================================================================
-__global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonical, TableView t,
-                                               StatsParams sp, tk_read_stat* __restrict__ out,
-                                               uint32_t slab, uint32_t chunk) {
+template <bool kShort>
+__device__ __forceinline__ void stats_pass(ReadsView rv, int k, int canonical, TableView t, StatsParams sp,
+                                           tk_read_stat* __restrict__ out, uint32_t slab) {
   extern __shared__ uint32_t s_dyn[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -1360,33 +1360,35 @@ __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonica
   uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * kLookupScratch;
   // XCD x (= blockIdx % 8, the dispatch order; checked on the device with
   // the XCC_ID register) takes positions [x*n/8, (x+1)*n/8) of the read
-  // order (gridDim % 8 == 0).  chunk == 0: its waves stride through that
-  // range together, so the reads in flight on one XCD are neighbours.
-  // chunk > 0: each wave scores `chunk` consecutive reads one after the
-  // other, so reads of one locus reuse the lines their predecessor brought
-  // into L2 instead of missing on them at the same time.  A shared work
-  // counter per XCD was measured 2.8x slower: same-address atomics
-  // serialize at the memory side.
+  // order (gridDim % 8 == 0) and its waves stride through that range
+  // together, so the reads in flight on one XCD are neighbours in the order
+  // and share its L2.  Measured slower: each wave scoring a run of
+  // consecutive reads by itself (fewer simultaneous misses on one locus, but
+  // too few reads in flight once the runs fit L2), and a shared work counter
+  // per XCD (same-address atomics serialize at the memory side).
   const uint64_t per_xcd = (rv.n_reads + 7) >> 3;
   const uint64_t x = blockIdx.x & 7;
   const uint64_t lo = x * per_xcd;
   const uint64_t hi = lo + per_xcd < rv.n_reads ? lo + per_xcd : rv.n_reads;
-  const uint64_t gw = (uint64_t)(blockIdx.x >> 3) * wpb + wv;  // wave of this XCD
-  const uint64_t nw = (uint64_t)(gridDim.x >> 3) * wpb;        // waves per XCD
-  if (chunk == 0) {
-    for (uint64_t i = lo + gw; i < hi; i += nw) {
-      const uint64_t r = read_at(rv, i);
-      stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
-    }
-  } else {
-    for (uint64_t c0 = lo + gw * chunk; c0 < hi; c0 += nw * chunk) {
-      const uint64_t ce = c0 + chunk < hi ? c0 + chunk : hi;
-      for (uint64_t i = c0; i < ce; i++) {
-        const uint64_t r = read_at(rv, i);
-        stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
-      }
-    }
+  for (uint64_t i = lo + (uint64_t)(blockIdx.x >> 3) * wpb + wv; i < hi;
+       i += (uint64_t)(gridDim.x >> 3) * wpb) {
+    const uint64_t r = read_at(rv, i);
+    stats_read<kShort>(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
   }
+}
+
+// The scoring pass for any read lengths, and for read sets whose reads all
+// have at most 128 windows (no long-read path compiled: 77 instead of 86
+// VGPRs, 6 waves per SIMD instead of 5).
+__global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonical, TableView t,
+                                               StatsParams sp, tk_read_stat* __restrict__ out,
+                                               uint32_t slab) {
+  stats_pass<false>(rv, k, canonical, t, sp, out, slab);
+}
+__global__ __launch_bounds__(256) void k_stats_short(ReadsView rv, int k, int canonical, TableView t,
+                                                     StatsParams sp, tk_read_stat* __restrict__ out,
+                                                     uint32_t slab) {
+  stats_pass<true>(rv, k, canonical, t, sp, out, slab);
 }
 
 // k_stats for ordered reads of at most 128 windows each: every block takes
@@ -2500,11 +2502,13 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
                 table_view(use), sp, d_out);
     return TK_OK;
   }
-  const char* ce = getenv("TK_STATS_CHUNK");
-  const uint32_t chunk = ce ? (uint32_t)strtoul(ce, nullptr, 10) : 0;
-  TK_LAUNCH(c, "stats", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, k_stats,
-            reads_view(r), t->k,
-            t->canonical, table_view(use), sp, d_out, slab, chunk);
+  const size_t lds = (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch;
+  if (max_nwin <= 128)
+    TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats_short, reads_view(r), t->k, t->canonical,
+              table_view(use), sp, d_out, slab);
+  else
+    TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats, reads_view(r), t->k, t->canonical, table_view(use), sp,
+              d_out, slab);
   return TK_OK;
 }
 

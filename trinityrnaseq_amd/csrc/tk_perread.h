@@ -90,6 +90,9 @@ __device__ __forceinline__ void wave_sum_u128(uint64_t& lo, uint64_t& hi) {
 // in registers (two per lane); longer reads stage them in `vals` (LDS, >=
 // nwin entries) 128 windows at a time.  tmp: 128 x u64 LDS scratch of the
 // cooperative lookup.  Lane 0 writes *out.
+// kShortOnly: the caller guarantees nwin <= 128 (the long-read path is not
+// compiled, which frees registers).
+template <bool kShortOnly = false>
 __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int canonical,
                                   const TableView& t, const StatsParams& sp, uint32_t* vals,
                                   uint64_t* tmp, int lane, tk_read_stat* out) {
@@ -98,7 +101,7 @@ __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int ca
   const uint64_t wb = rv.word_off[r];
   uint32_t n = 0, vmax = 0, med = 0;
   uint64_t sum = 0, sqlo = 0, sqhi = 0;
-  if (nwin <= 128) {
+  if (kShortOnly || nwin <= 128) {
     const uint32_t pa = lane, pb = 64 + lane;
     uint64_t fa = 0, fb = 0;
     bool va = false, vb = false;
