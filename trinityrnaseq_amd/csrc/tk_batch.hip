@@ -142,6 +142,8 @@ __global__ void k_batch_fold(TableView base, const CompLayout* __restrict__ lay,
   }
 }
 
+// kShort: every read has at most 128 windows (the long-read path is not compiled).
+template <bool kShort>
 __global__ __launch_bounds__(256) void k_batch_stats(ReadsView rv, const uint32_t* __restrict__ read_comp,
                                                      const CompLayout* __restrict__ lay, int k,
                                                      int canonical, TableView base, StatsParams sp,
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(256) void k_batch_stats(ReadsView rv, const uint32_
   uint64_t* tmp = reinterpret_cast<uint64_t*>(s_dyn + wpb * slab) + wv * kLookupScratch;
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads; r += (uint64_t)gridDim.x * wpb) {
     const TableView t = comp_view(base, lay[read_comp[r]]);
-    stats_read(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
+    stats_read<kShort>(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
   }
 }
 
@@ -337,9 +339,14 @@ static int batch_enqueue(tk_ctx* c, tk_batch* b, const tk_stats_opts* o, tk_read
   while (wpb > 1 && ((uint64_t)slab * 4 + 8 * kLookupScratch) * wpb > 65536) wpb >>= 1;
   if (d_stats && rv.n_reads) {
     StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
-    hipLaunchKernelGGL(k_batch_stats, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb),
-                       (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, c->stream, rv, b->d_read_comp, b->d_lay,
-                       b->k, b->canonical, base, sp, d_stats, slab);
+    if (b->max_nwin <= 128)
+      hipLaunchKernelGGL(k_batch_stats<true>, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb),
+                         (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, c->stream, rv, b->d_read_comp,
+                         b->d_lay, b->k, b->canonical, base, sp, d_stats, slab);
+    else
+      hipLaunchKernelGGL(k_batch_stats<false>, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb),
+                         (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch, c->stream, rv, b->d_read_comp,
+                         b->d_lay, b->k, b->canonical, base, sp, d_stats, slab);
   }
   if (d_votes && rv.n_reads) {
     hipLaunchKernelGGL(k_fill_zero, dim3(grid(b->n_ct_words, 256)), dim3(256), 0, c->stream, b->d_ct, b->n_ct_words);
