@@ -453,7 +453,9 @@ def main() -> int:
     ap.add_argument("--components", type=int, default=10_000)
     ap.add_argument("--plain-components", type=int, default=1000)
     ap.add_argument("--samples", type=int, default=64)
-    ap.add_argument("--serve-scale", type=float, default=0.1)
+    ap.add_argument("--serve-scale", type=float, default=1.0,
+                    help="serving: sample sizes as a fraction of 0.5M-30M pairs (1.0 = the configured "
+                         "stream, ~140 GB of host memory; 0.1 for a quick run)")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--serve-pinned", action="store_true",
                     help="serving: hold the samples in page-locked host memory")
