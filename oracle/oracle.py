@@ -186,3 +186,49 @@ def r2t_vote(seq: bytes, off, keys, comps, k: int = 25, canonical: bool = True) 
                             _p(keys) if len(keys) else None, _p(comps) if len(comps) else None,
                             len(keys), _p(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Multi-threaded CPU restatement of count + coverage stats (tk_cpuref.cpp):
+# the CPU baseline bench.py times on the host cores; checked against the
+# single-threaded functions above by tests/test_oracle.py.
+_cpuref = None
+
+
+def cpuref() -> C.CDLL:
+    global _cpuref
+    if _cpuref is None:
+        so = HERE / "build" / "libtk_cpuref.so"
+        if not so.exists():
+            build()
+        l = C.CDLL(str(so))
+        l.cr_count_stats.restype = C.c_int
+        l.cr_count_stats.argtypes = [C.c_char_p, _P, C.c_uint64, C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                     C.c_uint64, _P, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_uint64)]
+        _cpuref = l
+    return _cpuref
+
+
+def cpuref_count_stats(seq, off, k: int = 25, canonical: bool = True, min_count: int = 2,
+                       threads: int = 1, expected_distinct: int = 0) -> dict:
+    """Count, then per-read (median, nwin, sum) with missing = 1, N-windows
+    counted as missing and the upper median; also the wall-clock seconds of
+    the two passes and the distinct-key count."""
+    off = _off(off)
+    n = len(off) - 1
+    med = np.zeros(max(n, 1), np.uint32)
+    nwin = np.zeros(max(n, 1), np.uint32)
+    sm = np.zeros(max(n, 1), np.uint64)
+    tc, ts, nd = C.c_double(), C.c_double(), C.c_uint64()
+    if isinstance(seq, np.ndarray):
+        sp = C.cast(C.c_void_p(seq.ctypes.data), C.c_char_p)
+    else:
+        sp = seq
+    exp = expected_distinct or int(off[-1]) + 1024
+    rc = cpuref().cr_count_stats(sp, _p(off), n, k, int(canonical), min_count, threads, exp, _p(med),
+                                 _p(nwin), _p(sm), C.byref(tc), C.byref(ts), C.byref(nd))
+    if rc:
+        raise MemoryError("cr_count_stats: out of memory")
+    return {"median": med[:n], "nwin": nwin[:n], "sum": sm[:n], "count_s": tc.value,
+            "stats_s": ts.value, "distinct": nd.value}

@@ -198,3 +198,20 @@ def test_r2t_matches_brute_force(canonical):
     got = O.r2t_vote(rseq, roff, keys, cv, k, canonical)
     want = brute.r2t(bundles, comps, reads, k, canonical)
     assert [(int(g["comp"]), int(g["hits"]), int(g["nwin"])) for g in got] == want
+
+
+def test_cpuref_matches_oracle():
+    """The multi-threaded CPU baseline computes what the oracle computes."""
+    import numpy as np
+    from data import concat, edge_reads, random_reads
+    from oracle import oracle as O
+    seqs = random_reads(31, 3000, 0, 300, n_rate=0.01) + edge_reads()
+    seq, off = concat(seqs)
+    ok, oc = O.count(seq, off, 25, True)
+    want = O.read_stats(seq, off, ok, oc, 25, True, 2, 1, 0, 0)
+    for threads in (1, 4):
+        got = O.cpuref_count_stats(seq, off, 25, True, 2, threads)
+        assert got["distinct"] == len(ok)
+        assert np.array_equal(got["median"], want["median"])
+        assert np.array_equal(got["nwin"], want["nwin"])
+        assert np.array_equal(got["sum"], want["sum"])
