@@ -11,13 +11,20 @@
 //   TKM_GATHER128  sixteen lanes read one random 128-byte line
 //   TKM_ATOMIC     one 8-byte atomicAdd per lane, result unused
 //   TKM_ATOMIC_RET one 8-byte atomicAdd per lane, result consumed
+//   TKM_CAS        one 8-byte atomicCAS per lane (compare value almost never
+//                  matches), result consumed
+//   TKM_ATOMIC_WG  one 8-byte atomicAdd per lane at workgroup scope, result
+//                  consumed: performed in the XCD's L2 instead of at the
+//                  memory side (the buffer's contents are meaningless after
+//                  it; only the rate is measured)
 //
 // Benchmark infrastructure: not part of the k-mer path's C ABI.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 extern "C" {
-enum { TKM_GATHER8 = 0, TKM_GATHER64 = 1, TKM_GATHER128 = 2, TKM_ATOMIC = 3, TKM_ATOMIC_RET = 4 };
+enum { TKM_GATHER8 = 0, TKM_GATHER64 = 1, TKM_GATHER128 = 2, TKM_ATOMIC = 3, TKM_ATOMIC_RET = 4, TKM_CAS = 5,
+       TKM_ATOMIC_WG = 6 };
 int tkm_run(int kind, void* buf, uint64_t nbytes, uint64_t n_ops, float* ms, uint64_t* ops_done);
 // XCC (XCD) that each of `nblocks` workgroups ran on, into d_out (device, u32).
 int tkm_xcc_map(uint32_t* d_out, uint32_t nblocks);
@@ -82,6 +89,29 @@ __global__ __launch_bounds__(256) void k_atomic(uint64_t* __restrict__ buf, uint
   sink[tid] = acc;
 }
 
+template <int KIND>
+__global__ __launch_bounds__(256) void k_atomic2(uint64_t* __restrict__ buf, uint64_t nslots, uint64_t iters,
+                                                 uint64_t* __restrict__ sink) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < iters; i += kUnroll) {
+    uint64_t r[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint64_t x = mix((tid * iters + i + u) * 0x9E3779B97F4A7C15ULL + 1);
+      const uint64_t s = __umul64hi(x, nslots);
+      if (KIND == TKM_CAS)
+        r[u] = atomicCAS((unsigned long long*)&buf[s], (unsigned long long)(x | 1ull), 0ull);
+      else
+        r[u] = __hip_atomic_fetch_add((unsigned long long*)&buf[s], 1ull, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) acc ^= r[u];
+  }
+  sink[tid] = acc;
+}
+
 __global__ void k_xcc_map(uint32_t* __restrict__ out) {
   if (threadIdx.x == 0)
     out[blockIdx.x] = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu;  // XCC_ID[3:0]
@@ -96,7 +126,7 @@ extern "C" int tkm_xcc_map(uint32_t* d_out, uint32_t nblocks) {
 }
 
 extern "C" int tkm_run(int kind, void* buf, uint64_t nbytes, uint64_t n_ops, float* ms, uint64_t* ops_done) {
-  if (!buf || !ms || !ops_done || kind < 0 || kind > TKM_ATOMIC_RET || nbytes < 4096 || n_ops == 0) return 1;
+  if (!buf || !ms || !ops_done || kind < 0 || kind > TKM_ATOMIC_WG || nbytes < 4096 || n_ops == 0) return 1;
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -128,8 +158,14 @@ extern "C" int tkm_run(int kind, void* buf, uint64_t nbytes, uint64_t n_ops, flo
     case TKM_ATOMIC:
       hipLaunchKernelGGL(k_atomic<false>, dim3(grid), dim3(block), 0, 0, p, nbytes / 8, iters, sink);
       break;
-    default:
+    case TKM_ATOMIC_RET:
       hipLaunchKernelGGL(k_atomic<true>, dim3(grid), dim3(block), 0, 0, p, nbytes / 8, iters, sink);
+      break;
+    case TKM_CAS:
+      hipLaunchKernelGGL(k_atomic2<TKM_CAS>, dim3(grid), dim3(block), 0, 0, p, nbytes / 8, iters, sink);
+      break;
+    default:
+      hipLaunchKernelGGL(k_atomic2<TKM_ATOMIC_WG>, dim3(grid), dim3(block), 0, 0, p, nbytes / 8, iters, sink);
       break;
   }
   (void)hipEventRecord(b, 0);
