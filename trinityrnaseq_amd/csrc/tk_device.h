@@ -31,7 +31,7 @@ namespace tk {
 
 constexpr int kWave = 64;
 constexpr int kBucketSlots = 8;
-// u64 entries of LDS lookup scratch per wavefront (coop_get128)
+// u64 entries of LDS lookup scratch per wavefront (coop4_get with two keys per lane)
 constexpr int kLookupScratch = 128;
 
 __host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
@@ -276,96 +276,6 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
   return v;
 }
 
-// Cooperative lookup of the 64 keys held one per lane of a wavefront.  In
-// round q the eight lanes 8g..8g+7 read the eight slots of the bucket of the
-// key held by lane 8q+g: one coalesced 64-byte request per key, and eight
-// independent loads in flight per lane.  Matching lanes deposit the raw
-// field in tmp (64 x u64, LDS); a key whose home bucket is full without a
-// match continues with a full probe (rare at load <= 0.5).  Returns the
-// count of the calling lane's key (0 = absent or pruned).  `want` = false
-// lanes take part in the rounds but skip the follow-up probe.
-__device__ __forceinline__ uint64_t coop_get64(const TableView& t, uint64_t key, bool want,
-                                               uint64_t* tmp, int lane) {
-  const uint64_t tagged = key | t.occ;
-  const uint64_t b = bucket0(t, key);
-  const int g = lane >> 3, sl = lane & 7;
-  tmp[lane] = 0;
-  uint64_t sv[8];
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const uint64_t bq = __shfl(b, q * 8 + g);
-    sv[q] = t.slots[(bq << 3) + sl];
-  }
-  wave_sync();
-  uint64_t pending = 0;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const uint64_t tq = __shfl(tagged, q * 8 + g);
-    const bool m = (sv[q] & t.kmo) == tq;
-    if (m) tmp[q * 8 + g] = sv[q] >> t.cshift;
-    const uint32_t gm = byte_any8(__ballot(m));
-    const uint32_t ge = byte_any8(__ballot(sv[q] == 0));
-    pending |= (uint64_t)(~gm & ~ge & 0xFFu) << (8 * q);
-  }
-  wave_sync();
-  uint64_t f = tmp[lane];
-  if (want) {
-    if ((pending >> lane) & 1ull) {
-      f = table_get(t, key);
-    } else if (f == t.cmax) {
-      const uint64_t* o = ovf_find(t, tagged);
-      f = o ? *o : f;
-    }
-  }
-  wave_sync();
-  return f;
-}
-
-// coop_get64 for two keys per lane (128 windows): all sixteen bucket loads
-// are issued before any is resolved, so a 150-nt read costs one memory round
-// trip instead of two.  tmp: 128 x u64 LDS scratch.
-__device__ __forceinline__ void coop_get128(const TableView& t, uint64_t keyA, bool wantA,
-                                            uint64_t keyB, bool wantB, uint64_t* tmp, int lane,
-                                            uint64_t& cA, uint64_t& cB) {
-  const uint64_t tagA = keyA | t.occ, tagB = keyB | t.occ;
-  const uint64_t bA = bucket0(t, keyA), bB = bucket0(t, keyB);
-  const int g = lane >> 3, sl = lane & 7;
-  tmp[lane] = 0;
-  tmp[64 + lane] = 0;
-  uint64_t sa[8], sb[8];
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    sa[q] = t.slots[(__shfl(bA, q * 8 + g) << 3) + sl];
-    sb[q] = t.slots[(__shfl(bB, q * 8 + g) << 3) + sl];
-  }
-  wave_sync();
-  uint64_t pendA = 0, pendB = 0;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const uint64_t ta = __shfl(tagA, q * 8 + g);
-    const uint64_t tb = __shfl(tagB, q * 8 + g);
-    const bool ma = (sa[q] & t.kmo) == ta;
-    const bool mb = (sb[q] & t.kmo) == tb;
-    if (ma) tmp[q * 8 + g] = sa[q] >> t.cshift;
-    if (mb) tmp[64 + q * 8 + g] = sb[q] >> t.cshift;
-    pendA |= (uint64_t)(~byte_any8(__ballot(ma)) & ~byte_any8(__ballot(sa[q] == 0)) & 0xFFu) << (8 * q);
-    pendB |= (uint64_t)(~byte_any8(__ballot(mb)) & ~byte_any8(__ballot(sb[q] == 0)) & 0xFFu) << (8 * q);
-  }
-  wave_sync();
-  uint64_t fa = tmp[lane], fb = tmp[64 + lane];
-  if (wantA) {
-    if ((pendA >> lane) & 1ull) fa = table_get(t, keyA);
-    else if (fa == t.cmax) { const uint64_t* o = ovf_find(t, tagA); fa = o ? *o : fa; }
-  }
-  if (wantB) {
-    if ((pendB >> lane) & 1ull) fb = table_get(t, keyB);
-    else if (fb == t.cmax) { const uint64_t* o = ovf_find(t, tagB); fb = o ? *o : fb; }
-  }
-  wave_sync();
-  cA = fa;
-  cB = fb;
-}
-
 // Bit g of the result is set when nibble g (lanes 4g..4g+3) of x is non-zero.
 __device__ __forceinline__ uint32_t nibble_bits(uint64_t x) {
   x |= x >> 1;
@@ -443,7 +353,9 @@ __device__ __forceinline__ void coop4_get(const TableView& t, const uint64_t (&k
 }
 
 // Cooperative +1 of the 64 keys held one per lane (want = lane has a key).
-// Same bucket/round layout as coop_get64.  In each group the lane holding
+// Eight lanes per key: in round q the lanes 8g..8g+7 read the eight slots of
+// the home bucket of lane 8q+g's key (one coalesced 64-byte request per key,
+// eight loads in flight per lane).  In each group the lane holding
 // the key adds 1; otherwise the lowest empty slot of the snapshot is claimed
 // with CAS.  The snapshot may be stale, but a slot's key never changes once
 // set and every "empty" is confirmed by the CAS, so no key is inserted twice.
