@@ -548,7 +548,8 @@ def main() -> int:
     ms = 1000 * total / args.steps
 
     kern = {}
-    for name in ["pack", "read_words", "hll", "sort_pairs", "minimizer", "count", "part_hist", "part_scatter", "part_count",
+    for name in ["pack", "read_words", "hll", "sort_pairs", "minimizer", "count", "count_merge", "count_hist",
+                 "count_scatter", "count_bounds", "count_regions", "count_retry", "part_hist", "part_scatter", "part_count",
                  "add_keys", "compact", "dump_count", "ovf_fold", "stats", "select", "select_flags",
                  "scan_u64", "scan_u32", "windows", "rehash"]:
         tot, n = ctx.kernel_time(name)

@@ -43,12 +43,14 @@ def _count(api, ctx, seqs_or_pair, k=25, canonical=True, expected=None, max_load
     return r, t, seq, off
 
 
-@pytest.fixture(params=["direct", "regions", "ordered"])
+@pytest.fixture(params=["direct", "regions", "ordered", "resident"])
 def layout(request, monkeypatch):
     """Count through the direct kernel, through the region-partitioned
-    passes (1 MB regions so that small tables span many regions), or through
-    the direct kernel taking reads in the locality order (forced on small
-    inputs)."""
+    passes (1 MB regions so that small tables span many regions), through
+    the block-merging kernel with table atomics taking reads in the locality
+    order (forced on small inputs), or through the region-resident passes
+    (4 KB regions, so that small tables span many regions and both sort
+    passes have many bins)."""
     if request.param == "direct":
         monkeypatch.setenv("TK_PARTITION", "0")
     elif request.param == "regions":
@@ -58,6 +60,10 @@ def layout(request, monkeypatch):
         monkeypatch.setenv("TK_PARTITION", "0")
         monkeypatch.setenv("TK_ORDER", "1")
         monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
+        if request.param == "ordered":
+            monkeypatch.setenv("TK_RESIDENT", "0")
+        else:
+            monkeypatch.setenv("TK_REGION_KB", "4")
     return request.param
 
 
@@ -126,6 +132,27 @@ def test_table_growth(api, ctx, layout):
     assert cap * 0.5 >= len(ok)
     gk, gc = t.dump()
     assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+
+
+@pytest.mark.parametrize("region_kb", ["1", "64"])
+def test_count_resident_crowded_regions(api, ctx, region_kb, monkeypatch):
+    """Region-resident count into a table grown at load 0.9: with 128-slot
+    regions some regions receive more keys than they hold, and their words
+    are added after the table grows; with 64 KB regions the table is one
+    region or a few.  A second count adds onto the non-empty table."""
+    say("count_resident_crowded", region_kb)
+    monkeypatch.setenv("TK_PARTITION", "0")
+    monkeypatch.setenv("TK_ORDER", "1")
+    monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
+    monkeypatch.setenv("TK_REGION_KB", region_kb)
+    seqs = random_reads(17, 20000, 100, 150, n_rate=0.001) + repetitive_reads(25, 3000)
+    r, t, seq, off = _count(api, ctx, seqs, 25, True, expected=64, max_load=0.9)
+    ok, oc = O.count(seq, off, 25, True)
+    gk, gc = t.dump()
+    assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+    t.count(r)
+    gk, gc = t.dump()
+    assert np.array_equal(gk, ok) and np.array_equal(gc, 2 * oc)
 
 
 def test_hll_estimate(api, ctx, small_se):

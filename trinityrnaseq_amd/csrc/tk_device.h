@@ -90,6 +90,22 @@ __device__ __forceinline__ void window_at(const ReadsView& rv, uint64_t wbase,
   valid = (m >> (32 - k)) == 0u;
 }
 
+// As window_at, plus the window's non-ACGT mask: bit k-1-i set when base
+// pos+i is not ACGT (bit 0 = the window's last base).
+__device__ __forceinline__ void window_at_mask(const ReadsView& rv, uint64_t wbase, uint32_t pos, int k,
+                                               uint64_t& fwd, uint32_t& nmask) {
+  const uint64_t w = wbase + (pos >> 5);
+  const uint32_t o = pos & 31u;
+  const uint64_t hi = rv.codes[w];
+  const uint64_t lo = rv.codes[w + 1];
+  const uint32_t mhi = rv.nmask[w];
+  const uint32_t mlo = rv.nmask[w + 1];
+  const uint64_t x = o ? ((hi << (2 * o)) | (lo >> (64 - 2 * o))) : hi;
+  const uint32_t m = o ? ((mhi << o) | (mlo >> (32 - o))) : mhi;
+  fwd = x >> (64 - 2 * k);
+  nmask = m >> (32 - k);
+}
+
 __device__ __forceinline__ uint64_t canon_key(uint64_t fwd, int k, int canonical) {
   if (!canonical) return fwd;
   const uint64_t rc = revcomp(fwd, k);
@@ -231,6 +247,48 @@ __device__ __forceinline__ uint64_t slot_count(const TableView& t, uint64_t s) {
   if (f != t.cmax) return f;
   const uint64_t* o = ovf_find(t, s & t.kmo);
   return o ? *o : f;
+}
+
+// ---- LDS tables of 64-B buckets (eight slots in the global slot format) ----
+// Insert-or-add into an LDS table: `word` = tagged key | c << cshift.  From
+// bucket b (wrapping with bmask) each step reads one bucket, picks without
+// branching the slot holding the key or else the first empty one, and
+// issues a single atomic there: add c, or claim with CAS (a lost claim
+// rescans the same bucket).  Occupied slots stay a prefix of their bucket.
+// Returns 2 when a slot was claimed, 1 when added (old = previous slot
+// value), 0 when max_steps steps found no room.
+__device__ __forceinline__ int lds_bucket_add(uint64_t* tab, uint32_t b, uint32_t bmask, uint64_t word,
+                                              uint64_t kmo, uint32_t cshift, uint32_t max_steps,
+                                              uint64_t& old) {
+  const uint64_t tagged = word & kmo;
+  const uint64_t add = (word >> cshift) << cshift;
+  for (uint32_t p = 0; p < max_steps; p++) {
+    uint64_t* line = tab + ((uint64_t)b << 3);
+    const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(line);
+    const ulonglong2 v0 = l2[0], v1 = l2[1], v2 = l2[2], v3 = l2[3];
+    const uint64_t sv[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
+    int jm = 8, je = 8;
+#pragma unroll
+    for (int j = 7; j >= 0; j--) {
+      jm = (sv[j] & kmo) == tagged ? j : jm;
+      je = sv[j] == 0 ? j : je;
+    }
+    if (jm < 8) {
+      old = atomicAdd((unsigned long long*)&line[jm], (unsigned long long)add);
+      return 1;
+    }
+    if (je < 8) {
+      const uint64_t v = atomicCAS((unsigned long long*)&line[je], 0ull, (unsigned long long)word);
+      if (v == 0) return 2;
+      if ((v & kmo) == tagged) {
+        old = atomicAdd((unsigned long long*)&line[je], (unsigned long long)add);
+        return 1;
+      }
+      continue;  // another key took the slot: rescan this bucket
+    }
+    b = (b + 1) & bmask;
+  }
+  return 0;
 }
 
 // ---- wave helpers ----

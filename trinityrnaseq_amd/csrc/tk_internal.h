@@ -33,6 +33,9 @@ void dev_free(void* p, size_t bytes);
 // frees every object it creates on that thread.
 void cache_begin(size_t limit_bytes);
 void cache_end();
+// Device bytes the region-resident count of this thread may use for its
+// word streams (0 = 40% of the free device memory).
+void resident_limit(size_t bytes);
 
 struct PendingEvent {
   std::string name;
@@ -55,7 +58,7 @@ struct tk_ctx {
   std::map<std::string, std::pair<double, uint64_t>> times;
   uint32_t* d_flags = nullptr;     // 64 words of small device counters / error bits
   uint32_t* h_flags = nullptr;     // pinned mirror
-  tk::Scratch scratch[4];          // reusable device scratch buffers
+  tk::Scratch scratch[8];          // reusable device scratch buffers (4-7: region-resident count)
   int num_cu = 0;
   void* stage[2] = {nullptr, nullptr};        // pinned staging for pageable uploads
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -72,6 +75,7 @@ struct tk_reads {
   uint64_t* d_codes = nullptr;     // n_words + 1
   uint32_t* d_nmask = nullptr;     // n_words + 1
   std::map<int, uint64_t> windows; // k -> total windows
+  std::map<int, double> est;       // 2k + canonical -> distinct estimate of all reads
   uint32_t* d_order = nullptr;     // locality order of the table passes (n_reads)
   bool order_done = false;         // d_order built, or found not worth building
   uint64_t bytes = 0;

@@ -63,6 +63,11 @@ void cache_begin(size_t limit_bytes) {
   t_cache->limit = limit_bytes;
 }
 
+// Device bytes the region-resident count may hold for its streams (0 = 40%
+// of the free memory).  A server worker sets its own share.
+static thread_local uint64_t t_resident_limit = 0;
+void resident_limit(size_t bytes) { t_resident_limit = bytes; }
+
 void cache_end() {
   BlockCache* c = t_cache;
   if (!c) return;
@@ -757,115 +762,441 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
 // Each wavefront compacts and flushes its own quarter of the LDS table.
 // Needs cmax >= kMaxRun (k <= 25), so neither the LDS word nor one global
 // add carries more than once.
-template <int SLOTS>
+//
+// EMIT: instead of adding them to the table, a flush appends the block's
+// merged words to `out` for the region-resident passes below.  The grid is
+// then persistent (each block takes groups of 64 reads grid-stride) and a
+// block reserves the stream `chunk` words at a time (chunk >= SLOTS, the
+// most one flush emits), so the shared cursor sees one atomic per chunk,
+// not one per flush.  A flush that does not fit below `cap` is added to the
+// table as above; every reserved word below `cap` that holds no merged word
+// is zero, so out[0, min(*cursor, cap)) is always defined (0 = no word).
+
+template <int SLOTS, bool EMIT>
 __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
-                                                   int canonical, TableView t, uint32_t* __restrict__ flags) {
+                                                   int canonical, TableView t, uint32_t* __restrict__ flags,
+                                                   uint64_t* __restrict__ out, uint64_t cap, uint64_t chunk,
+                                                   unsigned long long* __restrict__ cursor) {
   extern __shared__ uint64_t s_tab[];  // SLOTS words, 0 = empty
   __shared__ uint32_t s_off[kReadsPerBlock + 1];
   __shared__ uint64_t s_wb[kReadsPerBlock];
   __shared__ uint32_t s_new, s_distinct;
+  __shared__ uint32_t s_nd[4];
+  __shared__ unsigned long long s_cur, s_end, s_z0, s_z1;  // EMIT: reserved chunk, range to zero
   constexpr uint32_t kMaxDistinct = SLOTS / 4 * 3;  // LDS load <= 3/4
   constexpr uint32_t kMaxRun = 8064;                // occurrences per merge <= cmax (k <= 25)
   constexpr uint32_t kQuarter = SLOTS / 4;
-  const uint64_t r0 = r_begin + xcd_block(blockIdx.x, gridDim.x) * kReadsPerBlock;
-  if (r0 >= r_end) return;
-  const int nr = (int)std::min<uint64_t>(kReadsPerBlock, r_end - r0);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (tid < kReadsPerBlock) {
-    uint32_t nw = 0;
-    if (tid < nr) {
-      const uint64_t rr = read_at(rv, r0 + tid);
-      const uint32_t L = rv.len[rr];
-      nw = L >= (uint32_t)k ? L - k + 1 : 0;
-      s_wb[tid] = rv.word_off[rr];
-    }
-    uint32_t x = nw;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      uint32_t y = __shfl_up(x, o);
-      if (tid >= o) x += y;
-    }
-    s_off[tid + 1] = x;
-    if (tid == 0) { s_off[0] = 0; s_new = 0; }
-  }
+  const uint64_t ngroups = (r_end - r_begin + kReadsPerBlock - 1) / kReadsPerBlock;
+  const uint64_t g0 = EMIT ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t gstride = EMIT ? gridDim.x : ngroups;
   for (int i = tid; i < SLOTS; i += blockDim.x) s_tab[i] = 0;
-  if (tid == 0) s_distinct = 0;
-  __syncthreads();
-  const uint32_t total = s_off[nr];
+  if (tid == 0) {
+    s_new = 0;
+    s_distinct = 0;
+    s_cur = s_end = 0;
+  }
   const uint64_t one = 1ull << t.cshift;
   uint64_t* quarter = s_tab + (uint32_t)wv * kQuarter;
   uint32_t mynew = 0;
-  uint32_t since = 0;  // windows merged since the last flush (block-uniform)
-  for (uint32_t w0 = 0; w0 < total; w0 += 256) {
-    // 1. 256 windows into the LDS table
-    const uint32_t w = w0 + tid;
-    if (w < total) {
-      int lo = 0, hi = nr;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_off[mid] <= w) lo = mid; else hi = mid;
+  for (uint64_t grp = g0; grp < ngroups; grp += gstride) {
+    const uint64_t r0 = r_begin + grp * kReadsPerBlock;
+    const int nr = (int)std::min<uint64_t>(kReadsPerBlock, r_end - r0);
+    __syncthreads();  // the previous group's reads of s_off / s_wb are done
+    if (tid < kReadsPerBlock) {
+      uint32_t nw = 0;
+      if (tid < nr) {
+        const uint64_t rr = read_at(rv, r0 + tid);
+        const uint32_t L = rv.len[rr];
+        nw = L >= (uint32_t)k ? L - k + 1 : 0;
+        s_wb[tid] = rv.word_off[rr];
       }
-      uint64_t fwd;
-      bool valid;
-      window_at(rv, s_wb[lo], w - s_off[lo], k, fwd, valid);
-      if (valid) {
-        const uint64_t tagged = canon_key(fwd, k, canonical) | t.occ;
-        uint32_t h = (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 40) & (SLOTS - 1);
-        while (true) {
-          uint64_t v = s_tab[h];
-          if (v == 0) {
-            v = atomicCAS((unsigned long long*)&s_tab[h], 0ULL, (unsigned long long)(tagged | one));
-            if (v == 0) {
-              atomicAdd(&s_distinct, 1u);
-              break;
-            }
+      uint32_t x = nw;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (tid >= o) x += y;
+      }
+      s_off[tid + 1] = x;
+      if (tid == 0) s_off[0] = 0;
+    }
+    __syncthreads();
+    // Four threads per read, each rolling over one quarter of its windows:
+    // per round a thread shifts one base into its forward and reverse
+    // k-mers (a code word is loaded once per 32 bases) and tracks the run of
+    // ACGT bases ending there (a window is valid when the run is >= k).
+    const int rj = tid >> 2, rp = tid & 3;
+    uint32_t rs = 0, rcnt = 0;
+    uint64_t rwb = 0;
+    if (rj < nr) {
+      const uint32_t nw = s_off[rj + 1] - s_off[rj];
+      const uint32_t q = (nw + 3) >> 2;
+      rs = min(nw, (uint32_t)rp * q);
+      rcnt = min(nw, rs + q) - rs;
+      rwb = s_wb[rj];
+    }
+    const uint32_t wmax = wave_max_u32(rcnt);
+    __syncthreads();  // every thread has read s_off
+    if (lane == 0) s_nd[wv] = wmax;
+    __syncthreads();
+    const uint32_t rounds = max(max(s_nd[0], s_nd[1]), max(s_nd[2], s_nd[3]));
+    const uint64_t kmask = (1ull << (2 * k)) - 1;
+    uint64_t fwd = 0, rcm = 0, cw = 0;
+    uint32_t cm = 0, run = 0, pos = 0;
+    uint32_t since = 0;  // windows merged since the last flush (block-uniform)
+    for (uint32_t r = 0; r < rounds; r++) {
+      // 1. one window per thread into the LDS table
+      bool isnew = false;
+      if (r < rcnt) {
+        if (r == 0) {
+          uint32_t nm;
+          window_at_mask(rv, rwb, rs, k, fwd, nm);
+          rcm = revcomp(fwd, k);
+          run = nm ? (uint32_t)__builtin_ctz(nm) : (uint32_t)k;
+          pos = rs + k;
+          cw = rv.codes[rwb + (pos >> 5)];
+          cm = rv.nmask[rwb + (pos >> 5)];
+        } else {
+          const uint32_t o = pos & 31u;
+          if (o == 0) {
+            cw = rv.codes[rwb + (pos >> 5)];
+            cm = rv.nmask[rwb + (pos >> 5)];
           }
-          if ((v & t.kmo) == tagged) {
-            atomicAdd((unsigned long long*)&s_tab[h], (unsigned long long)one);
-            break;
-          }
-          h = (h + 1) & (SLOTS - 1);
+          const uint64_t b = (cw >> (62 - 2 * o)) & 3u;
+          fwd = ((fwd << 2) | b) & kmask;
+          rcm = (rcm >> 2) | ((3u - b) << (2 * k - 2));
+          run = ((cm >> (31 - o)) & 1u) ? 0u : run + 1;
+          pos++;
+        }
+        if (run >= (uint32_t)k) {
+          const uint64_t tagged = ((canonical && rcm < fwd) ? rcm : fwd) | t.occ;
+          const uint32_t hb = (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 40) & (SLOTS / 8 - 1);
+          uint64_t old;
+          // the table never holds more than kMaxDistinct + 256 < SLOTS keys
+          isnew = lds_bucket_add(s_tab, hb, SLOTS / 8 - 1, tagged | one, t.kmo, t.cshift, 4 * SLOTS, old) == 2;
         }
       }
-    }
-    since += 256;
-    __syncthreads();
-    // Flush when another round could take the table past 3/4 load or a
-    // count past kMaxRun occurrences, and after the last round.
-    const uint32_t distinct = s_distinct;
-    if (w0 + 256 >= total || distinct + 256 > kMaxDistinct || since + 256 > kMaxRun) {
-      __syncthreads();  // every thread has read s_distinct
-      if (tid == 0) s_distinct = 0;
-      // 2. each wavefront compacts its quarter in place (a wavefront's LDS
-      //    reads of a batch precede its writes, and writes land at or below
-      //    the batch being read)...
-      uint32_t nd = 0;
-      for (uint32_t i0 = 0; i0 < kQuarter; i0 += 64) {
-        const uint64_t v = quarter[i0 + lane];
-        const uint64_t m = __ballot(v != 0);
-        if (v) quarter[nd + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = v;
-        nd += (uint32_t)__popcll(m);
-      }
-      wave_sync();
-      // 3. ...and adds its distinct keys to the global table, 64 at a time
-      for (uint32_t e0 = 0; e0 < nd; e0 += 64) {
-        const bool want = e0 + lane < nd;
-        const uint64_t v = want ? quarter[e0 + lane] : 0;
-        bool lost = false;
-        mynew += coop4_add(t, v & (t.occ - 1), want, lane, &flags[0], lost, want ? v >> t.cshift : 1);
-        if (lost) atomicOr(&flags[0], 1u);
-      }
-      wave_sync();
-      for (uint32_t i = lane; i < kQuarter; i += 64) quarter[i] = 0;
+      const uint32_t wnew = (uint32_t)__popcll(__ballot(isnew));
+      if (lane == 0 && wnew) atomicAdd(&s_distinct, wnew);
+      since += 256;
       __syncthreads();
-      since = 0;
+      // Flush when another round could take the table past 3/4 load or a
+      // count past kMaxRun occurrences, and after the last round.
+      const uint32_t distinct = s_distinct;
+      if (r + 1 >= rounds || distinct + 256 > kMaxDistinct || since + 256 > kMaxRun) {
+        __syncthreads();  // every thread has read s_distinct
+        if (tid == 0) s_distinct = 0;
+        // 2. each wavefront compacts its quarter in place (a wavefront's LDS
+        //    reads of a batch precede its writes, and writes land at or below
+        //    the batch being read)...
+        uint32_t nd = 0;
+        for (uint32_t i0 = 0; i0 < kQuarter; i0 += 64) {
+          const uint64_t v = quarter[i0 + lane];
+          const uint64_t m = __ballot(v != 0);
+          if (v) quarter[nd + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = v;
+          nd += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+        if (EMIT) {
+          // 3a. ...and appends them to the block's chunk of the stream (a
+          //     new chunk when they do not fit; the old one's rest zeroed)
+          if (lane == 0) s_nd[wv] = nd;
+          __syncthreads();
+          const uint32_t sum = s_nd[0] + s_nd[1] + s_nd[2] + s_nd[3];
+          if (tid == 0) {
+            s_z0 = s_z1 = 0;
+            if (s_cur + sum > s_end) {
+              s_z0 = s_cur;
+              s_z1 = s_end < cap ? s_end : cap;
+              s_cur = atomicAdd(cursor, (unsigned long long)chunk);
+              s_end = s_cur + chunk;
+            }
+          }
+          __syncthreads();
+          const uint64_t cur = s_cur, lim = s_end < cap ? s_end : cap;
+          for (uint64_t i = s_z0 + tid; i < s_z1; i += blockDim.x) out[i] = 0;
+          if (cur + sum <= lim) {
+            uint64_t at = cur;
+            for (int i = 0; i < wv; i++) at += s_nd[i];
+            for (uint32_t e0 = lane; e0 < nd; e0 += 64) out[at + e0] = quarter[e0];
+            nd = 0;
+          }
+          __syncthreads();  // every thread has read s_cur
+          if (tid == 0 && cur + sum <= lim) s_cur = cur + sum;
+        }
+        // 3. ...and adds its distinct keys to the global table, 64 at a time
+        //    (EMIT: only a flush past cap, one lane per key)
+        for (uint32_t e0 = 0; e0 < nd; e0 += 64) {
+          const bool want = e0 + lane < nd;
+          const uint64_t v = want ? quarter[e0 + lane] : 0;
+          if (EMIT) {
+            if (want) {
+              const uint32_t r = table_add(t, v & (t.occ - 1), v >> t.cshift, &flags[0]);
+              if (r == 2u) atomicOr(&flags[0], 1u);
+              else mynew += r;
+            }
+          } else {
+            bool lost = false;
+            mynew += coop4_add(t, v & (t.occ - 1), want, lane, &flags[0], lost, want ? v >> t.cshift : 1);
+            if (lost) atomicOr(&flags[0], 1u);
+          }
+        }
+        wave_sync();
+        for (uint32_t i = lane; i < kQuarter; i += 64) quarter[i] = 0;
+        __syncthreads();
+        since = 0;
+      }
     }
+  }
+  if (EMIT) {
+    __syncthreads();
+    const uint64_t lim = s_end < cap ? s_end : cap;
+    for (uint64_t i = s_cur + tid; i < lim; i += blockDim.x) out[i] = 0;
   }
   mynew = wave_sum_u32(mynew);
   if (lane == 0 && mynew) atomicAdd(&s_new, mynew);
   __syncthreads();
   if (tid == 0 && s_new)
     atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)s_new);
+}
+
+// ---------------------------------------------------------------------
+// Region-resident counting (reads in the locality order, k <= 25).  The
+// block merge above emits its merged words (tagged key | occurrences <<
+// (2k+1)) to a stream; two counting-sort passes group the stream by table
+// region (coarse bins of 2^fs regions, then single regions); then one
+// workgroup per region loads the region's buckets into LDS, adds the
+// region's words there with LDS atomics and writes the region back.  Each
+// table line is read and written once per pass instead of taking one
+// memory-side atomic per merged word, so the pass is bound by streaming
+// HBM traffic, not by the device's atomic rate.
+// ---------------------------------------------------------------------
+constexpr uint32_t kResRegionBitsMax = 10;  // 1024 buckets = 64 KB of LDS
+constexpr uint64_t kResSlice = 65536;       // words per block of a sort pass
+constexpr uint32_t kResMaxBins = 2048;      // bins per sort pass
+
+// A slice of a sort pass: words [begin, end) of the input, whose bin b
+// counts into hist[hbase + b * m + j] (m slices in the slice's group).
+struct ResSlice {
+  uint64_t begin, end, hbase;
+  uint32_t m, j;
+};
+
+__device__ __forceinline__ uint32_t res_region(const TableView& t, uint64_t w) {
+  return (uint32_t)((bucket0(t, w & (t.occ - 1)) - t.boff) >> t.rbits);
+}
+
+// Bin of every nonzero word: (region >> shift) & mask, bins < nbins.
+__global__ __launch_bounds__(256) void k_res_hist(const uint64_t* __restrict__ in,
+                                                  const ResSlice* __restrict__ sl, TableView t,
+                                                  uint32_t shift, uint32_t mask, uint32_t nbins,
+                                                  uint64_t* __restrict__ hist) {
+  __shared__ uint32_t s_cnt[kResMaxBins];
+  const ResSlice d = sl[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) s_cnt[i] = 0;
+  __syncthreads();
+  for (uint64_t i0 = d.begin; i0 < d.end; i0 += 4 * blockDim.x) {
+    uint64_t w[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint64_t i = i0 + u * blockDim.x + threadIdx.x;
+      w[u] = i < d.end ? in[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (w[u]) atomicAdd(&s_cnt[(res_region(t, w[u]) >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) hist[d.hbase + (uint64_t)i * d.m + d.j] = s_cnt[i];
+}
+
+// Scatter every nonzero word to its bin's range: out[base[bin entry] + rank].
+// Each tile of kResTile words is counting-sorted by bin in LDS first, so
+// the stores leave in runs of consecutive addresses (one run per bin and
+// tile) instead of one scattered 8-B store per word.
+constexpr uint32_t kResTile = 4096;
+static size_t res_scatter_lds(uint32_t nbins) { return (size_t)kResTile * 10 + (size_t)nbins * 16; }
+
+// Exclusive scan of s_cnt[0, nbins) into s_loc (nbins <= 2048, blockDim
+// 256); returns the total.
+__device__ __forceinline__ uint32_t res_bin_scan(const uint32_t* s_cnt, uint32_t* s_loc, uint32_t nbins,
+                                                 uint32_t* s_w) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const uint32_t E = (nbins + 255) / 256, b0 = tid * E;
+  uint32_t sum = 0;
+  for (uint32_t e = 0; e < E; e++)
+    if (b0 + e < nbins) sum += s_cnt[b0 + e];
+  uint32_t x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wv] = x;
+  __syncthreads();
+  uint32_t pre = x - sum;
+  for (int i = 0; i < wv; i++) pre += s_w[i];
+  for (uint32_t e = 0; e < E; e++)
+    if (b0 + e < nbins) {
+      s_loc[b0 + e] = pre;
+      pre += s_cnt[b0 + e];
+    }
+  return s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ __launch_bounds__(256) void k_res_scatter(const uint64_t* __restrict__ in,
+                                                     const ResSlice* __restrict__ sl, TableView t,
+                                                     uint32_t shift, uint32_t mask, uint32_t nbins,
+                                                     const uint64_t* __restrict__ base,
+                                                     uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t s_res[];
+  uint64_t* s_tile = s_res;                                              // kResTile words
+  uint64_t* s_pos = s_res + kResTile;                                    // nbins: next output index
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_pos + nbins);          // nbins: tile counts
+  uint32_t* s_loc = s_cnt + nbins;                                       // nbins: tile offsets
+  uint16_t* s_tbin = reinterpret_cast<uint16_t*>(s_loc + nbins);         // kResTile bins
+  __shared__ uint32_t s_w[4];
+  constexpr int kPer = kResTile / 256;
+  const int tid = threadIdx.x;
+  const ResSlice d = sl[blockIdx.x];
+  for (uint32_t i = tid; i < nbins; i += blockDim.x) s_pos[i] = base[d.hbase + (uint64_t)i * d.m + d.j];
+  for (uint64_t t0 = d.begin; t0 < d.end; t0 += kResTile) {
+    const uint32_t nt = (uint32_t)std::min<uint64_t>(kResTile, d.end - t0);
+    for (uint32_t i = tid; i < nbins; i += blockDim.x) s_cnt[i] = 0;
+    __syncthreads();
+    uint64_t w[kPer];
+    uint32_t bin[kPer], rank[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const uint32_t i = u * 256 + tid;
+      w[u] = i < nt ? in[t0 + i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      bin[u] = 0;
+      rank[u] = 0;
+      if (w[u]) {
+        bin[u] = (res_region(t, w[u]) >> shift) & mask;
+        rank[u] = atomicAdd(&s_cnt[bin[u]], 1u);
+      }
+    }
+    __syncthreads();
+    const uint32_t nv = res_bin_scan(s_cnt, s_loc, nbins, s_w);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPer; u++)
+      if (w[u]) {
+        const uint32_t p = s_loc[bin[u]] + rank[u];
+        s_tile[p] = w[u];
+        s_tbin[p] = (uint16_t)bin[u];
+      }
+    __syncthreads();
+    for (uint32_t i = tid; i < nv; i += blockDim.x) {
+      const uint32_t b = s_tbin[i];
+      out[s_pos[b] + (i - s_loc[b])] = s_tile[i];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nbins; i += blockDim.x) s_pos[i] += s_cnt[i];
+  }
+}
+
+// First word of every region (region g = coarse g >> fs, fine g & (2^fs-1))
+// from the scanned fine histogram; rstart[R] = total.
+__global__ void k_res_bounds(const uint64_t* __restrict__ base2, const uint64_t* __restrict__ hb,
+                             const uint32_t* __restrict__ m, uint32_t fs, uint32_t R, uint64_t total,
+                             uint64_t* __restrict__ rstart) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > R) return;
+  if (g == R) {
+    rstart[R] = total;
+    return;
+  }
+  const uint32_t c = (uint32_t)(g >> fs), f = (uint32_t)g & ((1u << fs) - 1);
+  rstart[g] = base2[hb[c] + (uint64_t)f * m[c]];
+}
+
+// One workgroup per region: its buckets in LDS (zero when the table is
+// known empty), its words added there with the global slot protocol (claim
+// an empty slot with CAS, add to the slot holding the key; occupied slots
+// stay a prefix of their bucket), carries to the side table, the region
+// written back.  A word that finds no room in the region is marked in place
+// (occupied bit cleared) and flags[6] set; it is added after the table grows.
+template <bool FRESH>
+__global__ __launch_bounds__(1024) void k_res_count(uint64_t* __restrict__ words,
+                                                    const uint64_t* __restrict__ rstart, TableView t,
+                                                    uint32_t* __restrict__ flags) {
+  extern __shared__ uint64_t s_reg[];  // 8 << rbits slots
+  __shared__ uint32_t s_new;
+  constexpr int kPre = 4;  // words loaded ahead of their LDS inserts
+  const uint64_t a = rstart[blockIdx.x], e = rstart[blockIdx.x + 1];
+  if (a == e) return;
+  const uint32_t n16 = 4u << t.rbits;
+  ulonglong2* g = reinterpret_cast<ulonglong2*>(t.slots + ((t.boff + ((uint64_t)blockIdx.x << t.rbits)) << 3));
+  ulonglong2* l = reinterpret_cast<ulonglong2*>(s_reg);
+  for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) {
+    if (FRESH) l[i] = make_ulonglong2(0ull, 0ull);
+    else l[i] = g[i];
+  }
+  if (threadIdx.x == 0) s_new = 0;
+  __syncthreads();
+  const uint32_t rmask = (uint32_t)t.rmask;
+  uint32_t mynew = 0;
+  bool spill = false;
+  for (uint64_t i0 = a; i0 < e; i0 += (uint64_t)kPre * blockDim.x) {
+    uint64_t wp[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+      const uint64_t i = i0 + (uint64_t)u * blockDim.x + threadIdx.x;
+      wp[u] = i < e ? words[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+      const uint64_t w = wp[u];
+      if (!w) continue;
+      const uint64_t tagged = w & t.kmo;
+      const uint64_t c = w >> t.cshift;
+      const uint32_t b = (uint32_t)(bucket0(t, w & (t.occ - 1)) - t.boff) & rmask;
+      uint64_t old = 0;
+      // every bucket of the region, each rescanned at most 8 times
+      const int res = lds_bucket_add(s_reg, b, rmask, w, t.kmo, t.cshift, 9 * (rmask + 1), old);
+      const bool carry = res == 1 && (old >> t.cshift) + c > t.cmax;
+      if (res == 2) mynew++;
+      if (carry) ovf_add(t, tagged, t.cmax + 1, &flags[0]);
+      if (!res) {
+        words[i0 + (uint64_t)u * blockDim.x + threadIdx.x] = w & ~t.occ;
+        spill = true;
+      }
+    }
+  }
+  if (spill) atomicOr(&flags[6], 1u);
+  mynew = wave_sum_u32(mynew);
+  if ((threadIdx.x & 63) == 0 && mynew) atomicAdd(&s_new, mynew);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) g[i] = l[i];
+  if (threadIdx.x == 0 && s_new)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)s_new);
+}
+
+// Add the words that k_res_count marked (occupied bit clear) to a grown
+// table; words that still find no room stay marked (flags[6]).
+__global__ void k_res_retry(uint64_t* __restrict__ words, uint64_t n, TableView t,
+                            uint32_t* __restrict__ flags) {
+  uint32_t mynew = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t w = words[i];
+    if (!w || (w & t.occ)) continue;
+    const uint32_t r = table_add(t, w & (t.occ - 1), w >> t.cshift, &flags[0]);
+    if (r == 2u) {
+      atomicOr(&flags[6], 1u);
+    } else {
+      mynew += r;
+      words[i] = w | t.occ;
+    }
+  }
+  mynew = wave_sum_u32(mynew);
+  if ((threadIdx.x & 63) == 0 && mynew)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)mynew);
 }
 
 // ---------------------------------------------------------------------
@@ -1107,11 +1438,15 @@ static int table_ensure_ovf(tk_ctx* ctx, tk_table* t, uint64_t bound) {
 
 // Region size for large tables (bytes); a region is the unit counted while
 // resident in the Infinity Cache.
-// TK_REGION_MB overrides the 64 MB default (read when a table is laid out).
+// Bytes per table region: 64 KB by default, the size one workgroup of the
+// region-resident count holds in LDS.  TK_REGION_KB or TK_REGION_MB
+// overrides it (read when a table is laid out).
 static uint64_t region_bytes() {
-  const char* e = getenv("TK_REGION_MB");
-  const uint64_t mb = e ? strtoull(e, nullptr, 10) : 64;
-  return next_pow2(std::max<uint64_t>(mb, 1)) << 20;
+  if (const char* e = getenv("TK_REGION_MB"))
+    return next_pow2(std::max<uint64_t>(strtoull(e, nullptr, 10), 1)) << 20;
+  if (const char* e = getenv("TK_REGION_KB"))
+    return next_pow2(std::max<uint64_t>(strtoull(e, nullptr, 10), 1)) << 10;
+  return 64ull << 10;
 }
 
 // Bucket layout holding `keys` at load max_load: nb = regions * 2^rbits.
@@ -1865,7 +2200,9 @@ int tk_estimate_distinct(tk_ctx* c, const tk_reads* r, int k, int canonical, dou
   uint64_t total = 0;
   int rc = reads_windows(c, r, k, &total);
   if (rc) return rc;
-  return hll_estimate(c, r, 0, r->n_reads, k, canonical, total, est);
+  rc = hll_estimate(c, r, 0, r->n_reads, k, canonical, total, est);
+  if (!rc) const_cast<tk_reads*>(r)->est[2 * k + (canonical ? 1 : 0)] = *est;
+  return rc;
 }
 
 int tk_table_create(tk_ctx* c, int k, int canonical, uint64_t expected, double max_load,
@@ -2126,12 +2463,218 @@ static int agg_slots(tk_ctx* c) {
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     big = 0;
     if (maxlds >= 8192 * 8 + 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) == hipSuccess)
       big = 1;
     (void)hipGetLastError();
   }
   return big ? 8192 : 4096;
+}
+
+// TK_RESIDENT=0 turns the region-resident count off (on by default for
+// reads in the locality order, k <= 25, tables of 64 KB regions).
+static bool resident_enabled() {
+  const char* e = getenv("TK_RESIDENT");
+  return !(e && e[0] == '0');
+}
+
+static bool resident_usable(tk_ctx* c, const tk_table* t) {
+  if (t->rbits > kResRegionBitsMax || (t->nb >> t->rbits) > (uint64_t)kResMaxBins * kResMaxBins) return false;
+  static int ok = -1;  // 64 KB or more of dynamic LDS per workgroup needs the attribute
+  if (ok < 0) {
+    ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<true>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 << kResRegionBitsMax) == hipSuccess &&
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<false>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 << kResRegionBitsMax) == hipSuccess &&
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)res_scatter_lds(kResMaxBins)) == hipSuccess;
+    (void)hipGetLastError();
+  }
+  (void)c;
+  return ok == 1;
+}
+
+// Scratch slots of the region-resident count (kept by the context, so the
+// multi-GB word streams are allocated once, not per count call).
+enum { kSlotResA = 4, kSlotResB = 5, kSlotResHist = 6, kSlotResMisc = 7 };
+
+// One counting-sort pass of the region-resident count: the nonzero words of
+// `in` (slice table dsl, copied from sl) to `out`, grouped by bin
+// (region >> shift) & mask.  h holds nhist + 1 counts, hs their scan.
+static int res_sort_pass(tk_ctx* c, const std::vector<ResSlice>& sl, ResSlice* dsl, uint64_t nhist,
+                         uint64_t* h, uint64_t* hs, const uint64_t* in, uint64_t* out, const TableView& v,
+                         uint32_t shift, uint32_t mask, uint32_t nbins) {
+  TK_HIP(hipMemcpyAsync(dsl, sl.data(), sl.size() * sizeof(ResSlice), hipMemcpyHostToDevice, c->stream));
+  TK_HIP(hipMemsetAsync(h + nhist, 0, 8, c->stream));
+  TK_LAUNCH(c, "count_hist", sl.size(), 256, 0, k_res_hist, in, dsl, v, shift, mask, nbins, h);
+  int rc = exclusive_scan_u64(c, h, hs, nhist + 1);
+  if (rc) return rc;
+  TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins), k_res_scatter, in, dsl, v, shift, mask,
+            nbins, hs, out);
+  // the slice table is host memory read by the copy above
+  TK_HIP(hipStreamSynchronize(c->stream));
+  return TK_OK;
+}
+
+// Count reads [rb, re) of the locality order through the region-resident
+// passes: merge + emit, coarse sort, fine sort, per-region LDS count.  The
+// streams hold at most `cap` words per pass; reads are taken in chunks whose
+// windows fit (merged words that would still overflow are added to the
+// table directly by the merge kernel).
+static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const ReadsView& rview, uint64_t rb,
+                          uint64_t re, uint64_t windows, uint32_t max_nwin) {
+  int rc = TK_OK;
+  // the table's final regions must be known before words are grouped by them
+  if ((double)(t->n_keys + windows) > 0.9 * (double)(t->nb * 8)) {
+    double est = 0;
+    auto it = r->est.find(2 * t->k + t->canonical);  // the caller's own estimate of these reads
+    if (rb == 0 && re == r->n_reads && it != r->est.end())
+      est = it->second;
+    else if ((rc = hll_estimate(c, r, rb, re, t->k, t->canonical, windows, &est)))
+      return rc;
+    const double want = (double)t->n_keys + 1.05 * est + 4096;
+    if (want > t->max_load * (double)(t->nb * 8))
+      if ((rc = table_grow(c, t, (uint64_t)want))) return rc;
+    if (!resident_usable(c, t)) return fail(TK_ERR_ARG, "count: grown table has %llu-bucket regions",
+                                            (unsigned long long)(1ull << t->rbits));
+  }
+  size_t fr = 0, tot = 0;
+  TK_HIP(hipMemGetInfo(&fr, &tot));
+  uint64_t held = 0;  // this context's own resident scratch counts as available
+  for (int i = kSlotResA; i <= kSlotResMisc; i++) held += c->scratch[i].bytes;
+  const uint64_t budget = t_resident_limit ? t_resident_limit : (uint64_t)(0.4 * (double)(fr + held));
+  const uint64_t cap_max = std::max<uint64_t>(budget / 17, 1ull << 20);  // two 8-B streams + histograms
+  // persistent merge grid (every block resident) and its reservation size
+  // (16K-64K words)
+  static int per_cu = 0;
+  if (!per_cu) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<4096, true>), 256, 4096 * 8) != hipSuccess ||
+        nb < 1)
+      nb = 1;
+    (void)hipGetLastError();
+    per_cu = nb;
+  }
+  const uint64_t mgrid = (uint64_t)std::max(c->num_cu, 1) * per_cu;
+  const uint64_t echunk = std::min<uint64_t>(65536, std::max<uint64_t>(16384, next_pow2(windows / (mgrid * 4))));
+  // Words reserved by the merge <= merged words (<= windows), plus < 4096
+  // abandoned per chunk (a flush that did not fit), plus each block's last
+  // chunk: below this bound no merged word is ever added to the table
+  // directly.
+  const uint64_t bound = windows + windows / (echunk / 4096 - 1) + mgrid * (echunk + 4096);
+  uint64_t cap = bound, chunk = re - rb;
+  if (bound > cap_max) {
+    // assume the merge halves the words; overflow goes straight to the table
+    cap = cap_max;
+    chunk = std::max<uint64_t>(2 * cap_max / std::max<uint32_t>(max_nwin, 1), kReadsPerBlock);
+  }
+  void *pa = nullptr, *pb = nullptr;
+  if ((rc = ensure_scratch(c, kSlotResA, cap * 8, &pa))) return rc;
+  if ((rc = ensure_scratch(c, kSlotResB, cap * 8, &pb))) return rc;
+  uint64_t* A = (uint64_t*)pa;
+  uint64_t* B = (uint64_t*)pb;
+  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
+  for (uint64_t q = rb; q < re; q += chunk) {
+    const uint64_t qe = std::min<uint64_t>(re, q + chunk);
+    const uint64_t keys_before = t->n_keys;
+    TableView v = table_view(t);
+    TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
+    // persistent: four blocks per CU (what its registers and LDS allow)
+    const uint64_t grid = std::max<uint64_t>(8, std::min<uint64_t>((ceil_div(qe - q, kReadsPerBlock) + 7) / 8 * 8, mgrid));
+    TK_LAUNCH(c, "count_merge", grid, 256, 4096 * 8, (k_count_agg<4096, true>), rview, q, qe, t->k,
+              t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipStreamSynchronize(c->stream));
+    if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table probe limit reached (flags=%u)", c->h_flags[0]);
+    uint64_t nnew = 0, n = 0;
+    memcpy(&nnew, &c->h_flags[2], 8);
+    memcpy(&n, &c->h_flags[8], 8);
+    t->n_keys += nnew;
+    n = std::min(n, cap);
+    if (!n) continue;
+    // coarse bins of F = 2^fs regions, then single regions
+    const uint32_t R = (uint32_t)(t->nb >> t->rbits);
+    uint32_t fs = 0;
+    while ((1ull << (2 * fs)) < R) fs++;
+    const uint32_t F = 1u << fs, C = (uint32_t)ceil_div(R, F);
+    const uint64_t ns1 = ceil_div(n, kResSlice);
+    const uint64_t ns2_max = ns1 + C;  // sum over coarse bins of ceil(n_c / slice)
+    const uint64_t nh1 = (uint64_t)C * ns1, nh_max = std::max<uint64_t>(nh1, (uint64_t)F * ns2_max);
+    void *hp = nullptr, *mp = nullptr;
+    const uint64_t sl_bytes = ns2_max * sizeof(ResSlice), hb_off = sl_bytes, m_off = hb_off + (uint64_t)C * 8,
+                   rs_off = (m_off + (uint64_t)C * 4 + 7) / 8 * 8, misc_bytes = rs_off + ((uint64_t)R + 1) * 8;
+    if ((rc = ensure_scratch(c, kSlotResHist, (nh_max + 1) * 16, &hp))) return rc;
+    if ((rc = ensure_scratch(c, kSlotResMisc, misc_bytes, &mp))) return rc;
+    uint64_t* h = (uint64_t*)hp;
+    uint64_t* hs = h + nh_max + 1;
+    ResSlice* dsl = (ResSlice*)mp;
+    uint64_t* dhb = (uint64_t*)((char*)mp + hb_off);
+    uint32_t* dm = (uint32_t*)((char*)mp + m_off);
+    uint64_t* drs = (uint64_t*)((char*)mp + rs_off);
+    std::vector<ResSlice> sl;
+    sl.reserve(ns2_max);
+    for (uint64_t j = 0; j < ns1; j++)
+      sl.push_back({j * kResSlice, std::min<uint64_t>(n, (j + 1) * kResSlice), 0, (uint32_t)ns1, (uint32_t)j});
+    if ((rc = res_sort_pass(c, sl, dsl, nh1, h, hs, A, B, v, fs, 0xFFFFFFFFu, C))) return rc;
+    std::vector<uint64_t> cs(C + 1);
+    TK_HIP(hipMemcpy2DAsync(cs.data(), 8, hs, ns1 * 8, 8, C, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(&cs[C], hs + nh1, 8, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipStreamSynchronize(c->stream));
+    const uint64_t total = cs[C];
+    std::vector<uint64_t> hb(C);
+    std::vector<uint32_t> m(C);
+    sl.clear();
+    uint64_t nh2 = 0;
+    for (uint32_t cc = 0; cc < C; cc++) {
+      const uint64_t ncc = cs[cc + 1] - cs[cc];
+      m[cc] = (uint32_t)std::max<uint64_t>(1, ceil_div(ncc, kResSlice));
+      hb[cc] = nh2;
+      for (uint32_t j = 0; j < m[cc]; j++)
+        sl.push_back({cs[cc] + j * kResSlice, std::min<uint64_t>(cs[cc + 1], cs[cc] + (j + 1) * kResSlice), nh2,
+                      m[cc], j});
+      nh2 += (uint64_t)F * m[cc];
+    }
+    if (sl.size() > ns2_max) return fail(TK_ERR_HIP, "count: %zu fine slices (bound %llu)", sl.size(),
+                                         (unsigned long long)ns2_max);
+    TK_HIP(hipMemcpyAsync(dhb, hb.data(), (size_t)C * 8, hipMemcpyHostToDevice, c->stream));
+    TK_HIP(hipMemcpyAsync(dm, m.data(), (size_t)C * 4, hipMemcpyHostToDevice, c->stream));
+    if ((rc = res_sort_pass(c, sl, dsl, nh2, h, hs, B, A, v, 0, F - 1, F))) return rc;
+    TK_LAUNCH(c, "count_bounds", ceil_div((uint64_t)R + 1, 256), 256, 0, k_res_bounds, hs, dhb, dm, fs, R, total,
+              drs);
+    TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
+    const size_t lds = (size_t)64 << t->rbits;
+    if (keys_before == 0 && nnew == 0)
+      TK_LAUNCH(c, "count_regions", R, 1024, lds, k_res_count<true>, A, drs, v, c->d_flags);
+    else
+      TK_LAUNCH(c, "count_regions", R, 1024, lds, k_res_count<false>, A, drs, v, c->d_flags);
+    TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipStreamSynchronize(c->stream));
+    if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table error (flags=%u)", c->h_flags[0]);
+    memcpy(&nnew, &c->h_flags[2], 8);
+    t->n_keys += nnew;
+    // words of crowded regions: grow, then add them (growing again while any
+    // still finds no room)
+    for (int round = 0; c->h_flags[6]; round++) {
+      if (round > 8) return fail(TK_ERR_FULL, "count: keys still find no room after growing");
+      if ((rc = table_grow(c, t, t->n_keys + total))) return rc;
+      TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
+      TableView v3 = table_view(t);
+      TK_LAUNCH(c, "count_retry", std::min<uint64_t>(ceil_div(total, 256), 65536), 256, 0, k_res_retry, A, total,
+                v3, c->d_flags);
+      TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
+      TK_HIP(hipStreamSynchronize(c->stream));
+      if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table error (flags=%u)", c->h_flags[0]);
+      memcpy(&nnew, &c->h_flags[2], 8);
+      t->n_keys += nnew;
+    }
+    if ((double)t->n_keys > t->max_load * (double)(t->nb * 8)) {
+      if ((rc = table_grow(c, t, t->n_keys))) return rc;
+      if (qe < re && !resident_usable(c, t))
+        return fail(TK_ERR_ARG, "count: grown table has %llu-bucket regions", (unsigned long long)(1ull << t->rbits));
+    }
+  }
+  return TK_OK;
 }
 
 int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
@@ -2154,6 +2697,9 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
   } else {
     rview.order = nullptr;
   }
+  if (rview.order && 63 - 2 * t->k >= 13 && agg_slots(c) && resident_enabled() && max_nwin &&
+      !partition_enabled() && resident_usable(c, t))
+    return count_resident(c, t, r, rview, 0, r->n_reads, total, max_nwin);
   uint64_t rb = 0;
   while (rb < r->n_reads) {
     const uint32_t nparts = (uint32_t)(t->nb >> t->rbits);
@@ -2186,11 +2732,11 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
     TableView v = table_view(t);
     const int agg = rview.order && 63 - 2 * t->k >= 13 ? agg_slots(c) : 0;
     if (agg == 8192)
-      TK_LAUNCH(c, "count", grid, 256, 8192 * 8, k_count_agg<8192>, rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags);
+      TK_LAUNCH(c, "count", grid, 256, 8192 * 8, (k_count_agg<8192, false>), rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
     else if (agg == 4096)
-      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, k_count_agg<4096>, rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags);
+      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false>), rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
     else if (coop8_enabled())
       TK_LAUNCH(c, "count", grid, 256, 0, k_count<8>, rview, rb, rb + nreads, t->k,
                 t->canonical, v, c->d_flags);

@@ -35,6 +35,9 @@ namespace tk {
 // the same few block sizes job after job without hipFree/hipMalloc.
 void cache_begin(size_t limit_bytes);
 void cache_end();
+// Device bytes this thread's region-resident counts may use for their word
+// streams (tk_lib.hip); larger jobs are counted in read chunks.
+void resident_limit(size_t bytes);
 }  // namespace tk
 
 namespace {
@@ -162,6 +165,7 @@ void worker(tk_server* s, int idx) {
   // Every object of a job is created and freed on this thread and its
   // context's stream, which is what the block cache requires.
   tk::cache_begin(s->budget / (2 * (uint64_t)s->opts.slots));
+  tk::resident_limit(s->budget / (2 * (uint64_t)s->opts.slots));
   tk_ctx* ctx = nullptr;
   int rc = tk_init(s->opts.device, &ctx);
   if (rc) {
