@@ -306,7 +306,8 @@ def bench_stress(args) -> dict:
         lat.append(time.perf_counter() - t)
         log(f"step {i}: {lat[-1]:.3f}s")
     kern = {}
-    for name in ["pack", "hll", "sort_pairs", "count", "stats", "prune", "ctable_build", "vote"]:
+    for name in ["pack", "hll", "sort_pairs", "count", "count_merge", "count_hist", "count_scatter",
+                 "count_regions", "stats", "prune", "ctable_build", "vote"]:
         tot, n = ctx.kernel_time(name)
         if n:
             kern[name] = {"total_ms": round(tot, 3), "launches": n, "avg_ms": round(tot / n, 4)}
@@ -352,6 +353,10 @@ def plain_job(ctx, seq: np.ndarray, off: np.ndarray, n: int, max_cov: int) -> in
 
 def bench_serving(args) -> dict:
     from trinityrnaseq_amd import api, synth
+    # both forms count with table atomics (the server's workers always do), so
+    # the comparison isolates the serving machinery: overlapped uploads, slots,
+    # admission, block reuse
+    os.environ["TK_RESIDENT"] = "0"
     rng = np.random.default_rng(2000)
     lo, hi = 0.5e6 * args.serve_scale, 30e6 * args.serve_scale
     sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), size=args.samples)).astype(np.int64)

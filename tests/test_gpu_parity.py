@@ -63,6 +63,7 @@ def layout(request, monkeypatch):
         if request.param == "ordered":
             monkeypatch.setenv("TK_RESIDENT", "0")
         else:
+            monkeypatch.setenv("TK_RESIDENT", "1")
             monkeypatch.setenv("TK_REGION_KB", "4")
     return request.param
 
@@ -145,6 +146,7 @@ def test_count_resident_crowded_regions(api, ctx, region_kb, monkeypatch):
     monkeypatch.setenv("TK_ORDER", "1")
     monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
     monkeypatch.setenv("TK_REGION_KB", region_kb)
+    monkeypatch.setenv("TK_RESIDENT", "1")
     seqs = random_reads(17, 20000, 100, 150, n_rate=0.001) + repetitive_reads(25, 3000)
     r, t, seq, off = _count(api, ctx, seqs, 25, True, expected=64, max_load=0.9)
     ok, oc = O.count(seq, off, 25, True)

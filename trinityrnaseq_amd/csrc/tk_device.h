@@ -291,6 +291,46 @@ __device__ __forceinline__ int lds_bucket_add(uint64_t* tab, uint32_t b, uint32_
   return 0;
 }
 
+// lds_bucket_add for 2k >= 32: the key's low 32 bits are all of the slot's
+// low word and the occupied bit lies in the high word, so a slot matches
+// when its low word equals the key's and its masked high word the tagged
+// key's, and it is empty when its high word is zero.  khi = high word of
+// the key + occupied mask.
+__device__ __forceinline__ int lds_bucket_add_khi(uint64_t* tab, uint32_t b, uint32_t bmask, uint64_t word,
+                                                  uint32_t khi, uint32_t cshift, uint32_t max_steps,
+                                                  uint64_t& old) {
+  const uint32_t tlo = (uint32_t)word, thi = (uint32_t)(word >> 32) & khi;
+  const uint64_t add = (word >> cshift) << cshift;
+  for (uint32_t p = 0; p < max_steps; p++) {
+    uint64_t* line = tab + ((uint64_t)b << 3);
+    const uint4* l4 = reinterpret_cast<const uint4*>(line);
+    const uint4 q0 = l4[0], q1 = l4[1], q2 = l4[2], q3 = l4[3];
+    const uint32_t lo[8] = {q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, q3.x, q3.z};
+    const uint32_t hi[8] = {q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, q3.y, q3.w};
+    int jm = 8, je = 8;
+#pragma unroll
+    for (int j = 7; j >= 0; j--) {
+      jm = (lo[j] == tlo && (hi[j] & khi) == thi) ? j : jm;
+      je = hi[j] == 0 ? j : je;
+    }
+    if (jm < 8) {
+      old = atomicAdd((unsigned long long*)&line[jm], (unsigned long long)add);
+      return 1;
+    }
+    if (je < 8) {
+      const uint64_t v = atomicCAS((unsigned long long*)&line[je], 0ull, (unsigned long long)word);
+      if (v == 0) return 2;
+      if ((uint32_t)v == tlo && ((uint32_t)(v >> 32) & khi) == thi) {
+        old = atomicAdd((unsigned long long*)&line[je], (unsigned long long)add);
+        return 1;
+      }
+      continue;  // another key took the slot: rescan this bucket
+    }
+    b = (b + 1) & bmask;
+  }
+  return 0;
+}
+
 // ---- wave helpers ----
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -772,8 +772,11 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
 // table as above; every reserved word below `cap` that holds no merged word
 // is zero, so out[0, min(*cursor, cap)) is always defined (0 = no word).
 
-template <int SLOTS, bool EMIT>
-__global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
+template <int SLOTS>
+constexpr int kAggBucketBits = SLOTS == 8192 ? 10 : 9;  // log2(SLOTS / 8)
+
+template <int SLOTS, bool EMIT, bool KHI = true>
+__global__ __launch_bounds__(256, EMIT ? 5 : 1) void k_count_agg(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
                                                    int canonical, TableView t, uint32_t* __restrict__ flags,
                                                    uint64_t* __restrict__ out, uint64_t cap, uint64_t chunk,
                                                    unsigned long long* __restrict__ cursor) {
@@ -825,15 +828,27 @@ __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begi
     // per round a thread shifts one base into its forward and reverse
     // k-mers (a code word is loaded once per 32 bases) and tracks the run of
     // ACGT bases ending there (a window is valid when the run is >= k).
+    // The two code / mask words holding the thread's first window are loaded
+    // here; later words when the rolling position reaches them.  The current
+    // words are kept shifted so that the next base sits in their top bits.
     const int rj = tid >> 2, rp = tid & 3;
     uint32_t rs = 0, rcnt = 0;
-    uint64_t rwb = 0;
+    uint64_t rwb = 0, c0 = 0, c1 = 0;
+    uint32_t m0 = 0, m1 = 0;
     if (rj < nr) {
       const uint32_t nw = s_off[rj + 1] - s_off[rj];
       const uint32_t q = (nw + 3) >> 2;
       rs = min(nw, (uint32_t)rp * q);
       rcnt = min(nw, rs + q) - rs;
       rwb = s_wb[rj];
+      if (rcnt) {
+        // words rs/32 and rs/32 + 1: the second is at most the word after
+        // the read's last one (the next read's first word or the padding)
+        c0 = rv.codes[rwb + (rs >> 5)];
+        m0 = rv.nmask[rwb + (rs >> 5)];
+        c1 = rv.codes[rwb + (rs >> 5) + 1];
+        m1 = rv.nmask[rwb + (rs >> 5) + 1];
+      }
     }
     const uint32_t wmax = wave_max_u32(rcnt);
     __syncthreads();  // every thread has read s_off
@@ -841,39 +856,53 @@ __global__ __launch_bounds__(256) void k_count_agg(ReadsView rv, uint64_t r_begi
     __syncthreads();
     const uint32_t rounds = max(max(s_nd[0], s_nd[1]), max(s_nd[2], s_nd[3]));
     const uint64_t kmask = (1ull << (2 * k)) - 1;
+    const uint32_t khi = (uint32_t)(t.kmo >> 32);
     uint64_t fwd = 0, rcm = 0, cw = 0;
-    uint32_t cm = 0, run = 0, pos = 0;
+    uint32_t cm = 0, run = 0, left = 0, wnext = 0;
     uint32_t since = 0;  // windows merged since the last flush (block-uniform)
     for (uint32_t r = 0; r < rounds; r++) {
       // 1. one window per thread into the LDS table
       bool isnew = false;
       if (r < rcnt) {
         if (r == 0) {
-          uint32_t nm;
-          window_at_mask(rv, rwb, rs, k, fwd, nm);
+          const uint32_t o = rs & 31u;
+          const uint64_t x = o ? ((c0 << (2 * o)) | (c1 >> (64 - 2 * o))) : c0;
+          const uint32_t mw = o ? ((m0 << o) | (m1 >> (32 - o))) : m0;
+          fwd = x >> (64 - 2 * k);
+          const uint32_t nm = mw >> (32 - k);
           rcm = revcomp(fwd, k);
           run = nm ? (uint32_t)__builtin_ctz(nm) : (uint32_t)k;
-          pos = rs + k;
-          cw = rv.codes[rwb + (pos >> 5)];
-          cm = rv.nmask[rwb + (pos >> 5)];
+          // the next base, rs + k, is in word rs/32 or rs/32 + 1 (k <= 32)
+          const uint32_t pos = rs + k, po = pos & 31u;
+          const bool second = (pos >> 5) != (rs >> 5);
+          cw = (second ? c1 : c0) << (2 * po);
+          cm = (second ? m1 : m0) << po;
+          left = 32 - po;
+          wnext = (pos >> 5) + 1;
         } else {
-          const uint32_t o = pos & 31u;
-          if (o == 0) {
-            cw = rv.codes[rwb + (pos >> 5)];
-            cm = rv.nmask[rwb + (pos >> 5)];
+          if (left == 0) {
+            cw = rv.codes[rwb + wnext];
+            cm = rv.nmask[rwb + wnext];
+            left = 32;
+            wnext++;
           }
-          const uint64_t b = (cw >> (62 - 2 * o)) & 3u;
+          const uint32_t b = (uint32_t)(cw >> 62);
+          const uint32_t isn = cm >> 31;
+          cw <<= 2;
+          cm <<= 1;
+          left--;
           fwd = ((fwd << 2) | b) & kmask;
-          rcm = (rcm >> 2) | ((3u - b) << (2 * k - 2));
-          run = ((cm >> (31 - o)) & 1u) ? 0u : run + 1;
-          pos++;
+          rcm = (rcm >> 2) | ((uint64_t)(b ^ 3u) << (2 * k - 2));
+          run = isn ? 0u : run + 1;
         }
         if (run >= (uint32_t)k) {
           const uint64_t tagged = ((canonical && rcm < fwd) ? rcm : fwd) | t.occ;
-          const uint32_t hb = (uint32_t)((tagged * 0x9E3779B97F4A7C15ULL) >> 40) & (SLOTS / 8 - 1);
+          const uint32_t lo = (uint32_t)tagged, hi = (uint32_t)(tagged >> 32);
+          const uint32_t hb = ((lo ^ (lo >> 15) ^ (hi << 7)) * 0x9E3779B1u) >> (32 - kAggBucketBits<SLOTS>);
           uint64_t old;
           // the table never holds more than kMaxDistinct + 256 < SLOTS keys
-          isnew = lds_bucket_add(s_tab, hb, SLOTS / 8 - 1, tagged | one, t.kmo, t.cshift, 4 * SLOTS, old) == 2;
+          isnew = (KHI ? lds_bucket_add_khi(s_tab, hb, SLOTS / 8 - 1, tagged | one, khi, t.cshift, 4 * SLOTS, old)
+                       : lds_bucket_add(s_tab, hb, SLOTS / 8 - 1, tagged | one, t.kmo, t.cshift, 4 * SLOTS, old)) == 2;
         }
       }
       const uint32_t wnew = (uint32_t)__popcll(__ballot(isnew));
@@ -1014,8 +1043,7 @@ __global__ __launch_bounds__(256) void k_res_hist(const uint64_t* __restrict__ i
 // Each tile of kResTile words is counting-sorted by bin in LDS first, so
 // the stores leave in runs of consecutive addresses (one run per bin and
 // tile) instead of one scattered 8-B store per word.
-constexpr uint32_t kResTile = 4096;
-static size_t res_scatter_lds(uint32_t nbins) { return (size_t)kResTile * 10 + (size_t)nbins * 16; }
+static size_t res_scatter_lds(uint32_t nbins, uint32_t tile) { return (size_t)tile * 10 + (size_t)nbins * 16; }
 
 // Exclusive scan of s_cnt[0, nbins) into s_loc (nbins <= 2048, blockDim
 // 256); returns the total.
@@ -1044,6 +1072,7 @@ __device__ __forceinline__ uint32_t res_bin_scan(const uint32_t* s_cnt, uint32_t
   return s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
+template <uint32_t kResTile>
 __global__ __launch_bounds__(256) void k_res_scatter(const uint64_t* __restrict__ in,
                                                      const ResSlice* __restrict__ sl, TableView t,
                                                      uint32_t shift, uint32_t mask, uint32_t nbins,
@@ -1098,6 +1127,14 @@ __global__ __launch_bounds__(256) void k_res_scatter(const uint64_t* __restrict_
     __syncthreads();
     for (uint32_t i = tid; i < nbins; i += blockDim.x) s_pos[i] += s_cnt[i];
   }
+}
+
+// First entry of every coarse bin of the scanned coarse histogram (bin c at
+// c * ns1) and the total, into out[0, C].
+__global__ void k_res_coarse_starts(const uint64_t* __restrict__ hs, uint64_t ns1, uint32_t C,
+                                    uint64_t* __restrict__ out) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= C) out[c] = hs[(uint64_t)c * ns1];
 }
 
 // First word of every region (region g = coarse g >> fs, fine g & (2^fs-1))
@@ -2463,7 +2500,9 @@ static int agg_slots(tk_ctx* c) {
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     big = 0;
     if (maxlds >= 8192 * 8 + 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, false>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, false, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, false, false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) == hipSuccess)
       big = 1;
     (void)hipGetLastError();
@@ -2471,11 +2510,18 @@ static int agg_slots(tk_ctx* c) {
   return big ? 8192 : 4096;
 }
 
-// TK_RESIDENT=0 turns the region-resident count off (on by default for
-// reads in the locality order, k <= 25, tables of 64 KB regions).
-static bool resident_enabled() {
+// Whether a count of `windows` windows takes the region-resident passes
+// (reads in the locality order, k <= 25, tables of 64 KB regions).  By
+// default only counts of >= 2^32 windows do: their word streams are tens of
+// GB held by the context, and on smaller jobs the gain does not pay for the
+// memory (a context that holds them leaves less room for the per-job tables
+// of a sample stream, whose allocations then stall).  TK_RESIDENT=0 never,
+// TK_RESIDENT=1 always (when the streams fit).
+static bool resident_enabled(uint64_t windows) {
   const char* e = getenv("TK_RESIDENT");
-  return !(e && e[0] == '0');
+  if (e && e[0] == '0') return false;
+  if (e && e[0] == '1') return true;
+  return windows >= (1ull << 32);
 }
 
 static bool resident_usable(tk_ctx* c, const tk_table* t) {
@@ -2486,13 +2532,23 @@ static bool resident_usable(tk_ctx* c, const tk_table* t) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, 64 << kResRegionBitsMax) == hipSuccess &&
          hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<false>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, 64 << kResRegionBitsMax) == hipSuccess &&
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter),
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<4096>),
                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)res_scatter_lds(kResMaxBins)) == hipSuccess;
+                             (int)res_scatter_lds(kResMaxBins, 4096)) == hipSuccess &&
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<2048>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)res_scatter_lds(kResMaxBins, 2048)) == hipSuccess;
     (void)hipGetLastError();
   }
   (void)c;
   return ok == 1;
+}
+
+// Words per LDS tile of the region sort's scatter: TK_RES_TILE=2048 or 4096
+// (default).
+static uint32_t res_tile() {
+  const char* e = getenv("TK_RES_TILE");
+  return e && atoi(e) == 2048 ? 2048 : 4096;
 }
 
 // Scratch slots of the region-resident count (kept by the context, so the
@@ -2510,18 +2566,55 @@ static int res_sort_pass(tk_ctx* c, const std::vector<ResSlice>& sl, ResSlice* d
   TK_LAUNCH(c, "count_hist", sl.size(), 256, 0, k_res_hist, in, dsl, v, shift, mask, nbins, h);
   int rc = exclusive_scan_u64(c, h, hs, nhist + 1);
   if (rc) return rc;
-  TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins), k_res_scatter, in, dsl, v, shift, mask,
-            nbins, hs, out);
+  if (res_tile() == 2048)
+    TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins, 2048), k_res_scatter<2048>, in, dsl, v,
+              shift, mask, nbins, hs, out);
+  else
+    TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins, 4096), k_res_scatter<4096>, in, dsl, v,
+              shift, mask, nbins, hs, out);
   // the slice table is host memory read by the copy above
   TK_HIP(hipStreamSynchronize(c->stream));
   return TK_OK;
 }
 
+// Stream capacity of a region-resident count of `windows` windows: the
+// words the merge can reserve (merged words <= windows, plus < 4096 given
+// up per reserved chunk of `echunk`, plus each block's last chunk), so no
+// merged word ever misses the stream.  False when that does not fit the
+// device memory the count may use: the table atomics then count instead
+// (the merge of a very large or error-rich read set leaves too many words).
+static bool resident_plan(tk_ctx* c, uint64_t windows, uint64_t* cap, uint64_t* echunk, uint64_t* mgrid,
+                          uint64_t* cap_max) {
+  static int per_cu = 0;
+  if (!per_cu) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<4096, true, true>), 256, 4096 * 8) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    (void)hipGetLastError();
+    per_cu = nb;
+  }
+  *mgrid = (uint64_t)std::max(c->num_cu, 1) * per_cu;
+  *echunk = std::min<uint64_t>(65536, std::max<uint64_t>(16384, next_pow2(windows / (*mgrid * 4))));
+  *cap = windows + windows / (*echunk / 4096 - 1) + *mgrid * (*echunk + 4096);
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  uint64_t held = 0;  // this context's own resident scratch counts as available
+  for (int i = kSlotResA; i <= kSlotResMisc; i++) held += c->scratch[i].bytes;
+  const uint64_t budget = t_resident_limit ? t_resident_limit : (uint64_t)(0.4 * (double)(fr + held));
+  *cap_max = budget / 17;  // two 8-B streams + histograms
+  return *cap <= *cap_max;
+}
+constexpr int kNotResident = -1000;
+
 // Count reads [rb, re) of the locality order through the region-resident
-// passes: merge + emit, coarse sort, fine sort, per-region LDS count.  The
-// streams hold at most `cap` words per pass; reads are taken in chunks whose
-// windows fit (merged words that would still overflow are added to the
-// table directly by the merge kernel).
+// passes: merge + emit, coarse sort, fine sort, per-region LDS count, in one
+// pass whose streams hold every word the merge can emit (resident_plan);
+// kNotResident, with the table unchanged, when they do not fit.
 static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const ReadsView& rview, uint64_t rb,
                           uint64_t re, uint64_t windows, uint32_t max_nwin) {
   int rc = TK_OK;
@@ -2539,36 +2632,10 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     if (!resident_usable(c, t)) return fail(TK_ERR_ARG, "count: grown table has %llu-bucket regions",
                                             (unsigned long long)(1ull << t->rbits));
   }
-  size_t fr = 0, tot = 0;
-  TK_HIP(hipMemGetInfo(&fr, &tot));
-  uint64_t held = 0;  // this context's own resident scratch counts as available
-  for (int i = kSlotResA; i <= kSlotResMisc; i++) held += c->scratch[i].bytes;
-  const uint64_t budget = t_resident_limit ? t_resident_limit : (uint64_t)(0.4 * (double)(fr + held));
-  const uint64_t cap_max = std::max<uint64_t>(budget / 17, 1ull << 20);  // two 8-B streams + histograms
-  // persistent merge grid (every block resident) and its reservation size
-  // (16K-64K words)
-  static int per_cu = 0;
-  if (!per_cu) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<4096, true>), 256, 4096 * 8) != hipSuccess ||
-        nb < 1)
-      nb = 1;
-    (void)hipGetLastError();
-    per_cu = nb;
-  }
-  const uint64_t mgrid = (uint64_t)std::max(c->num_cu, 1) * per_cu;
-  const uint64_t echunk = std::min<uint64_t>(65536, std::max<uint64_t>(16384, next_pow2(windows / (mgrid * 4))));
-  // Words reserved by the merge <= merged words (<= windows), plus < 4096
-  // abandoned per chunk (a flush that did not fit), plus each block's last
-  // chunk: below this bound no merged word is ever added to the table
-  // directly.
-  const uint64_t bound = windows + windows / (echunk / 4096 - 1) + mgrid * (echunk + 4096);
-  uint64_t cap = bound, chunk = re - rb;
-  if (bound > cap_max) {
-    // assume the merge halves the words; overflow goes straight to the table
-    cap = cap_max;
-    chunk = std::max<uint64_t>(2 * cap_max / std::max<uint32_t>(max_nwin, 1), kReadsPerBlock);
-  }
+  uint64_t cap = 0, echunk = 0, mgrid = 0, cap_max = 0;
+  if (!resident_plan(c, windows, &cap, &echunk, &mgrid, &cap_max)) return kNotResident;
+  const uint64_t chunk = re - rb;
+  (void)cap_max;
   void *pa = nullptr, *pb = nullptr;
   if ((rc = ensure_scratch(c, kSlotResA, cap * 8, &pa))) return rc;
   if ((rc = ensure_scratch(c, kSlotResB, cap * 8, &pb))) return rc;
@@ -2582,8 +2649,12 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
     // persistent: four blocks per CU (what its registers and LDS allow)
     const uint64_t grid = std::max<uint64_t>(8, std::min<uint64_t>((ceil_div(qe - q, kReadsPerBlock) + 7) / 8 * 8, mgrid));
-    TK_LAUNCH(c, "count_merge", grid, 256, 4096 * 8, (k_count_agg<4096, true>), rview, q, qe, t->k,
-              t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    if (t->k >= 16)
+      TK_LAUNCH(c, "count_merge", grid, 256, 4096 * 8, (k_count_agg<4096, true, true>), rview, q, qe, t->k,
+                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else
+      TK_LAUNCH(c, "count_merge", grid, 256, 4096 * 8, (k_count_agg<4096, true, false>), rview, q, qe, t->k,
+                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table probe limit reached (flags=%u)", c->h_flags[0]);
@@ -2603,7 +2674,8 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     const uint64_t nh1 = (uint64_t)C * ns1, nh_max = std::max<uint64_t>(nh1, (uint64_t)F * ns2_max);
     void *hp = nullptr, *mp = nullptr;
     const uint64_t sl_bytes = ns2_max * sizeof(ResSlice), hb_off = sl_bytes, m_off = hb_off + (uint64_t)C * 8,
-                   rs_off = (m_off + (uint64_t)C * 4 + 7) / 8 * 8, misc_bytes = rs_off + ((uint64_t)R + 1) * 8;
+                   rs_off = (m_off + (uint64_t)C * 4 + 7) / 8 * 8, cs_off = rs_off + ((uint64_t)R + 1) * 8,
+                   misc_bytes = cs_off + ((uint64_t)C + 1) * 8;
     if ((rc = ensure_scratch(c, kSlotResHist, (nh_max + 1) * 16, &hp))) return rc;
     if ((rc = ensure_scratch(c, kSlotResMisc, misc_bytes, &mp))) return rc;
     uint64_t* h = (uint64_t*)hp;
@@ -2612,14 +2684,15 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     uint64_t* dhb = (uint64_t*)((char*)mp + hb_off);
     uint32_t* dm = (uint32_t*)((char*)mp + m_off);
     uint64_t* drs = (uint64_t*)((char*)mp + rs_off);
+    uint64_t* dcs = (uint64_t*)((char*)mp + cs_off);
     std::vector<ResSlice> sl;
     sl.reserve(ns2_max);
     for (uint64_t j = 0; j < ns1; j++)
       sl.push_back({j * kResSlice, std::min<uint64_t>(n, (j + 1) * kResSlice), 0, (uint32_t)ns1, (uint32_t)j});
     if ((rc = res_sort_pass(c, sl, dsl, nh1, h, hs, A, B, v, fs, 0xFFFFFFFFu, C))) return rc;
     std::vector<uint64_t> cs(C + 1);
-    TK_HIP(hipMemcpy2DAsync(cs.data(), 8, hs, ns1 * 8, 8, C, hipMemcpyDeviceToHost, c->stream));
-    TK_HIP(hipMemcpyAsync(&cs[C], hs + nh1, 8, hipMemcpyDeviceToHost, c->stream));
+    TK_LAUNCH(c, "count_bounds", ceil_div((uint64_t)C + 1, 256), 256, 0, k_res_coarse_starts, hs, ns1, C, dcs);
+    TK_HIP(hipMemcpyAsync(cs.data(), dcs, ((size_t)C + 1) * 8, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     const uint64_t total = cs[C];
     std::vector<uint64_t> hb(C);
@@ -2697,9 +2770,12 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
   } else {
     rview.order = nullptr;
   }
-  if (rview.order && 63 - 2 * t->k >= 13 && agg_slots(c) && resident_enabled() && max_nwin &&
+  if (rview.order && 63 - 2 * t->k >= 13 && agg_slots(c) && resident_enabled(total) && max_nwin &&
       !partition_enabled() && resident_usable(c, t))
-    return count_resident(c, t, r, rview, 0, r->n_reads, total, max_nwin);
+  {
+    rc = count_resident(c, t, r, rview, 0, r->n_reads, total, max_nwin);
+    if (rc != kNotResident) return rc;
+  }
   uint64_t rb = 0;
   while (rb < r->n_reads) {
     const uint32_t nparts = (uint32_t)(t->nb >> t->rbits);
@@ -2731,11 +2807,17 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
     const uint64_t grid = (ceil_div(nreads, kReadsPerBlock) + 7) / 8 * 8;
     TableView v = table_view(t);
     const int agg = rview.order && 63 - 2 * t->k >= 13 ? agg_slots(c) : 0;
-    if (agg == 8192)
-      TK_LAUNCH(c, "count", grid, 256, 8192 * 8, (k_count_agg<8192, false>), rview, rb, rb + nreads, t->k,
+    if (agg == 8192 && t->k >= 16)
+      TK_LAUNCH(c, "count", grid, 256, 8192 * 8, (k_count_agg<8192, false, true>), rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
+    else if (agg == 8192)
+      TK_LAUNCH(c, "count", grid, 256, 8192 * 8, (k_count_agg<8192, false, false>), rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
+    else if (agg == 4096 && t->k >= 16)
+      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false, true>), rview, rb, rb + nreads, t->k,
                 t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
     else if (agg == 4096)
-      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false>), rview, rb, rb + nreads, t->k,
+      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false, false>), rview, rb, rb + nreads, t->k,
                 t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
     else if (coop8_enabled())
       TK_LAUNCH(c, "count", grid, 256, 0, k_count<8>, rview, rb, rb + nreads, t->k,

@@ -36,7 +36,7 @@ namespace tk {
 void cache_begin(size_t limit_bytes);
 void cache_end();
 // Device bytes this thread's region-resident counts may use for their word
-// streams (tk_lib.hip); larger jobs are counted in read chunks.
+// streams (tk_lib.hip); a count whose streams do not fit uses table atomics.
 void resident_limit(size_t bytes);
 }  // namespace tk
 
@@ -165,7 +165,9 @@ void worker(tk_server* s, int idx) {
   // Every object of a job is created and freed on this thread and its
   // context's stream, which is what the block cache requires.
   tk::cache_begin(s->budget / (2 * (uint64_t)s->opts.slots));
-  tk::resident_limit(s->budget / (2 * (uint64_t)s->opts.slots));
+  // jobs run side by side under the HBM budget: count with table atomics
+  // rather than holding region-resident word streams per worker
+  tk::resident_limit(1);
   tk_ctx* ctx = nullptr;
   int rc = tk_init(s->opts.device, &ctx);
   if (rc) {
