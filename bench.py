@@ -31,8 +31,14 @@ METRIC = "k-mers/s count+score vs plain GPU & host CPU; p50/p99 job latency, bit
 HBM_PEAK_GBS = 8000.0
 K = 25
 READ_LEN = 150
-# Algorithmic bytes (per k-mer window / per read) of the two dominant kernels.
-# count: 0.375 B packed base + 64 B bucket line read + 64 B write-back; per read 12 B (len, offset)
+# Algorithmic bytes of the table kernels.
+# count with table atomics: 0.375 B packed base + 64 B bucket line read + 64 B write-back per
+#   window; per read 12 B (len, offset)
+# region-resident count, per merged word w (8 B) of the stream:
+#   count_merge   0.375 B per packed base read + 8 B per word written
+#   count_hist    8 B per word read, two passes (coarse bins, then regions)
+#   count_scatter 8 B read + 8 B written per word, two passes
+#   count_regions 8 B per word read + every table line written once (fresh table)
 # stats: 0.375 B packed base + 64 B bucket line read; per read 12 B in + 32 B stats out
 COUNT_B_PER_WIN, COUNT_B_PER_READ = 128.375, 12.0
 STATS_B_PER_WIN, STATS_B_PER_READ = 64.375, 44.0
@@ -40,7 +46,7 @@ STATS_B_PER_WIN, STATS_B_PER_READ = 64.375, 44.0
 # 5.04 G windows, counters calibrated against kernels of the same access shape), from
 # profiles/large_pe_20M_pairs_count_stats.md.  The counters cannot be read inside this
 # process, so the profiled figures are carried here for roofline.traffic.
-TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (210.9 + 1.3) / 5.04}
+TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (214.5 + 1.3) / 5.04}
 
 
 def log(*a):
@@ -560,10 +566,17 @@ def main() -> int:
         tot, n = ctx.kernel_time(name)
         if n:
             kern[name] = {"total_ms": round(tot, 3), "launches": n, "avg_ms": round(tot / n, 4)}
+    words = ctx.count_words()  # merged words of the last step's count (0: table atomics)
     count_bytes = wins_per_step * COUNT_B_PER_WIN + n_reads * COUNT_B_PER_READ
     stats_bytes = wins_per_step * STATS_B_PER_WIN + n_reads * STATS_B_PER_READ
+    table_bytes = info["capacity"] * 8
+    stage_bytes = [("count", count_bytes), ("part_count", count_bytes), ("stats", stats_bytes)]
+    if words:
+        stage_bytes += [("count_merge", n_reads * READ_LEN * 0.375 + 8 * words),
+                        ("count_hist", 2 * 8 * words), ("count_scatter", 2 * 16 * words),
+                        ("count_regions", 8 * words + table_bytes)]
     roof = {}
-    for name, by in (("count", count_bytes), ("part_count", count_bytes), ("stats", stats_bytes)):
+    for name, by in stage_bytes:
         if name in kern:
             sec = kern[name]["total_ms"] / 1000.0 / args.steps
             roof[name] = {"achieved_gbs": by / sec / 1e9, "bytes_per_step": by,
@@ -581,7 +594,9 @@ def main() -> int:
                     "traffic_source": "profiles/large_pe_20M_pairs_count_stats.md (FETCH_SIZE + WRITE_SIZE)",
                     "algorithmic_bytes_per_launch": round(roof[dom]["bytes_per_step"] / launches),
                     "kernel": dom,
-                    "other": {n: round(v["achieved_gbs"], 1) for n, v in roof.items()}}
+                    "other": {n: {"gbs": round(v["achieved_gbs"], 1), "ms": round(v["kernel_ms_per_step"], 3)}
+                              for n, v in roof.items()},
+                    "merged_words": words}
 
     # the same job with the reads still in pageable host memory (upload included): not `value`
     t = time.perf_counter()

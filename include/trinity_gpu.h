@@ -78,6 +78,11 @@ int tk_timing_reset(tk_ctx* ctx);
 int tk_timing_get(tk_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
 /* Device memory currently held by objects of this process (bytes). */
 int tk_device_bytes_in_use(uint64_t* bytes);
+/* Words (distinct keys per merge, with their occurrences) that the last
+   tk_count on this context streamed through its region-resident passes; 0
+   when it counted with table atomics.  Diagnostic, for the traffic figures of
+   the count (no counterpart in the reference). */
+int tk_count_words(tk_ctx* ctx, uint64_t* words);
 
 /* Plain device buffers (inputs kept resident in HBM, outputs of _d calls). */
 int tk_buffer_alloc(tk_ctx* ctx, uint64_t bytes, void** d_ptr);
@@ -127,7 +132,10 @@ int tk_table_create(tk_ctx* ctx, int k, int canonical, uint64_t expected_distinc
                     double max_load, tk_table** out);
 int tk_table_info(const tk_table* t, int* k, int* canonical, uint64_t* capacity,
                   uint64_t* device_bytes);
-/* Count every N-free window of reads (jellyfish count).  Accumulates. */
+/* Count every N-free window of reads (jellyfish count).  Accumulates.  Large
+   counts (>= 2^32 windows) group their merged keys by table region and count
+   each region in LDS, keeping two word streams as context scratch (freed by
+   tk_ctx_free); others add merged keys to the table with atomics. */
 int tk_count(tk_ctx* ctx, tk_table* t, const tk_reads* reads);
 /* Number of keys whose count lies in [min_count, max_count]. */
 int tk_table_dump_size(tk_ctx* ctx, tk_table* t, uint32_t min_count, uint32_t max_count,

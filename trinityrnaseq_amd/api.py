@@ -37,6 +37,7 @@ SIGNATURES = {
     "tk_timing_reset": (C.c_int, [_P]),
     "tk_timing_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), _U64P]),
     "tk_device_bytes_in_use": (C.c_int, [_U64P]),
+    "tk_count_words": (C.c_int, [_P, _U64P]),
     "tk_buffer_alloc": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
     "tk_buffer_free": (C.c_int, [_P, _P]),
     "tk_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_uint64]),
@@ -161,6 +162,12 @@ class Context:
 
     def timing_reset(self):
         _check(lib().tk_timing_reset(self.h), "tk_timing_reset")
+
+    def count_words(self) -> int:
+        """Merged words of the last count's region-resident passes (0: table atomics)."""
+        w = C.c_uint64()
+        _check(lib().tk_count_words(self.h, C.byref(w)), "tk_count_words")
+        return w.value
 
     def kernel_time(self, name: str) -> tuple[float, int]:
         ms = C.c_double()

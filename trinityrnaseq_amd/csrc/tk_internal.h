@@ -62,6 +62,7 @@ struct tk_ctx {
   int num_cu = 0;
   void* stage[2] = {nullptr, nullptr};        // pinned staging for pageable uploads
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  uint64_t count_words = 0;  // merged words of the last region-resident count
 };
 
 struct tk_reads {

@@ -2017,6 +2017,12 @@ int tk_sync(tk_ctx* c) {
   return collect_timing(c);
 }
 
+int tk_count_words(tk_ctx* c, uint64_t* words) {
+  if (!c || !words) return fail(TK_ERR_ARG, "NULL argument");
+  *words = c->count_words;
+  return TK_OK;
+}
+
 int tk_timing_enable(tk_ctx* c, int on) {
   if (!c) return fail(TK_ERR_ARG, "ctx is NULL");
   c->timing = on != 0;
@@ -2695,6 +2701,7 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     TK_HIP(hipMemcpyAsync(cs.data(), dcs, ((size_t)C + 1) * 8, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     const uint64_t total = cs[C];
+    c->count_words += total;
     std::vector<uint64_t> hb(C);
     std::vector<uint32_t> m(C);
     sl.clear();
@@ -2754,6 +2761,7 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
   if (!c || !t || !r) return fail(TK_ERR_ARG, "NULL argument");
   if (t->device != c->device || r->device != c->device) return fail(TK_ERR_ARG, "device mismatch");
   if (set_device(c->device)) return TK_ERR_HIP;
+  c->count_words = 0;
   if (!r->n_reads) return TK_OK;
   int rc = table_set_final(c, t, false);
   if (rc) return rc;
