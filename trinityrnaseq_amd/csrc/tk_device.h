@@ -250,14 +250,14 @@ __device__ __forceinline__ uint64_t slot_count(const TableView& t, uint64_t s) {
 }
 
 // ---- LDS tables of 64-B buckets (eight slots in the global slot format) ----
-// Insert-or-add into an LDS table: `word` = tagged key | c << cshift.  From
-// bucket b (wrapping with bmask) each step reads one bucket, picks without
+// Insert-or-add into an LDS table of nb buckets: `word` = tagged key | c <<
+// cshift.  From bucket b (wrapping at nb) each step reads one bucket, picks without
 // branching the slot holding the key or else the first empty one, and
 // issues a single atomic there: add c, or claim with CAS (a lost claim
 // rescans the same bucket).  Occupied slots stay a prefix of their bucket.
 // Returns 2 when a slot was claimed, 1 when added (old = previous slot
 // value), 0 when max_steps steps found no room.
-__device__ __forceinline__ int lds_bucket_add(uint64_t* tab, uint32_t b, uint32_t bmask, uint64_t word,
+__device__ __forceinline__ int lds_bucket_add(uint64_t* tab, uint32_t b, uint32_t nb, uint64_t word,
                                               uint64_t kmo, uint32_t cshift, uint32_t max_steps,
                                               uint64_t& old) {
   const uint64_t tagged = word & kmo;
@@ -286,7 +286,7 @@ __device__ __forceinline__ int lds_bucket_add(uint64_t* tab, uint32_t b, uint32_
       }
       continue;  // another key took the slot: rescan this bucket
     }
-    b = (b + 1) & bmask;
+    b = b + 1 == nb ? 0 : b + 1;
   }
   return 0;
 }
@@ -296,7 +296,7 @@ __device__ __forceinline__ int lds_bucket_add(uint64_t* tab, uint32_t b, uint32_
 // when its low word equals the key's and its masked high word the tagged
 // key's, and it is empty when its high word is zero.  khi = high word of
 // the key + occupied mask.
-__device__ __forceinline__ int lds_bucket_add_khi(uint64_t* tab, uint32_t b, uint32_t bmask, uint64_t word,
+__device__ __forceinline__ int lds_bucket_add_khi(uint64_t* tab, uint32_t b, uint32_t nb, uint64_t word,
                                                   uint32_t khi, uint32_t cshift, uint32_t max_steps,
                                                   uint64_t& old) {
   const uint32_t tlo = (uint32_t)word, thi = (uint32_t)(word >> 32) & khi;
@@ -326,7 +326,7 @@ __device__ __forceinline__ int lds_bucket_add_khi(uint64_t* tab, uint32_t b, uin
       }
       continue;  // another key took the slot: rescan this bucket
     }
-    b = (b + 1) & bmask;
+    b = b + 1 == nb ? 0 : b + 1;
   }
   return 0;
 }

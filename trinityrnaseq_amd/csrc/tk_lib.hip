@@ -772,11 +772,8 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
 // table as above; every reserved word below `cap` that holds no merged word
 // is zero, so out[0, min(*cursor, cap)) is always defined (0 = no word).
 
-template <int SLOTS>
-constexpr int kAggBucketBits = SLOTS == 8192 ? 10 : 9;  // log2(SLOTS / 8)
-
 template <int SLOTS, bool EMIT, bool KHI = true>
-__global__ __launch_bounds__(256, EMIT ? 5 : 1) void k_count_agg(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
+__global__ __launch_bounds__(256, EMIT ? (SLOTS <= 3072 ? 6 : 5) : 1) void k_count_agg(ReadsView rv, uint64_t r_begin, uint64_t r_end, int k,
                                                    int canonical, TableView t, uint32_t* __restrict__ flags,
                                                    uint64_t* __restrict__ out, uint64_t cap, uint64_t chunk,
                                                    unsigned long long* __restrict__ cursor) {
@@ -857,29 +854,31 @@ __global__ __launch_bounds__(256, EMIT ? 5 : 1) void k_count_agg(ReadsView rv, u
     const uint32_t rounds = max(max(s_nd[0], s_nd[1]), max(s_nd[2], s_nd[3]));
     const uint64_t kmask = (1ull << (2 * k)) - 1;
     const uint32_t khi = (uint32_t)(t.kmo >> 32);
+    // the thread's first window, and its rolling state after it
     uint64_t fwd = 0, rcm = 0, cw = 0;
     uint32_t cm = 0, run = 0, left = 0, wnext = 0;
+    if (rcnt) {
+      const uint32_t o = rs & 31u;
+      const uint64_t x = o ? ((c0 << (2 * o)) | (c1 >> (64 - 2 * o))) : c0;
+      const uint32_t mw = o ? ((m0 << o) | (m1 >> (32 - o))) : m0;
+      fwd = x >> (64 - 2 * k);
+      const uint32_t nm = mw >> (32 - k);
+      rcm = revcomp(fwd, k);
+      run = nm ? (uint32_t)__builtin_ctz(nm) : (uint32_t)k;
+      // the next base, rs + k, is in word rs/32 or rs/32 + 1 (k <= 32)
+      const uint32_t pos = rs + k, po = pos & 31u;
+      const bool second = (pos >> 5) != (rs >> 5);
+      cw = (second ? c1 : c0) << (2 * po);
+      cm = (second ? m1 : m0) << po;
+      left = 32 - po;
+      wnext = (pos >> 5) + 1;
+    }
     uint32_t since = 0;  // windows merged since the last flush (block-uniform)
     for (uint32_t r = 0; r < rounds; r++) {
       // 1. one window per thread into the LDS table
       bool isnew = false;
       if (r < rcnt) {
-        if (r == 0) {
-          const uint32_t o = rs & 31u;
-          const uint64_t x = o ? ((c0 << (2 * o)) | (c1 >> (64 - 2 * o))) : c0;
-          const uint32_t mw = o ? ((m0 << o) | (m1 >> (32 - o))) : m0;
-          fwd = x >> (64 - 2 * k);
-          const uint32_t nm = mw >> (32 - k);
-          rcm = revcomp(fwd, k);
-          run = nm ? (uint32_t)__builtin_ctz(nm) : (uint32_t)k;
-          // the next base, rs + k, is in word rs/32 or rs/32 + 1 (k <= 32)
-          const uint32_t pos = rs + k, po = pos & 31u;
-          const bool second = (pos >> 5) != (rs >> 5);
-          cw = (second ? c1 : c0) << (2 * po);
-          cm = (second ? m1 : m0) << po;
-          left = 32 - po;
-          wnext = (pos >> 5) + 1;
-        } else {
+        if (r > 0) {
           if (left == 0) {
             cw = rv.codes[rwb + wnext];
             cm = rv.nmask[rwb + wnext];
@@ -898,11 +897,11 @@ __global__ __launch_bounds__(256, EMIT ? 5 : 1) void k_count_agg(ReadsView rv, u
         if (run >= (uint32_t)k) {
           const uint64_t tagged = ((canonical && rcm < fwd) ? rcm : fwd) | t.occ;
           const uint32_t lo = (uint32_t)tagged, hi = (uint32_t)(tagged >> 32);
-          const uint32_t hb = ((lo ^ (lo >> 15) ^ (hi << 7)) * 0x9E3779B1u) >> (32 - kAggBucketBits<SLOTS>);
+          const uint32_t hb = __umulhi((lo ^ (lo >> 15) ^ (hi << 7)) * 0x9E3779B1u, SLOTS / 8);
           uint64_t old;
           // the table never holds more than kMaxDistinct + 256 < SLOTS keys
-          isnew = (KHI ? lds_bucket_add_khi(s_tab, hb, SLOTS / 8 - 1, tagged | one, khi, t.cshift, 4 * SLOTS, old)
-                       : lds_bucket_add(s_tab, hb, SLOTS / 8 - 1, tagged | one, t.kmo, t.cshift, 4 * SLOTS, old)) == 2;
+          isnew = (KHI ? lds_bucket_add_khi(s_tab, hb, SLOTS / 8, tagged | one, khi, t.cshift, 4 * SLOTS, old)
+                       : lds_bucket_add(s_tab, hb, SLOTS / 8, tagged | one, t.kmo, t.cshift, 4 * SLOTS, old)) == 2;
         }
       }
       const uint32_t wnew = (uint32_t)__popcll(__ballot(isnew));
@@ -1195,7 +1194,7 @@ __global__ __launch_bounds__(1024) void k_res_count(uint64_t* __restrict__ words
       const uint32_t b = (uint32_t)(bucket0(t, w & (t.occ - 1)) - t.boff) & rmask;
       uint64_t old = 0;
       // every bucket of the region, each rescanned at most 8 times
-      const int res = lds_bucket_add(s_reg, b, rmask, w, t.kmo, t.cshift, 9 * (rmask + 1), old);
+      const int res = lds_bucket_add(s_reg, b, rmask + 1, w, t.kmo, t.cshift, 9 * (rmask + 1), old);
       const bool carry = res == 1 && (old >> t.cshift) + c > t.cmax;
       if (res == 2) mynew++;
       if (carry) ovf_add(t, tagged, t.cmax + 1, &flags[0]);
@@ -2583,27 +2582,56 @@ static int res_sort_pass(tk_ctx* c, const std::vector<ResSlice>& sl, ResSlice* d
   return TK_OK;
 }
 
+// LDS words of the merge table of the region-resident count: TK_RES_SLOTS =
+// 3840 (default, 30 KB: five blocks per CU), 3072 (24 KB: six blocks),
+// 4096 (32 KB: four blocks, with the block's other LDS) or 8192 (64 KB, two
+// blocks).
+static int res_slots() {
+  static int slots = 0;
+  if (!slots) {
+    const char* e = getenv("TK_RES_SLOTS");
+    const int v = e ? atoi(e) : 0;
+    slots = v == 8192 || v == 4096 || v == 3072 ? v : 3840;
+    if (slots == 8192 &&
+        (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, true, true>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess ||
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, true, false>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess)) {
+      (void)hipGetLastError();
+      slots = 4096;
+    }
+  }
+  return slots;
+}
+
 // Stream capacity of a region-resident count of `windows` windows: the
-// words the merge can reserve (merged words <= windows, plus < 4096 given
-// up per reserved chunk of `echunk`, plus each block's last chunk), so no
-// merged word ever misses the stream.  False when that does not fit the
+// words the merge can reserve (merged words <= windows, plus < one flush
+// (at most `slots` words) given up per reserved chunk of `echunk`, plus each
+// block's last chunk), so no merged word ever misses the stream.  False when that does not fit the
 // device memory the count may use: the table atomics then count instead
 // (the merge of a very large or error-rich read set leaves too many words).
 static bool resident_plan(tk_ctx* c, uint64_t windows, uint64_t* cap, uint64_t* echunk, uint64_t* mgrid,
                           uint64_t* cap_max) {
-  static int per_cu = 0;
-  if (!per_cu) {
+  const uint64_t slots = (uint64_t)res_slots();
+  static int per_cu[4] = {0, 0, 0, 0};
+  int& pc = per_cu[slots == 8192 ? 2 : slots == 4096 ? 1 : slots == 3072 ? 3 : 0];
+  if (!pc) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<4096, true, true>), 256, 4096 * 8) !=
-            hipSuccess ||
-        nb < 1)
-      nb = 1;
+    const hipError_t e =
+        slots == 8192
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<8192, true, true>), 256, 8192 * 8)
+        : slots == 4096
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<4096, true, true>), 256, 4096 * 8)
+        : slots == 3072
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<3072, true, true>), 256, 3072 * 8)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<3840, true, true>), 256, 3840 * 8);
+    if (e != hipSuccess || nb < 1) nb = 1;
     (void)hipGetLastError();
-    per_cu = nb;
+    pc = nb;
   }
-  *mgrid = (uint64_t)std::max(c->num_cu, 1) * per_cu;
+  *mgrid = (uint64_t)std::max(c->num_cu, 1) * pc;
   *echunk = std::min<uint64_t>(65536, std::max<uint64_t>(16384, next_pow2(windows / (*mgrid * 4))));
-  *cap = windows + windows / (*echunk / 4096 - 1) + *mgrid * (*echunk + 4096);
+  *cap = windows + windows / (*echunk / slots - 1) + *mgrid * (*echunk + slots);
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
     (void)hipGetLastError();
@@ -2653,13 +2681,32 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     const uint64_t keys_before = t->n_keys;
     TableView v = table_view(t);
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
-    // persistent: four blocks per CU (what its registers and LDS allow)
+    // persistent: as many blocks per CU as its registers and LDS allow
     const uint64_t grid = std::max<uint64_t>(8, std::min<uint64_t>((ceil_div(qe - q, kReadsPerBlock) + 7) / 8 * 8, mgrid));
-    if (t->k >= 16)
+    const int slots = res_slots();
+    if (slots == 8192 && t->k >= 16)
+      TK_LAUNCH(c, "count_merge", grid, 256, 8192 * 8, (k_count_agg<8192, true, true>), rview, q, qe, t->k,
+                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else if (slots == 8192)
+      TK_LAUNCH(c, "count_merge", grid, 256, 8192 * 8, (k_count_agg<8192, true, false>), rview, q, qe, t->k,
+                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else if (slots == 4096 && t->k >= 16)
       TK_LAUNCH(c, "count_merge", grid, 256, 4096 * 8, (k_count_agg<4096, true, true>), rview, q, qe, t->k,
                 t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else
+    else if (slots == 4096)
       TK_LAUNCH(c, "count_merge", grid, 256, 4096 * 8, (k_count_agg<4096, true, false>), rview, q, qe, t->k,
+                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else if (slots == 3072 && t->k >= 16)
+      TK_LAUNCH(c, "count_merge", grid, 256, 3072 * 8, (k_count_agg<3072, true, true>), rview, q, qe, t->k,
+                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else if (slots == 3072)
+      TK_LAUNCH(c, "count_merge", grid, 256, 3072 * 8, (k_count_agg<3072, true, false>), rview, q, qe, t->k,
+                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else if (t->k >= 16)
+      TK_LAUNCH(c, "count_merge", grid, 256, 3840 * 8, (k_count_agg<3840, true, true>), rview, q, qe, t->k,
+                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else
+      TK_LAUNCH(c, "count_merge", grid, 256, 3840 * 8, (k_count_agg<3840, true, false>), rview, q, qe, t->k,
                 t->canonical, v, c->d_flags, A, cap, echunk, cursor);
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
