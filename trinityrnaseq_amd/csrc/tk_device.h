@@ -331,6 +331,74 @@ __device__ __forceinline__ int lds_bucket_add_khi(uint64_t* tab, uint32_t b, uin
   return 0;
 }
 
+// ---- LDS tables with an 8-bit tag per slot ----
+// Tag of a key from its mix64 hash (low byte; the bucket comes from the high
+// bits): 1..255, 0 marks a slot with no tag yet.
+__device__ __forceinline__ uint32_t slot_tag(uint64_t h) {
+  const uint32_t t8 = (uint32_t)h & 0xFFu;
+  return t8 ? t8 : 1u;
+}
+
+// 0x80 in every byte of x that is zero, 0 elsewhere (exact: no carries
+// between bytes).
+__device__ __forceinline__ uint64_t zero_bytes(uint64_t x) {
+  const uint64_t m = 0x7F7F7F7F7F7F7F7Full;
+  return ~(((x & m) + m) | x | m);
+}
+
+// Insert-or-add into an LDS table of nb buckets of eight 8-B slots in the
+// global slot format, with tags[8 * b + j] the tag of slot j of bucket b:
+// `word` = tagged key | c << cshift.  A step reads the bucket's eight tags (8
+// bytes), reads only the slots whose tag matches (to confirm the key), and
+// otherwise claims the first slot with no tag by CAS, then writes its tag.
+// A slot claimed but not yet tagged by another thread shows as untagged: the
+// CAS then fails, and the slot is taken as the key's (add) or skipped.
+// Occupied slots stay a prefix of their bucket.  Returns 2 when a slot was
+// claimed, 1 when added (old = previous slot value), 0 when max_steps steps
+// found no room.
+__device__ __forceinline__ int lds_tag_add(uint64_t* tab, uint8_t* tags, uint32_t b, uint32_t nb, uint64_t word,
+                                           uint32_t tag, uint64_t kmo, uint32_t cshift, uint32_t max_steps,
+                                           uint64_t& old) {
+  const uint64_t tagged = word & kmo;
+  const uint64_t add = (word >> cshift) << cshift;
+  const uint64_t rep = 0x0101010101010101ull * tag;
+  uint64_t skip = 0;  // bytes of slots known to hold other keys
+  for (uint32_t p = 0; p < max_steps; p++) {
+    uint64_t* line = tab + ((uint64_t)b << 3);
+    const uint64_t tw = *reinterpret_cast<const uint64_t*>(tags + ((uint64_t)b << 3));
+    const uint64_t em = zero_bytes(tw) & ~skip;
+    uint64_t mm = zero_bytes(tw ^ rep) & ~skip;
+    const uint32_t je = em ? (uint32_t)__builtin_ctzll(em) >> 3 : 8u;
+    while (mm) {
+      const uint32_t j = (uint32_t)__builtin_ctzll(mm) >> 3;
+      if (j >= je) break;
+      const uint64_t v = line[j];
+      if ((v & kmo) == tagged) {
+        old = atomicAdd((unsigned long long*)&line[j], (unsigned long long)add);
+        return 1;
+      }
+      skip |= 0xFFull << (8 * j);
+      mm &= mm - 1;
+    }
+    if (je < 8) {
+      const uint64_t v = atomicCAS((unsigned long long*)&line[je], 0ull, (unsigned long long)word);
+      if (v == 0) {
+        tags[((uint64_t)b << 3) + je] = (uint8_t)tag;
+        return 2;
+      }
+      if ((v & kmo) == tagged) {
+        old = atomicAdd((unsigned long long*)&line[je], (unsigned long long)add);
+        return 1;
+      }
+      skip |= 0xFFull << (8 * je);
+      continue;  // rescan this bucket past the slot another key took
+    }
+    b = b + 1 == nb ? 0 : b + 1;
+    skip = 0;
+  }
+  return 0;
+}
+
 // ---- wave helpers ----
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -1152,26 +1152,38 @@ __global__ void k_res_bounds(const uint64_t* __restrict__ base2, const uint64_t*
 }
 
 // One workgroup per region: its buckets in LDS (zero when the table is
-// known empty), its words added there with the global slot protocol (claim
-// an empty slot with CAS, add to the slot holding the key; occupied slots
-// stay a prefix of their bucket), carries to the side table, the region
-// written back.  A word that finds no room in the region is marked in place
-// (occupied bit cleared) and flags[6] set; it is added after the table grows.
+// known empty) with an 8-bit tag per slot beside them, its words added there
+// with the global slot protocol (claim an empty slot with CAS, add to the
+// slot holding the key; occupied slots stay a prefix of their bucket) through
+// the tags, so that a probe reads 8 bytes of tags and only the slots whose
+// tag matches instead of the whole 64-B bucket; carries to the side table;
+// the region's slots written back.  A word that finds no room in the region
+// is marked in place (occupied bit cleared) and flags[6] set; it is added
+// after the table grows.
 template <bool FRESH>
 __global__ __launch_bounds__(1024) void k_res_count(uint64_t* __restrict__ words,
                                                     const uint64_t* __restrict__ rstart, TableView t,
                                                     uint32_t* __restrict__ flags) {
-  extern __shared__ uint64_t s_reg[];  // 8 << rbits slots
+  extern __shared__ uint64_t s_reg[];  // 8 << rbits slots, then 8 << rbits tag bytes
   __shared__ uint32_t s_new;
   constexpr int kPre = 4;  // words loaded ahead of their LDS inserts
   const uint64_t a = rstart[blockIdx.x], e = rstart[blockIdx.x + 1];
   if (a == e) return;
-  const uint32_t n16 = 4u << t.rbits;
+  const uint32_t nslots = 8u << t.rbits, n16 = nslots / 2;
+  uint8_t* s_tag = reinterpret_cast<uint8_t*>(s_reg + nslots);
   ulonglong2* g = reinterpret_cast<ulonglong2*>(t.slots + ((t.boff + ((uint64_t)blockIdx.x << t.rbits)) << 3));
   ulonglong2* l = reinterpret_cast<ulonglong2*>(s_reg);
   for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) {
-    if (FRESH) l[i] = make_ulonglong2(0ull, 0ull);
-    else l[i] = g[i];
+    if (FRESH) {
+      l[i] = make_ulonglong2(0ull, 0ull);
+      reinterpret_cast<uint16_t*>(s_tag)[i] = 0;
+    } else {
+      const ulonglong2 v = g[i];
+      l[i] = v;
+      const uint32_t t0 = v.x ? slot_tag(mix64(v.x & (t.occ - 1))) : 0u;
+      const uint32_t t1 = v.y ? slot_tag(mix64(v.y & (t.occ - 1))) : 0u;
+      reinterpret_cast<uint16_t*>(s_tag)[i] = (uint16_t)(t0 | (t1 << 8));
+    }
   }
   if (threadIdx.x == 0) s_new = 0;
   __syncthreads();
@@ -1191,10 +1203,11 @@ __global__ __launch_bounds__(1024) void k_res_count(uint64_t* __restrict__ words
       if (!w) continue;
       const uint64_t tagged = w & t.kmo;
       const uint64_t c = w >> t.cshift;
-      const uint32_t b = (uint32_t)(bucket0(t, w & (t.occ - 1)) - t.boff) & rmask;
+      const uint64_t h = mix64(w & (t.occ - 1));
+      const uint32_t b = (uint32_t)__umul64hi(h, t.nb) & rmask;
       uint64_t old = 0;
       // every bucket of the region, each rescanned at most 8 times
-      const int res = lds_bucket_add(s_reg, b, rmask + 1, w, t.kmo, t.cshift, 9 * (rmask + 1), old);
+      const int res = lds_tag_add(s_reg, s_tag, b, rmask + 1, w, slot_tag(h), t.kmo, t.cshift, 9 * (rmask + 1), old);
       const bool carry = res == 1 && (old >> t.cshift) + c > t.cmax;
       if (res == 2) mynew++;
       if (carry) ovf_add(t, tagged, t.cmax + 1, &flags[0]);
@@ -2534,9 +2547,9 @@ static bool resident_usable(tk_ctx* c, const tk_table* t) {
   static int ok = -1;  // 64 KB or more of dynamic LDS per workgroup needs the attribute
   if (ok < 0) {
     ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<true>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 << kResRegionBitsMax) == hipSuccess &&
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 72 << kResRegionBitsMax) == hipSuccess &&
          hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<false>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 << kResRegionBitsMax) == hipSuccess &&
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 72 << kResRegionBitsMax) == hipSuccess &&
          hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<4096>),
                              hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)res_scatter_lds(kResMaxBins, 4096)) == hipSuccess &&
@@ -2583,22 +2596,21 @@ static int res_sort_pass(tk_ctx* c, const std::vector<ResSlice>& sl, ResSlice* d
 }
 
 // LDS words of the merge table of the region-resident count: TK_RES_SLOTS =
-// 3840 (default, 30 KB: five blocks per CU), 3072 (24 KB: six blocks),
-// 4096 (32 KB: four blocks, with the block's other LDS) or 8192 (64 KB, two
-// blocks).
+// 3840 (default, 30 KB: five blocks per CU), 3328 (26 KB: more flushes) or
+// 8192 (64 KB: two blocks).
 static int res_slots() {
   static int slots = 0;
   if (!slots) {
     const char* e = getenv("TK_RES_SLOTS");
     const int v = e ? atoi(e) : 0;
-    slots = v == 8192 || v == 4096 || v == 3072 ? v : 3840;
+    slots = v == 8192 || v == 3328 ? v : 3840;
     if (slots == 8192 &&
         (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, true, true>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess ||
          hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, true, false>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess)) {
       (void)hipGetLastError();
-      slots = 4096;
+      slots = 3840;
     }
   }
   return slots;
@@ -2613,17 +2625,15 @@ static int res_slots() {
 static bool resident_plan(tk_ctx* c, uint64_t windows, uint64_t* cap, uint64_t* echunk, uint64_t* mgrid,
                           uint64_t* cap_max) {
   const uint64_t slots = (uint64_t)res_slots();
-  static int per_cu[4] = {0, 0, 0, 0};
-  int& pc = per_cu[slots == 8192 ? 2 : slots == 4096 ? 1 : slots == 3072 ? 3 : 0];
+  static int per_cu[3] = {0, 0, 0};
+  int& pc = per_cu[slots == 8192 ? 2 : slots == 3328 ? 1 : 0];
   if (!pc) {
     int nb = 0;
     const hipError_t e =
         slots == 8192
             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<8192, true, true>), 256, 8192 * 8)
-        : slots == 4096
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<4096, true, true>), 256, 4096 * 8)
-        : slots == 3072
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<3072, true, true>), 256, 3072 * 8)
+        : slots == 3328
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<3328, true, true>), 256, 3328 * 8)
             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<3840, true, true>), 256, 3840 * 8);
     if (e != hipSuccess || nb < 1) nb = 1;
     (void)hipGetLastError();
@@ -2684,30 +2694,16 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     // persistent: as many blocks per CU as its registers and LDS allow
     const uint64_t grid = std::max<uint64_t>(8, std::min<uint64_t>((ceil_div(qe - q, kReadsPerBlock) + 7) / 8 * 8, mgrid));
     const int slots = res_slots();
-    if (slots == 8192 && t->k >= 16)
-      TK_LAUNCH(c, "count_merge", grid, 256, 8192 * 8, (k_count_agg<8192, true, true>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else if (slots == 8192)
-      TK_LAUNCH(c, "count_merge", grid, 256, 8192 * 8, (k_count_agg<8192, true, false>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else if (slots == 4096 && t->k >= 16)
-      TK_LAUNCH(c, "count_merge", grid, 256, 4096 * 8, (k_count_agg<4096, true, true>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else if (slots == 4096)
-      TK_LAUNCH(c, "count_merge", grid, 256, 4096 * 8, (k_count_agg<4096, true, false>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else if (slots == 3072 && t->k >= 16)
-      TK_LAUNCH(c, "count_merge", grid, 256, 3072 * 8, (k_count_agg<3072, true, true>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else if (slots == 3072)
-      TK_LAUNCH(c, "count_merge", grid, 256, 3072 * 8, (k_count_agg<3072, true, false>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else if (t->k >= 16)
-      TK_LAUNCH(c, "count_merge", grid, 256, 3840 * 8, (k_count_agg<3840, true, true>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else
-      TK_LAUNCH(c, "count_merge", grid, 256, 3840 * 8, (k_count_agg<3840, true, false>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+#define TK_MERGE(S, KHI)                                                                               \
+  TK_LAUNCH(c, "count_merge", grid, 256, (S) * 8, (k_count_agg<S, true, KHI>), rview, q, qe, t->k, t->canonical, \
+            v, c->d_flags, A, cap, echunk, cursor)
+    if (slots == 8192 && t->k >= 16) TK_MERGE(8192, true);
+    else if (slots == 8192) TK_MERGE(8192, false);
+    else if (slots == 3328 && t->k >= 16) TK_MERGE(3328, true);
+    else if (slots == 3328) TK_MERGE(3328, false);
+    else if (t->k >= 16) TK_MERGE(3840, true);
+    else TK_MERGE(3840, false);
+#undef TK_MERGE
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table probe limit reached (flags=%u)", c->h_flags[0]);
@@ -2770,7 +2766,7 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     TK_LAUNCH(c, "count_bounds", ceil_div((uint64_t)R + 1, 256), 256, 0, k_res_bounds, hs, dhb, dm, fs, R, total,
               drs);
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
-    const size_t lds = (size_t)64 << t->rbits;
+    const size_t lds = (size_t)72 << t->rbits;  // slots + tags
     if (keys_before == 0 && nnew == 0)
       TK_LAUNCH(c, "count_regions", R, 1024, lds, k_res_count<true>, A, drs, v, c->d_flags);
     else
