@@ -291,6 +291,57 @@ __device__ __forceinline__ int lds_bucket_add(uint64_t* tab, uint32_t b, uint32_
   return 0;
 }
 
+// lds_bucket_add_khi reading each bucket in two halves: the second four
+// slots only when the first four hold neither the key nor an empty slot
+// (occupied slots form a prefix, so a half that ends in an empty slot ends
+// the bucket).  Same protocol and results.
+__device__ __forceinline__ int lds_bucket_add_khi_half(uint64_t* tab, uint32_t b, uint32_t nb, uint64_t word,
+                                                       uint32_t khi, uint32_t cshift, uint32_t max_steps,
+                                                       uint64_t& old) {
+  const uint32_t tlo = (uint32_t)word, thi = (uint32_t)(word >> 32) & khi;
+  const uint64_t add = (word >> cshift) << cshift;
+  for (uint32_t p = 0; p < max_steps; p++) {
+    uint64_t* line = tab + ((uint64_t)b << 3);
+    const uint4* l4 = reinterpret_cast<const uint4*>(line);
+    int jm = 8, je = 8;
+    {
+      const uint4 q0 = l4[0], q1 = l4[1];
+      const uint32_t lo[4] = {q0.x, q0.z, q1.x, q1.z};
+      const uint32_t hi[4] = {q0.y, q0.w, q1.y, q1.w};
+#pragma unroll
+      for (int j = 3; j >= 0; j--) {
+        jm = (lo[j] == tlo && (hi[j] & khi) == thi) ? j : jm;
+        je = hi[j] == 0 ? j : je;
+      }
+    }
+    if (jm == 8 && je == 8) {
+      const uint4 q2 = l4[2], q3 = l4[3];
+      const uint32_t lo[4] = {q2.x, q2.z, q3.x, q3.z};
+      const uint32_t hi[4] = {q2.y, q2.w, q3.y, q3.w};
+#pragma unroll
+      for (int j = 3; j >= 0; j--) {
+        jm = (lo[j] == tlo && (hi[j] & khi) == thi) ? j + 4 : jm;
+        je = hi[j] == 0 ? j + 4 : je;
+      }
+    }
+    if (jm < 8) {
+      old = atomicAdd((unsigned long long*)&line[jm], (unsigned long long)add);
+      return 1;
+    }
+    if (je < 8) {
+      const uint64_t v = atomicCAS((unsigned long long*)&line[je], 0ull, (unsigned long long)word);
+      if (v == 0) return 2;
+      if ((uint32_t)v == tlo && ((uint32_t)(v >> 32) & khi) == thi) {
+        old = atomicAdd((unsigned long long*)&line[je], (unsigned long long)add);
+        return 1;
+      }
+      continue;  // another key took the slot: rescan this bucket
+    }
+    b = b + 1 == nb ? 0 : b + 1;
+  }
+  return 0;
+}
+
 // lds_bucket_add for 2k >= 32: the key's low 32 bits are all of the slot's
 // low word and the occupied bit lies in the high word, so a slot matches
 // when its low word equals the key's and its masked high word the tagged

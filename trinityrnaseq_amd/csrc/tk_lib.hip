@@ -900,7 +900,9 @@ __global__ __launch_bounds__(256, EMIT ? (SLOTS <= 3072 ? 6 : 5) : 1) void k_cou
           const uint32_t hb = __umulhi((lo ^ (lo >> 15) ^ (hi << 7)) * 0x9E3779B1u, SLOTS / 8);
           uint64_t old;
           // the table never holds more than kMaxDistinct + 256 < SLOTS keys
-          isnew = (KHI ? lds_bucket_add_khi(s_tab, hb, SLOTS / 8, tagged | one, khi, t.cshift, 4 * SLOTS, old)
+          isnew = (KHI ? (EMIT
+                              ? lds_bucket_add_khi_half(s_tab, hb, SLOTS / 8, tagged | one, khi, t.cshift, 4 * SLOTS, old)
+                              : lds_bucket_add_khi(s_tab, hb, SLOTS / 8, tagged | one, khi, t.cshift, 4 * SLOTS, old))
                        : lds_bucket_add(s_tab, hb, SLOTS / 8, tagged | one, t.kmo, t.cshift, 4 * SLOTS, old)) == 2;
         }
       }
