@@ -150,11 +150,20 @@ def test_count_resident_crowded_regions(api, ctx, region_kb, monkeypatch):
     seqs = random_reads(17, 20000, 100, 150, n_rate=0.001) + repetitive_reads(25, 3000)
     r, t, seq, off = _count(api, ctx, seqs, 25, True, expected=64, max_load=0.9)
     ok, oc = O.count(seq, off, 25, True)
+    # the merged words of the resident passes: at least one per distinct key,
+    # at most one per counted window
+    assert len(ok) <= ctx.count_words() <= int(oc.sum())
     gk, gc = t.dump()
     assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
     t.count(r)
     gk, gc = t.dump()
     assert np.array_equal(gk, ok) and np.array_equal(gc, 2 * oc)
+    # table atomics stream nothing
+    monkeypatch.setenv("TK_RESIDENT", "0")
+    t.count(r)
+    assert ctx.count_words() == 0
+    gk, gc = t.dump()
+    assert np.array_equal(gk, ok) and np.array_equal(gc, 3 * oc)
 
 
 def test_hll_estimate(api, ctx, small_se):
