@@ -95,6 +95,7 @@ struct tk_table {
   uint64_t ovf_bound = 0;          // upper bound on keys that may need a side entry
   uint64_t n_keys = 0;             // distinct keys claimed (exact, from device counter)
   bool finalized = true;           // an empty table is trivially final
+  bool zero_pending = false;       // slots not yet cleared (cleared before first use)
   uint64_t bytes = 0;
   tk_table* compact = nullptr;     // keys with count >= compact_min, built for scoring
   uint32_t compact_min = 0;

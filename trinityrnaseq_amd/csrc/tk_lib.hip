@@ -1165,15 +1165,20 @@ __global__ void k_res_bounds(const uint64_t* __restrict__ base2, const uint64_t*
 template <bool FRESH>
 __global__ __launch_bounds__(1024) void k_res_count(uint64_t* __restrict__ words,
                                                     const uint64_t* __restrict__ rstart, TableView t,
-                                                    uint32_t* __restrict__ flags) {
+                                                    uint32_t* __restrict__ flags, int zero_empty) {
   extern __shared__ uint64_t s_reg[];  // 8 << rbits slots, then 8 << rbits tag bytes
   __shared__ uint32_t s_new;
   constexpr int kPre = 4;  // words loaded ahead of their LDS inserts
   const uint64_t a = rstart[blockIdx.x], e = rstart[blockIdx.x + 1];
-  if (a == e) return;
   const uint32_t nslots = 8u << t.rbits, n16 = nslots / 2;
-  uint8_t* s_tag = reinterpret_cast<uint8_t*>(s_reg + nslots);
   ulonglong2* g = reinterpret_cast<ulonglong2*>(t.slots + ((t.boff + ((uint64_t)blockIdx.x << t.rbits)) << 3));
+  if (a == e) {
+    // a region with no words of an uncleared table is written as empty
+    if (zero_empty)
+      for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) g[i] = make_ulonglong2(0ull, 0ull);
+    return;
+  }
+  uint8_t* s_tag = reinterpret_cast<uint8_t*>(s_reg + nslots);
   ulonglong2* l = reinterpret_cast<ulonglong2*>(s_reg);
   for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) {
     if (FRESH) {
@@ -1457,10 +1462,21 @@ __global__ void k_ovf_fold(TableView t, uint64_t n_pairs, int finalize,
   }
 }
 
-static int table_alloc_slots(tk_ctx* ctx, uint64_t nbuckets, uint64_t** p) {
+static int table_alloc_slots(tk_ctx* ctx, uint64_t nbuckets, uint64_t** p, bool zero = true) {
   int rc = dev_alloc((void**)p, nbuckets * 64);
   if (rc) return rc;
-  TK_HIP(hipMemsetAsync(*p, 0, nbuckets * 64, ctx->stream));
+  if (zero) TK_HIP(hipMemsetAsync(*p, 0, nbuckets * 64, ctx->stream));
+  return TK_OK;
+}
+
+// Clear the slots of a table created without clearing them.  Every entry
+// point that reads or writes slots calls this (through table_set_final)
+// first, except the region-resident count of an empty table, which writes
+// every region itself.
+static int table_zero(tk_ctx* ctx, tk_table* t) {
+  if (!t->zero_pending) return TK_OK;
+  TK_HIP(hipMemsetAsync(t->d_slots, 0, t->nb * 64, ctx->stream));
+  t->zero_pending = false;
   return TK_OK;
 }
 
@@ -1549,6 +1565,7 @@ static void drop_compact(tk_table* t) {
 }
 
 static int table_set_final(tk_ctx* ctx, tk_table* t, bool final_) {
+  if (int rc = table_zero(ctx, t)) return rc;
   if (!final_) drop_compact(t);  // the table is about to change
   if (t->finalized == final_) return TK_OK;
   TK_HIP(hipMemsetAsync(ctx->d_flags, 0, 4, ctx->stream));
@@ -2275,7 +2292,10 @@ int tk_table_create(tk_ctx* c, int k, int canonical, uint64_t expected, double m
   t->canonical = canonical ? 1 : 0;
   t->max_load = max_load;
   table_layout(std::max<uint64_t>(expected, 64), max_load, &t->nb, &t->rbits);
-  int rc = table_alloc_slots(c, t->nb, &t->d_slots);
+  // the slots are cleared on first use (the region-resident count of an
+  // empty table writes every region instead)
+  int rc = table_alloc_slots(c, t->nb, &t->d_slots, false);
+  t->zero_pending = true;
   if (!rc) rc = table_ensure_ovf(c, t, 0);
   if (!rc) { hipError_t e = hipStreamSynchronize(c->stream); if (e != hipSuccess) rc = hip_fail(e, "table init"); }
   if (rc) { tk_table_free(t); return rc; }
@@ -2673,8 +2693,10 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     else if ((rc = hll_estimate(c, r, rb, re, t->k, t->canonical, windows, &est)))
       return rc;
     const double want = (double)t->n_keys + 1.05 * est + 4096;
-    if (want > t->max_load * (double)(t->nb * 8))
+    if (want > t->max_load * (double)(t->nb * 8)) {
+      if ((rc = table_zero(c, t))) return rc;  // the rehash reads the old slots
       if ((rc = table_grow(c, t, (uint64_t)want))) return rc;
+    }
     if (!resident_usable(c, t)) return fail(TK_ERR_ARG, "count: grown table has %llu-bucket regions",
                                             (unsigned long long)(1ull << t->rbits));
   }
@@ -2712,9 +2734,14 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     uint64_t nnew = 0, n = 0;
     memcpy(&nnew, &c->h_flags[2], 8);
     memcpy(&n, &c->h_flags[8], 8);
+    if (t->zero_pending && nnew)  // the plan's stream holds every word the merge can emit
+      return fail(TK_ERR_STATE, "count: merged words missed the stream of an uncleared table");
     t->n_keys += nnew;
     n = std::min(n, cap);
-    if (!n) continue;
+    if (!n) {
+      if ((rc = table_zero(c, t))) return rc;
+      continue;
+    }
     // coarse bins of F = 2^fs regions, then single regions
     const uint32_t R = (uint32_t)(t->nb >> t->rbits);
     uint32_t fs = 0;
@@ -2770,9 +2797,11 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
     const size_t lds = (size_t)72 << t->rbits;  // slots + tags
     if (keys_before == 0 && nnew == 0)
-      TK_LAUNCH(c, "count_regions", R, 1024, lds, k_res_count<true>, A, drs, v, c->d_flags);
+      TK_LAUNCH(c, "count_regions", R, 1024, lds, k_res_count<true>, A, drs, v, c->d_flags,
+                t->zero_pending ? 1 : 0);
     else
-      TK_LAUNCH(c, "count_regions", R, 1024, lds, k_res_count<false>, A, drs, v, c->d_flags);
+      TK_LAUNCH(c, "count_regions", R, 1024, lds, k_res_count<false>, A, drs, v, c->d_flags, 0);
+    t->zero_pending = false;  // every region written (fresh) or the table was clear
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table error (flags=%u)", c->h_flags[0]);
@@ -2808,7 +2837,12 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
   if (set_device(c->device)) return TK_ERR_HIP;
   c->count_words = 0;
   if (!r->n_reads) return TK_OK;
+  // an uncleared (new, empty) table stays so here: the region-resident count
+  // writes every region, any other path clears it first
+  const bool unzeroed = t->zero_pending;
+  t->zero_pending = false;
   int rc = table_set_final(c, t, false);
+  t->zero_pending = unzeroed;
   if (rc) return rc;
   uint64_t total = 0;
   if ((rc = reads_windows(c, r, t->k, &total))) return rc;
@@ -2829,6 +2863,7 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
     rc = count_resident(c, t, r, rview, 0, r->n_reads, total, max_nwin);
     if (rc != kNotResident) return rc;
   }
+  if ((rc = table_zero(c, t))) return rc;
   uint64_t rb = 0;
   while (rb < r->n_reads) {
     const uint32_t nparts = (uint32_t)(t->nb >> t->rbits);
@@ -3111,6 +3146,7 @@ static int scoring_table(tk_ctx* c, tk_table* t, uint32_t min_count, tk_table** 
   if ((double)n_ge > 0.5 * (double)t->n_keys) return TK_OK;
   tk_table* ct = nullptr;
   if ((rc = tk_table_create(c, t->k, t->canonical, n_ge + 64, 0.5, &ct))) return rc;
+  if ((rc = table_zero(c, ct))) { tk_table_free(ct); return rc; }
   // every carry of the source may land in the copy's side table
   if ((rc = table_ensure_ovf(c, ct, std::max<uint64_t>(t->ovf_bound, 1)))) { tk_table_free(ct); return rc; }
   ct->finalized = false;
