@@ -84,6 +84,18 @@ int tk_device_bytes_in_use(uint64_t* bytes);
    the count (no counterpart in the reference). */
 int tk_count_words(tk_ctx* ctx, uint64_t* words);
 
+/* Keep the device blocks that the calling thread frees (tables, read sets,
+   buffers) on a free list of at most limit_bytes and hand them to its next
+   allocations of a similar size, instead of hipFree + hipMalloc per job:
+   hipFree waits for the whole device, and a large hipMalloc after large
+   frees can stall for seconds.  A reused block is not cleared (tables track
+   their own clearing).  limit_bytes = 0 releases the list and turns reuse
+   off.  The thread must issue all its work through one context: stream
+   order is what makes a reused block safe without a wait.  Server workers
+   do this by themselves.  No counterpart in the reference (its tools run
+   one process per job). */
+int tk_thread_cache(uint64_t limit_bytes);
+
 /* Plain device buffers (inputs kept resident in HBM, outputs of _d calls). */
 int tk_buffer_alloc(tk_ctx* ctx, uint64_t bytes, void** d_ptr);
 int tk_buffer_free(tk_ctx* ctx, void* d_ptr);

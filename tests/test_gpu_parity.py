@@ -135,13 +135,16 @@ def test_table_growth(api, ctx, layout):
     assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
 
 
+@pytest.mark.parametrize("tile", ["2048", "4096", "8192"])
 @pytest.mark.parametrize("region_kb", ["1", "64"])
-def test_count_resident_crowded_regions(api, ctx, region_kb, monkeypatch):
+def test_count_resident_crowded_regions(api, ctx, region_kb, tile, monkeypatch):
     """Region-resident count into a table grown at load 0.9: with 128-slot
     regions some regions receive more keys than they hold, and their words
     are added after the table grows; with 64 KB regions the table is one
-    region or a few.  A second count adds onto the non-empty table."""
-    say("count_resident_crowded", region_kb)
+    region or a few.  A second count adds onto the non-empty table.  Every
+    scatter tile size of the sort passes."""
+    say("count_resident_crowded", region_kb, "tile", tile)
+    monkeypatch.setenv("TK_RES_TILE", tile)
     monkeypatch.setenv("TK_PARTITION", "0")
     monkeypatch.setenv("TK_ORDER", "1")
     monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
@@ -164,6 +167,32 @@ def test_count_resident_crowded_regions(api, ctx, region_kb, monkeypatch):
     assert ctx.count_words() == 0
     gk, gc = t.dump()
     assert np.array_equal(gk, ok) and np.array_equal(gc, 3 * oc)
+
+
+@pytest.mark.parametrize("resident", ["1", "0"])
+def test_count_reused_blocks(api, ctx, resident, monkeypatch):
+    """With the thread's block reuse on, each table takes the block of the
+    table freed before it, slots and all: the region-resident passes write
+    every region of such an uncleared table, table atomics clear it first.
+    Three read sets in a row, each equal to the oracle's count."""
+    say("count_reused_blocks resident", resident)
+    monkeypatch.setenv("TK_PARTITION", "0")
+    monkeypatch.setenv("TK_ORDER", "1")
+    monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
+    monkeypatch.setenv("TK_RESIDENT", resident)
+    monkeypatch.setenv("TK_REGION_KB", "4")
+    sets = [random_reads(31, 8000, 100, 150, n_rate=0.001), random_reads(32, 5000, 60, 150, n_rate=0.01)]
+    api.thread_cache(1 << 30)
+    try:
+        for seqs in (sets[0], sets[1], sets[0]):
+            r, t, seq, off = _count(api, ctx, seqs, 25, True, expected=1 << 20)
+            ok, oc = O.count(seq, off, 25, True)
+            gk, gc = t.dump()
+            assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+            t.free()
+            r.free()
+    finally:
+        api.thread_cache(0)
 
 
 def test_hll_estimate(api, ctx, small_se):

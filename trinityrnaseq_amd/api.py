@@ -37,6 +37,7 @@ SIGNATURES = {
     "tk_timing_reset": (C.c_int, [_P]),
     "tk_timing_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), _U64P]),
     "tk_device_bytes_in_use": (C.c_int, [_U64P]),
+    "tk_thread_cache": (C.c_int, [C.c_uint64]),
     "tk_count_words": (C.c_int, [_P, _U64P]),
     "tk_buffer_alloc": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
     "tk_buffer_free": (C.c_int, [_P, _P]),
@@ -474,6 +475,13 @@ def device_bytes_in_use() -> int:
     b = C.c_uint64()
     _check(lib().tk_device_bytes_in_use(C.byref(b)), "tk_device_bytes_in_use")
     return b.value
+
+
+def thread_cache(limit_bytes: int) -> None:
+    """Reuse the device blocks this thread frees, up to limit_bytes held
+    (0 turns reuse off and releases them); the thread must work through one
+    context."""
+    _check(lib().tk_thread_cache(int(limit_bytes)), "tk_thread_cache")
 
 
 class Batch:

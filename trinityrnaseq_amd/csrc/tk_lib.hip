@@ -1044,14 +1044,19 @@ __global__ __launch_bounds__(256) void k_res_hist(const uint64_t* __restrict__ i
 // Each tile of kResTile words is counting-sorted by bin in LDS first, so
 // the stores leave in runs of consecutive addresses (one run per bin and
 // tile) instead of one scattered 8-B store per word.
-static size_t res_scatter_lds(uint32_t nbins, uint32_t tile) { return (size_t)tile * 10 + (size_t)nbins * 16; }
+// A tile of 8,192 words keeps no per-word bin in LDS (the write loop hashes
+// each word again), so that two blocks still fit one CU.
+static size_t res_scatter_lds(uint32_t nbins, uint32_t tile) {
+  return (size_t)tile * (tile > 4096 ? 8 : 10) + (size_t)nbins * 16;
+}
 
 // Exclusive scan of s_cnt[0, nbins) into s_loc (nbins <= 2048, blockDim
-// 256); returns the total.
+// NT); returns the total.
+template <uint32_t NT>
 __device__ __forceinline__ uint32_t res_bin_scan(const uint32_t* s_cnt, uint32_t* s_loc, uint32_t nbins,
                                                  uint32_t* s_w) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const uint32_t E = (nbins + 255) / 256, b0 = tid * E;
+  const uint32_t E = (nbins + NT - 1) / NT, b0 = tid * E;
   uint32_t sum = 0;
   for (uint32_t e = 0; e < E; e++)
     if (b0 + e < nbins) sum += s_cnt[b0 + e];
@@ -1070,35 +1075,41 @@ __device__ __forceinline__ uint32_t res_bin_scan(const uint32_t* s_cnt, uint32_t
       s_loc[b0 + e] = pre;
       pre += s_cnt[b0 + e];
     }
-  return s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  uint32_t tot = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < NT / 64; i++) tot += s_w[i];
+  return tot;
 }
 
-template <uint32_t kResTile>
-__global__ __launch_bounds__(256) void k_res_scatter(const uint64_t* __restrict__ in,
-                                                     const ResSlice* __restrict__ sl, TableView t,
-                                                     uint32_t shift, uint32_t mask, uint32_t nbins,
-                                                     const uint64_t* __restrict__ base,
-                                                     uint64_t* __restrict__ out) {
+// kThreads threads per block; tiles above 4,096 words recompute each word's
+// bin in the write loop instead of keeping it in LDS.
+template <uint32_t kResTile, uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_res_scatter(const uint64_t* __restrict__ in,
+                                                          const ResSlice* __restrict__ sl, TableView t,
+                                                          uint32_t shift, uint32_t mask, uint32_t nbins,
+                                                          const uint64_t* __restrict__ base,
+                                                          uint64_t* __restrict__ out) {
+  constexpr bool kKeepBin = kResTile <= 4096;
   extern __shared__ uint64_t s_res[];
   uint64_t* s_tile = s_res;                                              // kResTile words
   uint64_t* s_pos = s_res + kResTile;                                    // nbins: next output index
   uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_pos + nbins);          // nbins: tile counts
   uint32_t* s_loc = s_cnt + nbins;                                       // nbins: tile offsets
-  uint16_t* s_tbin = reinterpret_cast<uint16_t*>(s_loc + nbins);         // kResTile bins
-  __shared__ uint32_t s_w[4];
-  constexpr int kPer = kResTile / 256;
+  uint16_t* s_tbin = reinterpret_cast<uint16_t*>(s_loc + nbins);         // kResTile bins (kKeepBin)
+  __shared__ uint32_t s_w[kThreads / 64];
+  constexpr int kPer = kResTile / kThreads;
   const int tid = threadIdx.x;
   const ResSlice d = sl[blockIdx.x];
-  for (uint32_t i = tid; i < nbins; i += blockDim.x) s_pos[i] = base[d.hbase + (uint64_t)i * d.m + d.j];
+  for (uint32_t i = tid; i < nbins; i += kThreads) s_pos[i] = base[d.hbase + (uint64_t)i * d.m + d.j];
   for (uint64_t t0 = d.begin; t0 < d.end; t0 += kResTile) {
     const uint32_t nt = (uint32_t)std::min<uint64_t>(kResTile, d.end - t0);
-    for (uint32_t i = tid; i < nbins; i += blockDim.x) s_cnt[i] = 0;
+    for (uint32_t i = tid; i < nbins; i += kThreads) s_cnt[i] = 0;
     __syncthreads();
     uint64_t w[kPer];
     uint32_t bin[kPer], rank[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; u++) {
-      const uint32_t i = u * 256 + tid;
+      const uint32_t i = u * kThreads + tid;
       w[u] = i < nt ? in[t0 + i] : 0;
     }
 #pragma unroll
@@ -1111,22 +1122,23 @@ __global__ __launch_bounds__(256) void k_res_scatter(const uint64_t* __restrict_
       }
     }
     __syncthreads();
-    const uint32_t nv = res_bin_scan(s_cnt, s_loc, nbins, s_w);
+    const uint32_t nv = res_bin_scan<kThreads>(s_cnt, s_loc, nbins, s_w);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kPer; u++)
       if (w[u]) {
         const uint32_t p = s_loc[bin[u]] + rank[u];
         s_tile[p] = w[u];
-        s_tbin[p] = (uint16_t)bin[u];
+        if (kKeepBin) s_tbin[p] = (uint16_t)bin[u];
       }
     __syncthreads();
-    for (uint32_t i = tid; i < nv; i += blockDim.x) {
-      const uint32_t b = s_tbin[i];
-      out[s_pos[b] + (i - s_loc[b])] = s_tile[i];
+    for (uint32_t i = tid; i < nv; i += kThreads) {
+      const uint64_t x = s_tile[i];
+      const uint32_t b = kKeepBin ? (uint32_t)s_tbin[i] : (res_region(t, x) >> shift) & mask;
+      out[s_pos[b] + (i - s_loc[b])] = x;
     }
     __syncthreads();
-    for (uint32_t i = tid; i < nbins; i += blockDim.x) s_pos[i] += s_cnt[i];
+    for (uint32_t i = tid; i < nbins; i += kThreads) s_pos[i] += s_cnt[i];
   }
 }
 
@@ -2083,6 +2095,14 @@ int tk_device_bytes_in_use(uint64_t* bytes) {
   return TK_OK;
 }
 
+int tk_thread_cache(uint64_t limit_bytes) {
+  if (limit_bytes)
+    cache_begin((size_t)limit_bytes);
+  else
+    cache_end();
+  return TK_OK;
+}
+
 // ---------------- plain buffers ----------------
 static std::mutex g_buf_mu;
 static std::map<void*, uint64_t> g_bufs;
@@ -2572,10 +2592,13 @@ static bool resident_usable(tk_ctx* c, const tk_table* t) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, 72 << kResRegionBitsMax) == hipSuccess &&
          hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<false>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, 72 << kResRegionBitsMax) == hipSuccess &&
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<4096>),
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<8192, 512>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)res_scatter_lds(kResMaxBins, 8192)) == hipSuccess &&
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<4096, 256>),
                              hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)res_scatter_lds(kResMaxBins, 4096)) == hipSuccess &&
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<2048>),
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<2048, 256>),
                              hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)res_scatter_lds(kResMaxBins, 2048)) == hipSuccess;
     (void)hipGetLastError();
@@ -2584,11 +2607,14 @@ static bool resident_usable(tk_ctx* c, const tk_table* t) {
   return ok == 1;
 }
 
-// Words per LDS tile of the region sort's scatter: TK_RES_TILE=2048 or 4096
-// (default).
+// Words per LDS tile of the region sort's scatter: TK_RES_TILE=8192
+// (default; 512 threads per block, two blocks per CU), 4096 or 2048 (256
+// threads).  Larger tiles leave longer runs per bin: 8,192 words over 512
+// bins are 128 B per run on average.
 static uint32_t res_tile() {
   const char* e = getenv("TK_RES_TILE");
-  return e && atoi(e) == 2048 ? 2048 : 4096;
+  const int v = e ? atoi(e) : 0;
+  return v == 2048 || v == 4096 ? (uint32_t)v : 8192;
 }
 
 // Scratch slots of the region-resident count (kept by the context, so the
@@ -2606,12 +2632,16 @@ static int res_sort_pass(tk_ctx* c, const std::vector<ResSlice>& sl, ResSlice* d
   TK_LAUNCH(c, "count_hist", sl.size(), 256, 0, k_res_hist, in, dsl, v, shift, mask, nbins, h);
   int rc = exclusive_scan_u64(c, h, hs, nhist + 1);
   if (rc) return rc;
-  if (res_tile() == 2048)
-    TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins, 2048), k_res_scatter<2048>, in, dsl, v,
-              shift, mask, nbins, hs, out);
+  const uint32_t tile = res_tile();
+  if (tile == 2048)
+    TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins, 2048), (k_res_scatter<2048, 256>), in,
+              dsl, v, shift, mask, nbins, hs, out);
+  else if (tile == 8192)
+    TK_LAUNCH(c, "count_scatter", sl.size(), 512, res_scatter_lds(nbins, 8192), (k_res_scatter<8192, 512>), in,
+              dsl, v, shift, mask, nbins, hs, out);
   else
-    TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins, 4096), k_res_scatter<4096>, in, dsl, v,
-              shift, mask, nbins, hs, out);
+    TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins, 4096), (k_res_scatter<4096, 256>), in,
+              dsl, v, shift, mask, nbins, hs, out);
   // the slice table is host memory read by the copy above
   TK_HIP(hipStreamSynchronize(c->stream));
   return TK_OK;
