@@ -493,9 +493,13 @@ def _components(seed, n_comp, max_reads=400):
     return comps
 
 
+@pytest.mark.parametrize("packed", ["1", "0"])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_components_match_per_component_oracle(api, ctx, use_graph):
-    say("batch_components graph", use_graph)
+def test_batch_components_match_per_component_oracle(api, ctx, use_graph, packed, monkeypatch):
+    """Packed: one-word component-table slots (the default at k = 25); else
+    the two-word slots of the plain component table."""
+    say("batch_components graph", use_graph, "packed", packed)
+    monkeypatch.setenv("TK_BATCH_CT_PACKED", packed)
     comps = _components(11, 40)
     all_reads = [r for rs, _ in comps for r in rs]
     all_contigs = [c for _, cs in comps for c in cs]

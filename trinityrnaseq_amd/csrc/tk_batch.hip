@@ -160,7 +160,9 @@ __global__ __launch_bounds__(256) void k_batch_stats(ReadsView rv, const uint32_
   }
 }
 
-// Contig k-mers of each component -> index of the contig within its component.
+// Contig k-mers of each component -> index of the contig within its component
+// (kPacked: pctable_add's one-word slots, else ctable_add's two-word slots).
+template <bool kPacked>
 __global__ void k_batch_ctable(ReadsView cv, const uint32_t* __restrict__ contig_comp,
                                const uint32_t* __restrict__ contig_local,
                                const CompLayout* __restrict__ lay, int k, int canonical,
@@ -180,11 +182,16 @@ __global__ void k_batch_ctable(ReadsView cv, const uint32_t* __restrict__ contig
       uint64_t fwd;
       bool valid;
       window_at(cv, wb, p, k, fwd, valid);
-      if (valid) ctable_add(ct, L.ct_boff, L.ct_mask, occ, canon_key(fwd, k, canonical), local, dummy, &flags[0]);
+      if (!valid) continue;
+      if (kPacked)
+        pctable_add(ct, L.ct_boff, L.ct_mask, k, canon_key(fwd, k, canonical), local, &flags[0]);
+      else
+        ctable_add(ct, L.ct_boff, L.ct_mask, occ, canon_key(fwd, k, canonical), local, dummy, &flags[0]);
     }
   }
 }
 
+template <bool kPacked>
 __global__ __launch_bounds__(256) void k_batch_vote(ReadsView rv, const uint32_t* __restrict__ read_comp,
                                                     const CompLayout* __restrict__ lay, int k,
                                                     int canonical, const uint64_t* __restrict__ ct,
@@ -197,7 +204,7 @@ __global__ __launch_bounds__(256) void k_batch_vote(ReadsView rv, const uint32_t
   uint32_t* tmp = s_dyn + wpb * slab + wv * 64;
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + wv; r < rv.n_reads; r += (uint64_t)gridDim.x * wpb) {
     const CompLayout L = lay[read_comp[r]];
-    vote_read(rv, r, k, canonical, ct, L.ct_boff, L.ct_mask, hits, tmp, lane, out + r);
+    vote_read<kPacked>(rv, r, k, canonical, ct, L.ct_boff, L.ct_mask, hits, tmp, lane, out + r);
   }
 }
 
@@ -221,6 +228,7 @@ struct tk_batch {
   double window_load = 0.7;
   int stats_waves = 4;
   bool use_graph = true;
+  bool ct_packed = false;  // one-word component-table slots (k <= 25, < 2^(63-2k) contigs per component)
   uint32_t max_nwin = 0;
   // device
   CompLayout* d_lay = nullptr;
@@ -273,7 +281,9 @@ static int batch_layout(tk_ctx* c, tk_batch* b) {
     L.ovf_off = ooff;
     L.omask = oc - 1;
     ooff += oc;
-    const uint64_t cb = p2(std::max<uint64_t>((2 * b->comp_cwindows[i] + 3) / 4, 4));
+    // buckets of eight packed or four two-word slots, at most half full
+    const uint64_t cb = p2(std::max<uint64_t>(b->ct_packed ? (2 * b->comp_cwindows[i] + 7) / 8
+                                                            : (2 * b->comp_cwindows[i] + 3) / 4, 4));
     L.ct_boff = coff;
     L.ct_mask = cb - 1;
     coff += cb;
@@ -351,11 +361,20 @@ static int batch_enqueue(tk_ctx* c, tk_batch* b, const tk_stats_opts* o, tk_read
   if (d_votes && rv.n_reads) {
     hipLaunchKernelGGL(k_fill_zero, dim3(grid(b->n_ct_words, 256)), dim3(256), 0, c->stream, b->d_ct, b->n_ct_words);
     const ReadsView cv = rview(b->contigs);
-    if (cv.n_reads)
-      hipLaunchKernelGGL(k_batch_ctable, dim3(grid(cv.n_reads, 4)), dim3(256), 0, c->stream, cv, b->d_contig_comp,
-                         b->d_contig_local, b->d_lay, b->k, b->canonical, b->d_ct, c->d_flags);
-    hipLaunchKernelGGL(k_batch_vote, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb), (size_t)slab * 4 * wpb + (size_t)wpb * 256,
-                       c->stream, rv, b->d_read_comp, b->d_lay, b->k, b->canonical, b->d_ct, d_votes, slab);
+    const size_t vlds = (size_t)slab * 4 * wpb + (size_t)wpb * 256;
+    if (b->ct_packed) {
+      if (cv.n_reads)
+        hipLaunchKernelGGL(k_batch_ctable<true>, dim3(grid(cv.n_reads, 4)), dim3(256), 0, c->stream, cv,
+                           b->d_contig_comp, b->d_contig_local, b->d_lay, b->k, b->canonical, b->d_ct, c->d_flags);
+      hipLaunchKernelGGL(k_batch_vote<true>, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb), vlds, c->stream, rv,
+                         b->d_read_comp, b->d_lay, b->k, b->canonical, b->d_ct, d_votes, slab);
+    } else {
+      if (cv.n_reads)
+        hipLaunchKernelGGL(k_batch_ctable<false>, dim3(grid(cv.n_reads, 4)), dim3(256), 0, c->stream, cv,
+                           b->d_contig_comp, b->d_contig_local, b->d_lay, b->k, b->canonical, b->d_ct, c->d_flags);
+      hipLaunchKernelGGL(k_batch_vote<false>, dim3(grid(rv.n_reads, wpb)), dim3(64 * wpb), vlds, c->stream, rv,
+                         b->d_read_comp, b->d_lay, b->k, b->canonical, b->d_ct, d_votes, slab);
+    }
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "batch launch");
@@ -473,6 +492,15 @@ int tk_batch_create(tk_ctx* c, int k, int canonical, const tk_reads* reads, cons
     if ((e = hipMemcpy(b->d_contig_comp, ccomp.data(), contigs->n_reads * 4, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_contig_local, clocal.data(), contigs->n_reads * 4, hipMemcpyHostToDevice)) != hipSuccess)
       rc = hip_fail(e, "upload contig components");
+  }
+  // one-word component-table slots when every contig index fits beside the
+  // key (TK_BATCH_CT_PACKED=0: always the two-word slots)
+  {
+    const char* ev = getenv("TK_BATCH_CT_PACKED");
+    uint64_t max_contigs = 0;
+    for (uint64_t i = 0; i < n_comp; i++)
+      max_contigs = std::max<uint64_t>(max_contigs, contig_comp_off[i + 1] - contig_comp_off[i]);
+    b->ct_packed = !(ev && ev[0] == '0') && k <= 25 && max_contigs <= (1ull << (63 - 2 * k));
   }
   if (!rc) rc = batch_layout(c, b);
   if (rc) { tk_batch_free(b); return rc; }

@@ -342,6 +342,90 @@ __device__ __forceinline__ uint32_t ctable_get(const uint64_t* slots, uint64_t b
   return 0xFFFFFFFFu;
 }
 
+// Packed component table (k <= 25): 64-byte buckets of eight 8-byte slots
+// [vmax - comp : 63-2k | occupied : 1 | key : 2k], vmax = 2^(63-2k) - 1, so
+// a new key costs one CAS and the largest word of one key holds its
+// smallest comp; comp <= vmax.
+__device__ inline void pctable_add(uint64_t* slots, uint64_t boff, uint64_t bmask, int k, uint64_t key,
+                                   uint32_t comp, uint32_t* err) {
+  const uint64_t occ = 1ull << (2 * k), km = (occ << 1) - 1, tagged = key | occ;
+  const uint64_t vmax = (1ull << (63 - 2 * k)) - 1;
+  const uint64_t word = ((vmax - comp) << (2 * k + 1)) | tagged;
+  uint64_t b = mix64(key) & bmask;
+  for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
+    uint64_t* line = slots + ((boff + b) << 3);
+    for (int j = 0; j < 8; j++) {
+      uint64_t s = line[j];
+      if (s == 0) {
+        s = atomicCAS((unsigned long long*)&line[j], 0ULL, (unsigned long long)word);
+        if (s == 0) return;
+      }
+      if ((s & km) == tagged) {
+        if (s < word) atomicMax((unsigned long long*)&line[j], (unsigned long long)word);
+        return;
+      }
+    }
+    b = (b + 1) & bmask;
+  }
+  atomicOr(err, 1u);
+}
+
+__device__ __forceinline__ uint32_t pctable_comp(uint64_t s, int k) {
+  return (uint32_t)(((1ull << (63 - 2 * k)) - 1) - (s >> (2 * k + 1)));
+}
+
+__device__ inline uint32_t pctable_get(const uint64_t* slots, uint64_t boff, uint64_t bmask, int k,
+                                       uint64_t key) {
+  const uint64_t occ = 1ull << (2 * k), km = (occ << 1) - 1, tagged = key | occ;
+  uint64_t b = mix64(key) & bmask;
+  for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
+    const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(slots + ((boff + b) << 3));
+    ulonglong2 v[4] = {l2[0], l2[1], l2[2], l2[3]};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if ((v[j].x & km) == tagged) return pctable_comp(v[j].x, k);
+      if (v[j].x == 0) return 0xFFFFFFFFu;
+      if ((v[j].y & km) == tagged) return pctable_comp(v[j].y, k);
+      if (v[j].y == 0) return 0xFFFFFFFFu;
+    }
+    b = (b + 1) & bmask;
+  }
+  return 0xFFFFFFFFu;
+}
+
+// coop_cget64 for the packed table: the four lanes of a group read two
+// 8-byte slots each of the key's home bucket.
+__device__ __forceinline__ uint32_t coop_pget64(const uint64_t* slots, uint64_t boff, uint64_t bmask, int k,
+                                                uint64_t key, bool want, uint32_t* tmp, int lane) {
+  const uint64_t occ = 1ull << (2 * k), km = (occ << 1) - 1, tagged = key | occ;
+  const uint64_t b = boff + (mix64(key) & bmask);
+  const int g = lane >> 2, sl = lane & 3;
+  tmp[lane] = 0xFFFFFFFFu;
+  ulonglong2 sv[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint64_t bq = __shfl(b, q * 16 + g);
+    sv[q] = reinterpret_cast<const ulonglong2*>(slots + (bq << 3))[sl];
+  }
+  wave_sync();
+  uint64_t pending = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint64_t tq = __shfl(tagged, q * 16 + g);
+    const bool mx = (sv[q].x & km) == tq, my = (sv[q].y & km) == tq;
+    if (mx) tmp[q * 16 + g] = pctable_comp(sv[q].x, k);
+    else if (my) tmp[q * 16 + g] = pctable_comp(sv[q].y, k);
+    const uint32_t gm = nibble_bits(__ballot(mx || my));
+    const uint32_t ge = nibble_bits(__ballot(sv[q].x == 0 || sv[q].y == 0));
+    pending |= (uint64_t)(~gm & ~ge & 0xFFFFu) << (16 * q);
+  }
+  wave_sync();
+  uint32_t c = tmp[lane];
+  if (want && ((pending >> lane) & 1ull)) c = pctable_get(slots, boff, bmask, k, key);
+  wave_sync();
+  return want ? c : 0xFFFFFFFFu;
+}
+
 // Cooperative component-table lookup of the 64 keys held one per lane: in
 // round q the four lanes 4g..4g+3 read the four 16-byte slots of the bucket
 // of lane 16q+g's key (one 64-byte request per key, four loads in flight per
@@ -381,6 +465,8 @@ __device__ __forceinline__ uint32_t coop_cget64(const uint64_t* slots, uint64_t 
 // ReadsToTranscripts vote of read r, one wavefront: component ids of the
 // hitting windows staged in `hits` (LDS), distinct ids walked in ascending
 // order so the first strictly largest count wins (ties -> smallest id).
+// kPacked: the component table is the packed layout (pctable_add).
+template <bool kPacked = false>
 __device__ inline void vote_read(const ReadsView& rv, uint64_t r, int k, int canonical,
                                  const uint64_t* slots, uint64_t boff, uint64_t bmask,
                                  uint32_t* hits, uint32_t* tmp, int lane, tk_r2t_vote* out) {
@@ -395,7 +481,8 @@ __device__ inline void vote_read(const ReadsView& rv, uint64_t r, int k, int can
       bool valid = false;
       if (p < nwin) window_at(rv, wb, p, k, fwd, valid);
       const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
-      const uint32_t c = coop_cget64(slots, boff, bmask, occ, key, valid, tmp, lane);
+      const uint32_t c = kPacked ? coop_pget64(slots, boff, bmask, k, key, valid, tmp, lane)
+                                 : coop_cget64(slots, boff, bmask, occ, key, valid, tmp, lane);
       const bool has = c != 0xFFFFFFFFu;
       const uint64_t m = __ballot(has);
       if (has) hits[n + __popcll(m & ((1ull << lane) - 1))] = c;
