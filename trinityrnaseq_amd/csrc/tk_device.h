@@ -471,6 +471,21 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
+// Adds the block's sum of v to *out with one atomic (every thread of the
+// block must call it; blockDim <= 1024).  Atomics on one address are
+// serialized at the memory side, so one per wavefront of a large grid costs
+// milliseconds.
+__device__ __forceinline__ void block_sum_add(uint64_t v, unsigned long long* out) {
+  __shared__ uint64_t s_part[16];
+  v = wave_sum_u64(v);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t tot = 0;
+    for (uint32_t w = 0; w < (blockDim.x + 63) / 64; w++) tot += s_part[w];
+    if (tot) atomicAdd(out, (unsigned long long)tot);
+  }
+}
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);

@@ -407,8 +407,7 @@ __global__ void k_windows(const uint32_t* __restrict__ len, uint64_t n, int k,
     uint32_t L = len[i];
     s += L >= (uint32_t)k ? L - k + 1 : 0;
   }
-  s = wave_sum_u64(s);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(total, (unsigned long long)s);
+  block_sum_add(s, total);
 }
 
 static int reads_build(tk_ctx* ctx, const char* d_seq, const uint64_t* d_off, uint64_t n,
@@ -1624,8 +1623,7 @@ __global__ void k_dump_count(TableView t, uint64_t n_slots, uint64_t lo, uint64_
     v = v > 0xFFFFFFFFull ? 0xFFFFFFFFull : v;  // counts print saturated at 2^32-1
     c += (v >= lo && v <= hi);
   }
-  c = wave_sum_u64(c);
-  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
+  block_sum_add(c, out);
 }
 
 __global__ void k_dump(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
@@ -1728,8 +1726,7 @@ __global__ void k_prune(TableView t, uint64_t n_slots, int k, uint64_t min_count
       t.slots[i] = s & t.kmo;
     }
   }
-  nk = wave_sum_u64(nk);
-  if ((threadIdx.x & 63) == 0 && nk) atomicAdd(kept, (unsigned long long)nk);
+  block_sum_add(nk, kept);
 }
 
 __global__ void k_lookup(TableView t, const uint64_t* __restrict__ keys, uint64_t n,
@@ -1855,11 +1852,15 @@ __global__ void k_select_flags(const tk_read_stat* __restrict__ L, const tk_read
   }
 }
 
-// drand48 state after `steps` draws from x: x <- A_j x + C_j for each set bit j.
+// drand48 state after `steps` < 2^NB draws from x: x <- A_j x + C_j for each
+// set bit j.  The loop runs over all NB bits so that every jump.a[j] /
+// jump.c[j] index is a constant (one scalar load each, shared by the wave).
+template <int NB>
 __device__ __forceinline__ uint64_t lcg48_jump(const Lcg48Jump& jump, uint64_t x, uint64_t steps) {
   const uint64_t M = (1ull << 48) - 1;
-  for (int j = 0; steps; j++, steps >>= 1)
-    if (steps & 1) x = (jump.a[j] * x + jump.c[j]) & M;
+#pragma unroll
+  for (int j = 0; j < NB; j++)
+    if ((steps >> j) & 1) x = (jump.a[j] * x + jump.c[j]) & M;
   return x;
 }
 
@@ -1879,14 +1880,15 @@ __global__ __launch_bounds__(256) void k_select(const tk_read_stat* __restrict__
     __syncthreads();
     if (threadIdx.x == 0) {
       s_idx0 = idx[i0];
-      s_base = lcg48_jump(jump, state0, idx[i0]);  // state before draw idx[i0]
+      s_base = lcg48_jump<32>(jump, state0, idx[i0]);  // state before draw idx[i0] (u32 rank)
     }
     __syncthreads();
     const uint64_t i = i0 + threadIdx.x;
     if (i < n) {
       uint8_t s = 0;
       if (consumes[i]) {
-        const uint64_t x = lcg48_jump(jump, s_base, (uint64_t)(idx[i] - s_idx0) + 1);
+        // ranks within a block of 256 pairs differ by at most 255: at most 256 draws
+        const uint64_t x = lcg48_jump<9>(jump, s_base, (uint64_t)(idx[i] - s_idx0) + 1);
         const double r = (double)x * 0x1p-48;
         const double med = ((double)L[i].median + (double)R[i].median) / 2.0;
         s = (r <= (double)max_cov / med) ? 1 : 0;
@@ -1895,8 +1897,7 @@ __global__ __launch_bounds__(256) void k_select(const tk_read_stat* __restrict__
       cnt += s;
     }
   }
-  cnt = wave_sum_u64(cnt);
-  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(nsel, (unsigned long long)cnt);
+  block_sum_add(cnt, nsel);
 }
 
 // =====================================================================
@@ -3311,7 +3312,8 @@ int tk_select_pairs_d(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr,
   const uint64_t state0 = ((uint64_t)12345 << 16) | 0x330Eull;
   unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
   TK_HIP(hipMemsetAsync(d, 0, 8, c->stream));
-  TK_LAUNCH(c, "select", grid, 256, 0, k_select, dl, dr, n, consumes, idx, j, state0, max_cov,
+  const int sgrid = (int)std::min<uint64_t>(ceil_div(n, 256), (uint64_t)std::max(c->num_cu, 1) * 16);
+  TK_LAUNCH(c, "select", sgrid, 256, 0, k_select, dl, dr, n, consumes, idx, j, state0, max_cov,
             dsel, d);
   uint64_t h = 0;
   TK_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream));
