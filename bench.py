@@ -541,7 +541,7 @@ def main() -> int:
 
     # one sample after another in one process: the blocks of one job's table and read set go to
     # the next job instead of hipFree + hipMalloc (a large hipMalloc after large frees can stall
-    # for seconds on a box that ran earlier processes)
+    # for seconds on a device that earlier processes used)
     api.thread_cache(64 << 30)
     for i in range(args.warmup):
         t = time.perf_counter()
