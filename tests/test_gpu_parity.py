@@ -169,6 +169,28 @@ def test_count_resident_crowded_regions(api, ctx, region_kb, tile, monkeypatch):
     assert np.array_equal(gk, ok) and np.array_equal(gc, 3 * oc)
 
 
+@pytest.mark.parametrize("wpr", ["1", "2"])
+def test_count_resident_merge_windows_per_round(api, ctx, wpr, monkeypatch):
+    """The region-resident merge with one or two windows per thread between
+    its block barriers: homopolymer reads put a block's every window on one
+    key (the merge's per-flush occurrence bound, and counts past the slot's
+    count field), random ones fill its table to the load bound."""
+    say("count_resident_merge wpr", wpr)
+    monkeypatch.setenv("TK_MERGE_WPR", wpr)
+    monkeypatch.setenv("TK_PARTITION", "0")
+    monkeypatch.setenv("TK_ORDER", "1")
+    monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
+    monkeypatch.setenv("TK_RESIDENT", "1")
+    monkeypatch.setenv("TK_REGION_KB", "4")
+    seqs = (random_reads(41, 30000, 25, 150, n_rate=0.002) + repetitive_reads(25, 6000)
+            + ["A" * 150] * 3000 + ["ACG" * 50] * 500)
+    r, t, seq, off = _count(api, ctx, seqs, 25, True)
+    ok, oc = O.count(seq, off, 25, True)
+    assert oc.max() > 8064
+    gk, gc = t.dump()
+    assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+
+
 @pytest.mark.parametrize("resident", ["1", "0"])
 def test_count_reused_blocks(api, ctx, resident, monkeypatch):
     """With the thread's block reuse on, each table takes the block of the
