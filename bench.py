@@ -46,7 +46,7 @@ STATS_B_PER_WIN, STATS_B_PER_READ = 64.375, 44.0
 # 5.04 G windows, counters calibrated against kernels of the same access shape), from
 # profiles/large_pe_20M_pairs_count_stats.md.  The counters cannot be read inside this
 # process, so the profiled figures are carried here for roofline.traffic.
-TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (212.5 + 1.3) / 5.04}
+TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (211.8 + 1.3) / 5.04}
 
 
 def log(*a):
