@@ -2671,6 +2671,15 @@ static int merge_wpr() {
   return e && atoi(e) == 1 ? 1 : 2;
 }
 
+// Windows per thread between the block barriers of the merge that adds its
+// words with table atomics (TK_AGG_WPR = 2, the default, or 1; 4,096 slots,
+// k >= 16): 22.3 vs 22.6 ms per 1.26 G windows, the table atomics setting
+// most of its pace.
+static int agg_wpr() {
+  const char* e = getenv("TK_AGG_WPR");
+  return e && atoi(e) == 1 ? 1 : 2;
+}
+
 // LDS words of the merge table of the region-resident count: TK_RES_SLOTS =
 // 3840 (default, 30 KB: five blocks per CU), 3328 (26 KB: more flushes) or
 // 8192 (64 KB: two blocks).
@@ -2957,6 +2966,9 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
                 t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
     else if (agg == 8192)
       TK_LAUNCH(c, "count", grid, 256, 8192 * 8, (k_count_agg<8192, false, false>), rview, rb, rb + nreads, t->k,
+                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
+    else if (agg == 4096 && t->k >= 16 && agg_wpr() == 2)
+      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false, true, 2>), rview, rb, rb + nreads, t->k,
                 t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
     else if (agg == 4096 && t->k >= 16)
       TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false, true>), rview, rb, rb + nreads, t->k,
