@@ -2363,10 +2363,10 @@ static int hll_estimate(tk_ctx* c, const tk_reads* r, uint64_t rb, uint64_t re, 
     if ((rc = dev_alloc((void**)&order, r->n_reads * 4))) { dev_free(keys, r->n_reads * 8); return rc; }
   }
   if (re > rb) {
-    // blocks per CU (TK_HLL_BLOCKS_PER_CU, 16 default): each block ends with one
+    // blocks per CU (TK_HLL_BLOCKS_PER_CU, 64 default): each block ends with one
     // atomicMax per register into the global registers
     const char* hb = getenv("TK_HLL_BLOCKS_PER_CU");
-    const uint64_t per_cu = hb && atoi(hb) > 0 ? (uint64_t)atoi(hb) : 16;
+    const uint64_t per_cu = hb && atoi(hb) > 0 ? (uint64_t)atoi(hb) : 64;
     const uint64_t grid = std::min<uint64_t>(ceil_div(re - rb, 256), (uint64_t)std::max(c->num_cu, 1) * per_cu);
     hipEvent_t a;
     timing_begin(c, &a);
