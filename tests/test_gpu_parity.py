@@ -135,7 +135,7 @@ def test_table_growth(api, ctx, layout):
     assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
 
 
-@pytest.mark.parametrize("tile", ["2048", "4096", "8192"])
+@pytest.mark.parametrize("tile", ["2048", "4096", "8192", "16384"])
 @pytest.mark.parametrize("region_kb", ["1", "64"])
 def test_count_resident_crowded_regions(api, ctx, region_kb, tile, monkeypatch):
     """Region-resident count into a table grown at load 0.9: with 128-slot

@@ -2623,12 +2623,24 @@ static bool resident_usable(tk_ctx* c, const tk_table* t) {
 }
 
 // Words per LDS tile of the region sort's scatter: TK_RES_TILE=8192
-// (default; 512 threads per block, two blocks per CU), 4096 or 2048 (256
-// threads).  Larger tiles leave longer runs per bin: 8,192 words over 512
-// bins are 128 B per run on average.
+// (default; 512 threads per block, two blocks per CU), 16384 (1,024
+// threads, one block per CU, when its LDS can be granted: slower, its
+// phases no longer overlap another block's), 4096 or 2048 (256 threads).  Larger tiles leave longer runs per bin: 8,192 words over
+// 512 bins are 128 B per run on average.
+static bool res_tile_16k_ok() {
+  static int ok = -1;
+  if (ok < 0) {
+    ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<16384, 1024>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)res_scatter_lds(kResMaxBins, 16384)) == hipSuccess;
+    (void)hipGetLastError();
+  }
+  return ok == 1;
+}
 static uint32_t res_tile() {
   const char* e = getenv("TK_RES_TILE");
   const int v = e ? atoi(e) : 0;
+  if (v == 16384 && res_tile_16k_ok()) return 16384;
   return v == 2048 || v == 4096 ? (uint32_t)v : 8192;
 }
 
@@ -2647,13 +2659,17 @@ static int res_sort_pass(tk_ctx* c, const std::vector<ResSlice>& sl, ResSlice* d
   TK_LAUNCH(c, "count_hist", sl.size(), 256, 0, k_res_hist, in, dsl, v, shift, mask, nbins, h);
   int rc = exclusive_scan_u64(c, h, hs, nhist + 1);
   if (rc) return rc;
-  const uint32_t tile = res_tile();
+  uint32_t tile = res_tile();
+  if (tile == 16384 && res_scatter_lds(nbins, 16384) + 1024 > 160 * 1024) tile = 8192;  // LDS of one CU
   if (tile == 2048)
     TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins, 2048), (k_res_scatter<2048, 256>), in,
               dsl, v, shift, mask, nbins, hs, out);
   else if (tile == 8192)
     TK_LAUNCH(c, "count_scatter", sl.size(), 512, res_scatter_lds(nbins, 8192), (k_res_scatter<8192, 512>), in,
               dsl, v, shift, mask, nbins, hs, out);
+  else if (tile == 16384)
+    TK_LAUNCH(c, "count_scatter", sl.size(), 1024, res_scatter_lds(nbins, 16384), (k_res_scatter<16384, 1024>),
+              in, dsl, v, shift, mask, nbins, hs, out);
   else
     TK_LAUNCH(c, "count_scatter", sl.size(), 256, res_scatter_lds(nbins, 4096), (k_res_scatter<4096, 256>), in,
               dsl, v, shift, mask, nbins, hs, out);
