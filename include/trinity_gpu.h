@@ -211,8 +211,9 @@ int tk_read_cov_stats_d(tk_ctx* ctx, tk_table* t, const tk_reads* reads,
  * are already in sorted-accession order with left i paired to right i.
  * Requires stats computed with missing_value >= 1 and
  * max_cv >= 1.2*sqrt(max nwin) + 1 (then the coefficient-of-variation and
- * zero-mean tests depend on nwin only and the selection is exact).
- * d_selected[i] = 1 when pair i is kept.
+ * zero-mean tests depend on nwin only and the selection is exact); a read
+ * with windows whose mean is below 1, or a drawn pair whose merged median
+ * is 0, fails with TK_ERR_ARG.  d_selected[i] = 1 when pair i is kept.
  */
 int tk_select_pairs_d(tk_ctx* ctx, const tk_read_stat* d_left, const tk_read_stat* d_right,
                       uint64_t n_pairs, int max_cov, int min_cov, int max_cv,

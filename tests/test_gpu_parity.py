@@ -378,6 +378,73 @@ def test_select_pairs_parity(api, ctx):
         assert ns == int(want.sum())
 
 
+def test_select_pairs_edge_rules(api, ctx):
+    """Pairs whose reads have no window are skipped without a rand() draw
+    (merged mean "0.0"), and the draws after them keep Perl's sequence; stats
+    whose mean is below 1 (missing k-mers counted as 0) are refused, and so is
+    a drawn pair whose merged median is 0 (the script divides by it)."""
+    say("select_pairs_edge_rules")
+    rng = np.random.default_rng(8)
+    n = 5000
+    st = np.zeros(2 * n, api.READ_STAT_DTYPE)
+    nwin = rng.integers(1, 127, 2 * n).astype(np.uint32)
+    nwin[rng.random(2 * n) < 0.05] = 0
+    both = rng.random(n) < 0.03
+    nwin[:n][both] = 0
+    nwin[n:][both] = 0
+    med = rng.integers(1, 300, 2 * n).astype(np.uint32)
+    med[nwin == 0] = 0
+    st["nwin"] = nwin
+    st["median"] = med
+    st["sum"] = nwin.astype(np.uint64) * med.astype(np.uint64)
+    st["sumsq_lo"] = st["sum"] * med.astype(np.uint64)
+    sl, sr = st[:n], st[n:]
+    for min_cov in (0, 1, 5):
+        sel, ns = api.select_pairs(ctx, sl, sr, 50, min_cov, 10000)
+        row = lambda s: O.format_stats_row("x", s).split("\t")
+        lf = [row(s) for s in sl]
+        rf = [row(s) for s in sr]
+        want = O.select_pairs([float(x[1]) for x in lf], [float(x[1]) for x in rf],
+                              [float(x[2]) for x in lf], [float(x[2]) for x in rf],
+                              [float(x[3]) for x in lf], [float(x[3]) for x in rf], 50, min_cov, 10000)
+        assert np.array_equal(sel, want), min_cov
+        assert ns == int(want.sum())
+        assert not sel[both].any()
+    low = st.copy()
+    low["sum"][0] = 0  # a read with windows and mean 0
+    with pytest.raises(api.TkError, match="below 1"):
+        api.select_pairs(ctx, low[:n], low[n:], 50, 0, 10000)
+    zero = st.copy()
+    zero["median"][0] = 0
+    zero["median"][n] = 0
+    with pytest.raises(api.TkError, match="median 0"):
+        api.select_pairs(ctx, zero[:n], zero[n:], 50, 0, 10000)
+
+
+@pytest.mark.parametrize("chunk", ["", "1000", "65536"])
+def test_dump_in_key_range_chunks(api, ctx, small_se, chunk, monkeypatch):
+    """A dump larger than one sort call takes goes out in key-range chunks,
+    each sorted on the device; a small chunk limit forces many chunks.  The
+    concatenation is the oracle's ascending dump, for -L 1 and -L 2."""
+    say("dump_in_key_range_chunks", chunk or "default")
+    if chunk:
+        monkeypatch.setenv("TK_DUMP_CHUNK", chunk)
+    seq, off = small_se
+    r, t, _, _ = _count(api, ctx, (seq, off))
+    ok, oc = O.count(seq, off, 25, True)
+    gk, gc = t.dump()
+    assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+    k2, c2 = t.dump(2)
+    m = oc >= 2
+    assert np.array_equal(k2, ok[m]) and np.array_equal(c2, oc[m])
+    # short keys: the bins cover the whole 2k-bit key space
+    seqs = random_reads(5, 3000, 20, 60, n_rate=0.01)
+    r5, t5, s5, o5 = _count(api, ctx, seqs, 5, True, expected=64)
+    gk, gc = t5.dump()
+    ok5, oc5 = O.count(s5, o5, 5, True)
+    assert np.array_equal(gk, ok5) and np.array_equal(gc, oc5)
+
+
 @pytest.mark.parametrize("ordered", [False, True])
 @pytest.mark.parametrize("canonical", [True, False])
 def test_r2t_parity(api, ctx, canonical, ordered, monkeypatch):

@@ -460,8 +460,12 @@ int tk_batch_create(tk_ctx* c, int k, int canonical, const tk_reads* reads, cons
   std::vector<uint32_t> rl(reads->n_reads), cl(contigs->n_reads);
   int rc = TK_OK;
   hipError_t e = hipSuccess;
-  if (reads->n_reads) e = hipMemcpy(rl.data(), reads->d_len, reads->n_reads * 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && contigs->n_reads) e = hipMemcpy(cl.data(), contigs->d_len, contigs->n_reads * 4, hipMemcpyDeviceToHost);
+  // copies on the context's stream (non-blocking: a null-stream copy would not
+  // wait for work queued there, and a reused block may still be in use by it)
+  if (reads->n_reads) e = hipMemcpyAsync(rl.data(), reads->d_len, reads->n_reads * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && contigs->n_reads)
+    e = hipMemcpyAsync(cl.data(), contigs->d_len, contigs->n_reads * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) { delete b; return hip_fail(e, "batch lengths"); }
   b->comp_windows.assign(n_comp, 0);
   b->comp_cwindows.assign(n_comp, 0);
@@ -486,13 +490,18 @@ int tk_batch_create(tk_ctx* c, int k, int canonical, const tk_reads* reads, cons
   if (!rc) rc = dev_alloc((void**)&b->d_contig_comp, std::max<uint64_t>(contigs->n_reads, 1) * 4);
   if (!rc) rc = dev_alloc((void**)&b->d_contig_local, std::max<uint64_t>(contigs->n_reads, 1) * 4);
   if (!rc && reads->n_reads)
-    if ((e = hipMemcpy(b->d_read_comp, rcomp.data(), reads->n_reads * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(b->d_read_comp, rcomp.data(), reads->n_reads * 4, hipMemcpyHostToDevice, c->stream)) !=
+        hipSuccess)
       rc = hip_fail(e, "upload read components");
   if (!rc && contigs->n_reads) {
-    if ((e = hipMemcpy(b->d_contig_comp, ccomp.data(), contigs->n_reads * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(b->d_contig_local, clocal.data(), contigs->n_reads * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(b->d_contig_comp, ccomp.data(), contigs->n_reads * 4, hipMemcpyHostToDevice,
+                            c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(b->d_contig_local, clocal.data(), contigs->n_reads * 4, hipMemcpyHostToDevice,
+                            c->stream)) != hipSuccess)
       rc = hip_fail(e, "upload contig components");
   }
+  // the host vectors are freed on return
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess && !rc) rc = hip_fail(e, "batch uploads");
   // one-word component-table slots when every contig index fits beside the
   // key (TK_BATCH_CT_PACKED=0: always the two-word slots)
   {
@@ -531,11 +540,13 @@ int tk_batch_run(tk_ctx* c, tk_batch* b, const tk_stats_opts* o, tk_read_stat* s
   if (!rc) rc = tk_batch_run_d(c, b, o, ds, dv);
   b->use_graph = g;
   if (!rc && stats && n) {
-    hipError_t e = hipMemcpy(stats, ds, n * sizeof(tk_read_stat), hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpyAsync(stats, ds, n * sizeof(tk_read_stat), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) rc = hip_fail(e, "batch stats download");
   }
   if (!rc && votes && n) {
-    hipError_t e = hipMemcpy(votes, dv, n * sizeof(tk_r2t_vote), hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpyAsync(votes, dv, n * sizeof(tk_r2t_vote), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) rc = hip_fail(e, "batch votes download");
   }
   dev_free(ds, std::max<uint64_t>(n, 1) * sizeof(tk_read_stat));

@@ -97,8 +97,6 @@ struct tk_table {
   bool finalized = true;           // an empty table is trivially final
   bool zero_pending = false;       // slots not yet cleared (cleared before first use)
   uint64_t bytes = 0;
-  tk_table* compact = nullptr;     // keys with count >= compact_min, built for scoring
-  uint32_t compact_min = 0;
 };
 
 struct tk_ctable {
@@ -118,7 +116,18 @@ int sort_pairs_u64_u32(tk_ctx* ctx, uint64_t* d_keys, uint32_t* d_vals, uint64_t
                        int key_bits);
 int exclusive_scan_u64(tk_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n);
 int exclusive_scan_u32(tk_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n);
+int exclusive_scan_u32_in(tk_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* temp,
+                          size_t temp_bytes);
 int ensure_scratch(tk_ctx* ctx, int slot, size_t bytes, void** p);
+// The pair selection on device stats (tk_select_pairs_d) with its working
+// memory given: consumes and idx (n words each), the scan's temporary storage
+// (NULL: the context's) and 16 flag words; select_check validates the
+// arguments.
+int select_check(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint64_t n, int max_cov,
+                 int max_cv, uint32_t max_nwin, uint8_t* dsel);
+int select_run(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint64_t n, int max_cov,
+               int min_cov, uint8_t* dsel, uint32_t* consumes, uint32_t* idx, void* scan_temp,
+               size_t scan_bytes, uint32_t* flags, uint64_t* n_selected);
 // Launch bracket for per-kernel timing on ctx->stream.
 void timing_begin(tk_ctx* ctx, hipEvent_t* a);
 void timing_end(tk_ctx* ctx, const char* name, hipEvent_t a);

@@ -528,14 +528,6 @@ __global__ __launch_bounds__(256) void k_minimizer(ReadsView rv, uint64_t* __res
   }
 }
 
-// TK_COOP8=1 counts with eight lanes (8 bytes each) per key instead of four
-// lanes of 16 bytes.
-static bool coop8_enabled() {
-  const char* e = getenv("TK_COOP8");
-  return e && e[0] == '1';
-}
-
-
 // Which table passes take reads in the locality order: TK_ORDER=0 none,
 // TK_ORDER=stats the scoring pass only, TK_ORDER=1 (default) count and
 // scoring.
@@ -688,8 +680,8 @@ __global__ __launch_bounds__(256) void k_hll(ReadsView rv, uint64_t r_begin, uin
 constexpr int kReadsPerBlock = 64;
 
 // One thread per k-window of a block of 64 consecutive reads.  flags[0] gets
-// error bits, flags[2..3] (u64) the number of newly claimed slots.
-template <int LANES>
+// error bits, flags[2..3] (u64) the number of newly claimed slots.  Four
+// lanes of 16 bytes probe each key's bucket.
 __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
                                                uint64_t r_end, int k, int canonical,
                                                TableView t, uint32_t* __restrict__ flags) {
@@ -736,10 +728,7 @@ __global__ __launch_bounds__(256) void k_count(ReadsView rv, uint64_t r_begin,
     }
     const uint64_t key = valid ? canon_key(fwd, k, canonical) : 0;
     bool lost = false;
-    if (LANES == 4)
-      mynew += coop4_add(t, key, valid, tid & 63, &flags[0], lost);
-    else
-      mynew += coop_add64(t, key, valid, tid & 63, &flags[0], lost);
+    mynew += coop4_add(t, key, valid, tid & 63, &flags[0], lost);
     if (lost) atomicOr(&flags[0], 1u);
   }
   mynew = wave_sum_u32(mynew);
@@ -1576,18 +1565,8 @@ static int table_grow(tk_ctx* ctx, tk_table* t, uint64_t min_keys) {
   return TK_OK;
 }
 
-// Free the scoring copy of a table (it is rebuilt on demand).
-static void drop_compact(tk_table* t) {
-  if (t->compact) {
-    tk_table_free(t->compact);
-    t->compact = nullptr;
-    t->compact_min = 0;
-  }
-}
-
 static int table_set_final(tk_ctx* ctx, tk_table* t, bool final_) {
   if (int rc = table_zero(ctx, t)) return rc;
-  if (!final_) drop_compact(t);  // the table is about to change
   if (t->finalized == final_) return TK_OK;
   TK_HIP(hipMemsetAsync(ctx->d_flags, 0, 4, ctx->stream));
   TableView v = table_view(t);
@@ -1636,7 +1615,8 @@ __global__ void k_dump_count(TableView t, uint64_t n_slots, uint64_t lo, uint64_
   block_sum_add(c, out);
 }
 
-__global__ void k_dump(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
+// Keys with count in [lo, hi] and key in [klo, khi] appended at *cursor.
+__global__ void k_dump(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi, uint64_t klo, uint64_t khi,
                        uint64_t* __restrict__ keys, uint32_t* __restrict__ counts,
                        uint64_t cap, unsigned long long* __restrict__ cursor) {
   const int lane = threadIdx.x & 63;
@@ -1650,8 +1630,8 @@ __global__ void k_dump(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
       if (s && (s >> t.cshift) != 0) {
         v = slot_count(t, s);
         v = v > 0xFFFFFFFFull ? 0xFFFFFFFFull : v;
-        take = v >= lo && v <= hi;
         key = s & (t.occ - 1);
+        take = v >= lo && v <= hi && key >= klo && key <= khi;
       }
     }
     const uint64_t m = __ballot(take);
@@ -1670,6 +1650,27 @@ __global__ void k_dump(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
 }
 
 constexpr int kHistoLds = 4096;
+
+// Keys with count in [lo, hi] per key range: bin = key >> shift (< kHistoLds
+// bins), so a dump of more keys than one sort call takes can be cut into
+// key ranges that are sorted and downloaded one after another.
+__global__ __launch_bounds__(256) void k_dump_bins(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
+                                                   uint32_t shift, unsigned long long* __restrict__ bins) {
+  __shared__ uint32_t s_bins[kHistoLds];
+  for (int i = threadIdx.x; i < kHistoLds; i += blockDim.x) s_bins[i] = 0;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = t.slots[i];
+    if (!s || (s >> t.cshift) == 0) continue;
+    uint64_t v = slot_count(t, s);
+    v = v > 0xFFFFFFFFull ? 0xFFFFFFFFull : v;
+    if (v >= lo && v <= hi) atomicAdd(&s_bins[(s & (t.occ - 1)) >> shift], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHistoLds; i += blockDim.x)
+    if (s_bins[i]) atomicAdd(&bins[i], (unsigned long long)s_bins[i]);
+}
 
 __global__ __launch_bounds__(256) void k_histo(TableView t, uint64_t n_slots, uint64_t base,
                                                uint64_t ceil_, uint64_t inc, uint64_t nb,
@@ -1747,25 +1748,6 @@ __global__ void k_lookup(TableView t, const uint64_t* __restrict__ keys, uint64_
     uint64_t v = (key & ~(t.occ - 1)) ? 0 : table_get(t, key);
     out[i] = v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v;
   }
-}
-
-// Copy the keys of a finalised table whose count is >= min_count into dst.
-__global__ void k_compact(TableView src, uint64_t n_slots, uint64_t min_count, TableView dst,
-                          uint32_t* __restrict__ flags) {
-  uint32_t mynew = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t s = src.slots[i];
-    if (!s || (s >> src.cshift) == 0) continue;
-    const uint64_t c = slot_count(src, s);
-    if (c < min_count) continue;
-    const uint32_t r = table_add(dst, s & (src.occ - 1), c, &flags[0]);
-    if (r == 2u) atomicOr(&flags[0], 1u);
-    else mynew += r;
-  }
-  mynew = wave_sum_u32(mynew);
-  if ((threadIdx.x & 63) == 0 && mynew)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)mynew);
 }
 
 // =====================================================================
@@ -1851,15 +1833,33 @@ struct Lcg48Jump {
   uint64_t c[48];
 };
 
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+// consumes[i] = 1 when pair i reaches nbkc_normalize.pl's rand() draw
+// (:91-111).  The merged median is exact ((l+r)/2 of integers, printed with
+// "%.1f").  The zero-mean test is exact when every read with windows has a
+// mean of at least 1 (sum >= nwin, as with missing k-mers counted as 1): the
+// merged mean is then 0 exactly when neither read has a window.  err gets
+// bit 0 for a read whose mean is below 1 (its printed mean could round the
+// merged mean to "0.0"), bit 1 for a drawn pair whose median is 0 (the
+// script divides max_cov by it and dies).
 __global__ void k_select_flags(const tk_read_stat* __restrict__ L, const tk_read_stat* __restrict__ R,
-                               uint64_t n, int min_cov, uint32_t* __restrict__ consumes) {
+                               uint64_t n, int min_cov, uint32_t* __restrict__ consumes,
+                               uint32_t* __restrict__ err) {
+  uint32_t e = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    const double med = ((double)L[i].median + (double)R[i].median) / 2.0;
+    const tk_read_stat l = L[i], r = R[i];
+    const double med = ((double)l.median + (double)r.median) / 2.0;
     bool use = !(med < (double)min_cov);
-    if (use && L[i].nwin == 0 && R[i].nwin == 0) use = false;  // merged mean "0.0" <= 0
+    if (l.sum + r.sum == 0) use = false;  // merged mean "0.0" <= 0
+    if ((l.nwin && l.sum < l.nwin) || (r.nwin && r.sum < r.nwin)) e |= 1u;
+    if (use && med == 0.0) e |= 2u;
     consumes[i] = use ? 1u : 0u;
   }
+  if (e) atomicOr(err, e);
 }
 
 // drand48 state after `steps` < 2^NB draws from x: x <- A_j x + C_j for each
@@ -2397,13 +2397,6 @@ static int hll_estimate(tk_ctx* c, const tk_reads* r, uint64_t rb, uint64_t re, 
   return TK_OK;
 }
 
-// TK_COMPACT=1 scores against a compact copy when min_count > 1 (off by
-// default: measured no faster on the 20M-pair workload once build time counts).
-static bool compact_enabled() {
-  const char* e = getenv("TK_COMPACT");
-  return e && e[0] == '1';
-}
-
 // TK_PARTITION=1 counts large tables in region-partitioned passes (off by
 // default: measured slower than the direct count on the 20M-pair workload).
 static bool partition_enabled() {
@@ -2560,30 +2553,16 @@ static int count_partitioned(tk_ctx* c, tk_table* t, const tk_reads* r, uint64_t
   return rc;
 }
 
-// LDS words of the block-level aggregation of the ordered count (0 = none).
-// Default 4096 (32 KB, windows merged 2048 at a time): four blocks per CU,
-// which the kernel's registers allow anyway; the 64 KB table merges more per
-// pass but halves the blocks per CU and measured slower.  TK_AGG=0/8192
-// overrides.
-static int agg_slots(tk_ctx* c) {
-  const char* e = getenv("TK_AGG");
-  if (e && e[0] == '0') return 0;
-  if (!e || atoi(e) != 8192) return 4096;
-  static int big = -1;
-  if (big < 0) {
-    int maxlds = 0;
-    (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-    big = 0;
-    if (maxlds >= 8192 * 8 + 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, false, true>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) == hipSuccess &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, false, false>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) == hipSuccess)
-      big = 1;
-    (void)hipGetLastError();
-  }
-  return big ? 8192 : 4096;
-}
+// LDS words of the block-level aggregation of the ordered count that adds its
+// merged words with table atomics: 4096 (32 KB, windows merged 2048 at a
+// time, four blocks per CU, which the kernel's registers allow anyway).  A
+// 64 KB table merged more per pass but halved the blocks per CU and measured
+// slower.
+constexpr int kAggSlots = 4096;
+// LDS words of the merge table of the region-resident count: 3840 (30 KB,
+// five blocks per CU).  3,328 slots (more flushes) and 8,192 (two blocks per
+// CU) measured slower.
+constexpr int kResSlots = 3840;
 
 // Whether a count of `windows` windows takes the region-resident passes
 // (reads in the locality order, k <= 25, tables of 64 KB regions).  By
@@ -2599,27 +2578,32 @@ static bool resident_enabled(uint64_t windows) {
   return windows >= (1ull << 32);
 }
 
+// The kernel attributes below are set once per process, by whichever thread
+// gets here first (a function-local static is initialised under the
+// language's own once-only guard, so server workers may race to it).
 static bool resident_usable(tk_ctx* c, const tk_table* t) {
   if (t->rbits > kResRegionBitsMax || (t->nb >> t->rbits) > (uint64_t)kResMaxBins * kResMaxBins) return false;
-  static int ok = -1;  // 64 KB or more of dynamic LDS per workgroup needs the attribute
-  if (ok < 0) {
-    ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<true>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 72 << kResRegionBitsMax) == hipSuccess &&
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<false>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 72 << kResRegionBitsMax) == hipSuccess &&
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<8192, 512>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)res_scatter_lds(kResMaxBins, 8192)) == hipSuccess &&
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<4096, 256>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)res_scatter_lds(kResMaxBins, 4096)) == hipSuccess &&
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<2048, 256>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)res_scatter_lds(kResMaxBins, 2048)) == hipSuccess;
+  // 64 KB or more of dynamic LDS per workgroup needs the attribute
+  static const bool ok = [] {
+    const bool r =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 72 << kResRegionBitsMax) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_count<false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 72 << kResRegionBitsMax) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<8192, 512>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)res_scatter_lds(kResMaxBins, 8192)) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<4096, 256>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)res_scatter_lds(kResMaxBins, 4096)) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<2048, 256>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)res_scatter_lds(kResMaxBins, 2048)) == hipSuccess;
     (void)hipGetLastError();
-  }
+    return r;
+  }();
   (void)c;
-  return ok == 1;
+  return ok;
 }
 
 // Words per LDS tile of the region sort's scatter: TK_RES_TILE=8192
@@ -2628,14 +2612,14 @@ static bool resident_usable(tk_ctx* c, const tk_table* t) {
 // phases no longer overlap another block's), 4096 or 2048 (256 threads).  Larger tiles leave longer runs per bin: 8,192 words over
 // 512 bins are 128 B per run on average.
 static bool res_tile_16k_ok() {
-  static int ok = -1;
-  if (ok < 0) {
-    ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<16384, 1024>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)res_scatter_lds(kResMaxBins, 16384)) == hipSuccess;
+  static const bool ok = [] {
+    const bool r = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_scatter<16384, 1024>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)res_scatter_lds(kResMaxBins, 16384)) == hipSuccess;
     (void)hipGetLastError();
-  }
-  return ok == 1;
+    return r;
+  }();
+  return ok;
 }
 static uint32_t res_tile() {
   const char* e = getenv("TK_RES_TILE");
@@ -2687,36 +2671,6 @@ static int merge_wpr() {
   return e && atoi(e) == 1 ? 1 : 2;
 }
 
-// Windows per thread between the block barriers of the merge that adds its
-// words with table atomics (TK_AGG_WPR = 2, the default, or 1; 4,096 slots,
-// k >= 16): 22.3 vs 22.6 ms per 1.26 G windows, the table atomics setting
-// most of its pace.
-static int agg_wpr() {
-  const char* e = getenv("TK_AGG_WPR");
-  return e && atoi(e) == 1 ? 1 : 2;
-}
-
-// LDS words of the merge table of the region-resident count: TK_RES_SLOTS =
-// 3840 (default, 30 KB: five blocks per CU), 3328 (26 KB: more flushes) or
-// 8192 (64 KB: two blocks).
-static int res_slots() {
-  static int slots = 0;
-  if (!slots) {
-    const char* e = getenv("TK_RES_SLOTS");
-    const int v = e ? atoi(e) : 0;
-    slots = v == 8192 || v == 3328 ? v : 3840;
-    if (slots == 8192 &&
-        (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, true, true>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess ||
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count_agg<8192, true, false>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 8192 * 8) != hipSuccess)) {
-      (void)hipGetLastError();
-      slots = 3840;
-    }
-  }
-  return slots;
-}
-
 // Stream capacity of a region-resident count of `windows` windows: the
 // words the merge can reserve (merged words <= windows, plus < one flush
 // (at most `slots` words) given up per reserved chunk of `echunk`, plus each
@@ -2725,21 +2679,15 @@ static int res_slots() {
 // (the merge of a very large or error-rich read set leaves too many words).
 static bool resident_plan(tk_ctx* c, uint64_t windows, uint64_t* cap, uint64_t* echunk, uint64_t* mgrid,
                           uint64_t* cap_max) {
-  const uint64_t slots = (uint64_t)res_slots();
-  static int per_cu[3] = {0, 0, 0};
-  int& pc = per_cu[slots == 8192 ? 2 : slots == 3328 ? 1 : 0];
-  if (!pc) {
+  const uint64_t slots = (uint64_t)kResSlots;
+  static const int pc = [] {  // resident merge blocks per CU (same for every device of the process)
     int nb = 0;
-    const hipError_t e =
-        slots == 8192
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<8192, true, true>), 256, 8192 * 8)
-        : slots == 3328
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<3328, true, true>), 256, 3328 * 8)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<3840, true, true>), 256, 3840 * 8);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_count_agg<kResSlots, true, true, 2>),
+                                                                      256, kResSlots * 8);
     if (e != hipSuccess || nb < 1) nb = 1;
     (void)hipGetLastError();
-    pc = nb;
-  }
+    return nb;
+  }();
   *mgrid = (uint64_t)std::max(c->num_cu, 1) * pc;
   *echunk = std::min<uint64_t>(65536, std::max<uint64_t>(16384, next_pow2(windows / (*mgrid * 4))));
   *cap = windows + windows / (*echunk / slots - 1) + *mgrid * (*echunk + slots);
@@ -2796,20 +2744,15 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
     // persistent: as many blocks per CU as its registers and LDS allow
     const uint64_t grid = std::max<uint64_t>(8, std::min<uint64_t>((ceil_div(qe - q, kReadsPerBlock) + 7) / 8 * 8, mgrid));
-    const int slots = res_slots();
-#define TK_MERGE(S, KHI)                                                                               \
-  TK_LAUNCH(c, "count_merge", grid, 256, (S) * 8, (k_count_agg<S, true, KHI>), rview, q, qe, t->k, t->canonical, \
-            v, c->d_flags, A, cap, echunk, cursor)
-    if (slots == 3840 && t->k >= 16 && merge_wpr() == 2)
-      TK_LAUNCH(c, "count_merge", grid, 256, 3840 * 8, (k_count_agg<3840, true, true, 2>), rview, q, qe, t->k,
-                t->canonical, v, c->d_flags, A, cap, echunk, cursor);
-    else if (slots == 8192 && t->k >= 16) TK_MERGE(8192, true);
-    else if (slots == 8192) TK_MERGE(8192, false);
-    else if (slots == 3328 && t->k >= 16) TK_MERGE(3328, true);
-    else if (slots == 3328) TK_MERGE(3328, false);
-    else if (t->k >= 16) TK_MERGE(3840, true);
-    else TK_MERGE(3840, false);
-#undef TK_MERGE
+    if (t->k >= 16 && merge_wpr() == 2)
+      TK_LAUNCH(c, "count_merge", grid, 256, kResSlots * 8, (k_count_agg<kResSlots, true, true, 2>), rview, q, qe,
+                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else if (t->k >= 16)
+      TK_LAUNCH(c, "count_merge", grid, 256, kResSlots * 8, (k_count_agg<kResSlots, true, true>), rview, q, qe,
+                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+    else
+      TK_LAUNCH(c, "count_merge", grid, 256, kResSlots * 8, (k_count_agg<kResSlots, true, false>), rview, q, qe,
+                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor);
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table probe limit reached (flags=%u)", c->h_flags[0]);
@@ -2939,7 +2882,7 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
   } else {
     rview.order = nullptr;
   }
-  if (rview.order && 63 - 2 * t->k >= 13 && agg_slots(c) && resident_enabled(total) && max_nwin &&
+  if (rview.order && 63 - 2 * t->k >= 13 && resident_enabled(total) && max_nwin &&
       !partition_enabled() && resident_usable(c, t))
   {
     rc = count_resident(c, t, r, rview, 0, r->n_reads, total, max_nwin);
@@ -2976,28 +2919,17 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 16, c->stream));
     const uint64_t grid = (ceil_div(nreads, kReadsPerBlock) + 7) / 8 * 8;
     TableView v = table_view(t);
-    const int agg = rview.order && 63 - 2 * t->k >= 13 ? agg_slots(c) : 0;
-    if (agg == 8192 && t->k >= 16)
-      TK_LAUNCH(c, "count", grid, 256, 8192 * 8, (k_count_agg<8192, false, true>), rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
-    else if (agg == 8192)
-      TK_LAUNCH(c, "count", grid, 256, 8192 * 8, (k_count_agg<8192, false, false>), rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
-    else if (agg == 4096 && t->k >= 16 && agg_wpr() == 2)
-      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false, true, 2>), rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
-    else if (agg == 4096 && t->k >= 16)
-      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false, true>), rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
-    else if (agg == 4096)
-      TK_LAUNCH(c, "count", grid, 256, 4096 * 8, (k_count_agg<4096, false, false>), rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
-    else if (coop8_enabled())
-      TK_LAUNCH(c, "count", grid, 256, 0, k_count<8>, rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags);
+    // reads in the locality order: merge each block's windows in LDS first
+    // (two windows per thread per round for k >= 16); else a window per thread
+    const bool agg = rview.order && 63 - 2 * t->k >= 13;
+    if (agg && t->k >= 16)
+      TK_LAUNCH(c, "count", grid, 256, kAggSlots * 8, (k_count_agg<kAggSlots, false, true, 2>), rview, rb,
+                rb + nreads, t->k, t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
+    else if (agg)
+      TK_LAUNCH(c, "count", grid, 256, kAggSlots * 8, (k_count_agg<kAggSlots, false, false>), rview, rb,
+                rb + nreads, t->k, t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
     else
-      TK_LAUNCH(c, "count", grid, 256, 0, k_count<4>, rview, rb, rb + nreads, t->k,
-                t->canonical, v, c->d_flags);
+      TK_LAUNCH(c, "count", grid, 256, 0, k_count, rview, rb, rb + nreads, t->k, t->canonical, v, c->d_flags);
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 16, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table probe limit reached (flags=%u)", c->h_flags[0]);
@@ -3089,6 +3021,48 @@ int tk_table_dump_size(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_
   return TK_OK;
 }
 
+// Entries per sorted chunk of a dump: one sort call takes < 2^31 items, and a
+// chunk holds 24 bytes per entry on the device (keys and counts, twice for
+// the sort) within 40% of the free memory.  TK_DUMP_CHUNK sets a smaller
+// limit (read per call), so that small tables take the multi-chunk path too.
+static uint64_t dump_chunk_limit() {
+  uint64_t lim = 0x7FFFFFFFull;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess)
+    lim = std::min<uint64_t>(lim, std::max<uint64_t>(1u << 20, (uint64_t)(0.4 * (double)fr) / 24));
+  else
+    (void)hipGetLastError();
+  if (const char* e = getenv("TK_DUMP_CHUNK")) {
+    const uint64_t v = strtoull(e, nullptr, 10);
+    if (v) lim = std::min(lim, v);
+  }
+  return lim;
+}
+
+// Collect, sort and download the entries with key in [klo, khi] (n of them)
+// to keys/counts through the device buffers dk/dc.
+static int dump_range(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_t klo, uint64_t khi, uint64_t n,
+                      uint64_t* dk, uint32_t* dc, uint64_t* keys, uint32_t* counts) {
+  unsigned long long* cur = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
+  TK_HIP(hipMemsetAsync(cur, 0, 8, c->stream));
+  TableView v = table_view(t);
+  const uint64_t ns = t->nb * 8;
+  TK_LAUNCH(c, "dump", std::min<uint64_t>(ceil_div(ns, 256), 65536), 256, 0, k_dump, v, ns, (uint64_t)lo,
+            (uint64_t)hi, klo, khi, dk, dc, n, cur);
+  uint64_t got = 0;
+  TK_HIP(hipMemcpyAsync(&got, cur, 8, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  if (got != n)
+    return fail(TK_ERR_STATE, "dump: %llu entries in key range, %llu expected", (unsigned long long)got,
+                (unsigned long long)n);
+  int rc = sort_pairs_u64_u32(c, dk, dc, n, 2 * t->k);
+  if (rc) return rc;
+  TK_HIP(hipMemcpyAsync(keys, dk, n * 8, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipMemcpyAsync(counts, dc, n * 4, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipStreamSynchronize(c->stream));
+  return TK_OK;
+}
+
 int tk_table_dump(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_t* keys,
                   uint32_t* counts, uint64_t cap, uint64_t* n_out) {
   if (!c || !t || !n_out || (cap && (!keys || !counts))) return fail(TK_ERR_ARG, "NULL argument");
@@ -3099,32 +3073,63 @@ int tk_table_dump(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_t* ke
   if (n > cap) return fail(TK_ERR_CAP, "dump needs %llu entries, buffer holds %llu",
                            (unsigned long long)n, (unsigned long long)cap);
   if (!n) return TK_OK;
+  const uint64_t limit = dump_chunk_limit();
+  const uint64_t kmax = t->k >= 32 ? ~0ull : (1ull << (2 * t->k)) - 1;
+  // key ranges: one, or groups of consecutive bins of the key's top bits
+  std::vector<std::pair<uint64_t, uint64_t>> ranges;  // (first bin, entries), bins of `shift` low bits
+  uint32_t shift = 0;
+  std::vector<uint64_t> bins;
+  if (n <= limit) {
+    ranges.push_back({0, n});
+  } else {
+    const int kbits = 2 * t->k;
+    const int bbits = std::min(kbits, 12);  // kHistoLds bins
+    shift = (uint32_t)(kbits - bbits);
+    const uint64_t nbins = 1ull << bbits;
+    void* p = nullptr;
+    if ((rc = ensure_scratch(c, 1, nbins * 8, &p))) return rc;
+    TK_HIP(hipMemsetAsync(p, 0, nbins * 8, c->stream));
+    TableView v = table_view(t);
+    const uint64_t ns = t->nb * 8;
+    TK_LAUNCH(c, "dump_bins", std::min<uint64_t>(ceil_div(ns, 256), 65536), 256, 0, k_dump_bins, v, ns,
+              (uint64_t)lo, (uint64_t)hi, shift, (unsigned long long*)p);
+    bins.resize(nbins);
+    TK_HIP(hipMemcpyAsync(bins.data(), p, nbins * 8, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipStreamSynchronize(c->stream));
+    uint64_t first = 0, acc = 0, total = 0;
+    for (uint64_t b = 0; b < nbins; b++) {
+      if (bins[b] > limit)
+        return fail(TK_ERR_ARG, "dump: %llu keys share the top %d key bits (chunk limit %llu)",
+                    (unsigned long long)bins[b], bbits, (unsigned long long)limit);
+      if (acc && acc + bins[b] > limit) {
+        ranges.push_back({first, acc});
+        first = b;
+        acc = 0;
+      }
+      acc += bins[b];
+      total += bins[b];
+    }
+    ranges.push_back({first, acc});
+    if (total != n) return fail(TK_ERR_STATE, "dump: binned %llu of %llu entries", (unsigned long long)total,
+                                (unsigned long long)n);
+  }
+  uint64_t most = 0;
+  for (auto& r : ranges) most = std::max(most, r.second);
   uint64_t* dk = nullptr;
   uint32_t* dc = nullptr;
-  if ((rc = dev_alloc((void**)&dk, n * 8))) return rc;
-  if ((rc = dev_alloc((void**)&dc, n * 4))) { dev_free(dk, n * 8); return rc; }
-  do {
-    unsigned long long* cur = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
-    hipError_t e = hipMemsetAsync(cur, 0, 8, c->stream);
-    if (e != hipSuccess) { rc = hip_fail(e, "memset"); break; }
-    TableView v = table_view(t);
-    uint64_t ns = t->nb * 8;
-    {
-      hipEvent_t a;
-      timing_begin(c, &a);
-      hipLaunchKernelGGL(k_dump, dim3(std::min<uint64_t>(ceil_div(ns, 256), 65536)), dim3(256), 0,
-                         c->stream, v, ns, (uint64_t)lo, (uint64_t)hi, dk, dc, n, cur);
-      timing_end(c, "dump", a);
-      if ((e = hipGetLastError()) != hipSuccess) { rc = hip_fail(e, "k_dump"); break; }
-    }
-    if ((rc = sort_pairs_u64_u32(c, dk, dc, n, 2 * t->k))) break;
-    e = hipMemcpyAsync(keys, dk, n * 8, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(counts, dc, n * 4, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) rc = hip_fail(e, "dump download");
-  } while (0);
-  dev_free(dk, n * 8);
-  dev_free(dc, n * 4);
+  if ((rc = dev_alloc((void**)&dk, most * 8))) return rc;
+  if ((rc = dev_alloc((void**)&dc, most * 4))) { dev_free(dk, most * 8); return rc; }
+  uint64_t at = 0;
+  for (size_t i = 0; i < ranges.size() && !rc; i++) {
+    if (!ranges[i].second) continue;
+    const uint64_t klo = ranges.size() == 1 ? 0 : ranges[i].first << shift;
+    const uint64_t khi = (ranges.size() == 1 || i + 1 == ranges.size()) ? kmax : (ranges[i + 1].first << shift) - 1;
+    rc = dump_range(c, t, lo, hi, klo, khi, ranges[i].second, dk, dc, keys + at, counts + at);
+    at += ranges[i].second;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(dk, most * 8);
+  dev_free(dc, most * 4);
   return rc;
 }
 
@@ -3154,7 +3159,6 @@ int tk_table_histo(tk_ctx* c, tk_table* t, uint64_t low, uint64_t high, uint64_t
 int tk_table_prune(tk_ctx* c, tk_table* t, uint32_t min_count, const uint8_t* keep, uint64_t* n_kept) {
   if (!c || !t) return fail(TK_ERR_ARG, "NULL argument");
   if (set_device(c->device)) return TK_ERR_HIP;
-  drop_compact(t);
   int rc = table_set_final(c, t, true);
   if (rc) return rc;
   const uint64_t kt = (uint64_t)(t->k + 1) * (t->k + 1) * (t->k + 1);
@@ -3208,7 +3212,6 @@ int tk_table_lookup(tk_ctx* c, tk_table* t, const uint64_t* keys, uint64_t n, ui
 int tk_table_free(tk_table* t) {
   if (!t) return TK_OK;
   (void)hipSetDevice(t->device);
-  drop_compact(t);
   if (t->d_slots) dev_free(t->d_slots, t->nb * 64);
   if (t->d_ovf) dev_free(t->d_ovf, t->ovf_cap * 16);
   delete t;
@@ -3216,48 +3219,6 @@ int tk_table_free(tk_table* t) {
 }
 
 // ---------------- stats ----------------
-// Scoring with min_count > 1 treats keys below min_count as missing.  When
-// those are most of the table (error k-mers of a read set), score against a
-// compact copy holding only the keys >= min_count: an order of magnitude
-// fewer, mostly cache-resident buckets.  Returns the table to score against.
-static int scoring_table(tk_ctx* c, tk_table* t, uint32_t min_count, tk_table** use) {
-  *use = t;
-  if (min_count <= 1 || !compact_enabled()) return TK_OK;
-  if (t->compact && t->compact_min == min_count) { *use = t->compact; return TK_OK; }
-  drop_compact(t);
-  uint64_t n_ge = 0;
-  int rc = tk_table_dump_size(c, t, min_count, 0xFFFFFFFFu, &n_ge);
-  if (rc) return rc;
-  if ((double)n_ge > 0.5 * (double)t->n_keys) return TK_OK;
-  tk_table* ct = nullptr;
-  if ((rc = tk_table_create(c, t->k, t->canonical, n_ge + 64, 0.5, &ct))) return rc;
-  if ((rc = table_zero(c, ct))) { tk_table_free(ct); return rc; }
-  // every carry of the source may land in the copy's side table
-  if ((rc = table_ensure_ovf(c, ct, std::max<uint64_t>(t->ovf_bound, 1)))) { tk_table_free(ct); return rc; }
-  ct->finalized = false;
-  TK_HIP(hipMemsetAsync(c->d_flags, 0, 16, c->stream));
-  TableView sv = table_view(t), dv = table_view(ct);
-  const uint64_t ns = t->nb * 8;
-  {
-    hipEvent_t a;
-    timing_begin(c, &a);
-    hipLaunchKernelGGL(k_compact, dim3(std::min<uint64_t>(ceil_div(ns, 256), 65536)), dim3(256), 0,
-                       c->stream, sv, ns, (uint64_t)min_count, dv, c->d_flags);
-    timing_end(c, "compact", a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) { tk_table_free(ct); return hip_fail(e, "k_compact"); }
-  }
-  TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 16, hipMemcpyDeviceToHost, c->stream));
-  TK_HIP(hipStreamSynchronize(c->stream));
-  if (c->h_flags[0]) { tk_table_free(ct); return fail(TK_ERR_FULL, "compact copy found no room"); }
-  memcpy(&ct->n_keys, &c->h_flags[2], 8);
-  if ((rc = table_set_final(c, ct, true))) { tk_table_free(ct); return rc; }
-  t->compact = ct;
-  t->compact_min = min_count;
-  *use = ct;
-  return TK_OK;
-}
-
 // LDS count-cache slots of the ordered scoring pass: TK_STATS_CACHE = 0
 // (default), 2048 or 4096.  Off by default: on the 20M-pair workload the
 // cached pass took 116 ms (2048) and 156 ms (4096) against 78 ms without,
@@ -3287,7 +3248,6 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
   if (slab_layout(max_nwin, &slab, &wpb))
     return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16128 + t->k - 1);
   tk_table* use = t;
-  if ((rc = scoring_table(c, t, o->min_count, &use))) return rc;
   StatsParams sp{o->min_count, o->missing_value, o->nwin_policy, o->median_policy};
   if ((rc = reads_order(c, r))) return rc;
   const int grid = (per_read_grid(c, r->n_reads, wpb) + 7) / 8 * 8;
@@ -3334,26 +3294,23 @@ int tk_read_cov_stats(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stats_
 }
 
 // ---------------- selection ----------------
-int tk_select_pairs_d(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint64_t n,
-                      int max_cov, int min_cov, int max_cv, uint32_t max_nwin, uint8_t* dsel,
-                      uint64_t* n_selected) {
-  if (!c || (n && (!dl || !dr || !dsel))) return fail(TK_ERR_ARG, "NULL argument");
-  if (max_cov <= 0) return fail(TK_ERR_ARG, "max_cov must be positive");
-  if ((double)max_cv < 1.2 * std::sqrt((double)max_nwin) + 1.0)
-    return fail(TK_ERR_ARG, "max_cv=%d below the bound 1.2*sqrt(%u)+1 where the CV test is exact",
-                max_cv, max_nwin);
-  if (n > 0xFFFFFFFFull) return fail(TK_ERR_ARG, "too many pairs");
-  if (set_device(c->device)) return TK_ERR_HIP;
-  if (n_selected) *n_selected = 0;
-  if (!n) return TK_OK;
-  void* p = nullptr;
-  int rc = ensure_scratch(c, 1, n * 8 + 256, &p);
-  if (rc) return rc;
-  uint32_t* consumes = (uint32_t*)p;
-  uint32_t* idx = consumes + ((n + 63) / 64) * 64;
+}  // extern "C"
+// The pair selection of nbkc_normalize.pl on device stats: flags of the pairs
+// that draw a rand() value, their ranks (exclusive scan), then the draws.
+// consumes/idx hold n words each; flags (16 words, cleared here by a kernel)
+// get the error bits (word 0) and the selected count (words 8-9); scan_temp
+// (scan_bytes) is the scan's temporary storage, or NULL for the context's.
+namespace tk {
+int select_run(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint64_t n, int max_cov,
+               int min_cov, uint8_t* dsel, uint32_t* consumes, uint32_t* idx, void* scan_temp,
+               size_t scan_bytes, uint32_t* flags, uint64_t* n_selected) {
+  int rc = TK_OK;
+  TK_LAUNCH(c, "select_flags", 1, 64, 0, k_fill_u32, flags, 16u, 0u);
   int grid = (int)std::min<uint64_t>(ceil_div(n, 256), 65536);
-  TK_LAUNCH(c, "select_flags", grid, 256, 0, k_select_flags, dl, dr, n, min_cov, consumes);
-  if ((rc = exclusive_scan_u32(c, consumes, idx, n))) return rc;
+  TK_LAUNCH(c, "select_flags", grid, 256, 0, k_select_flags, dl, dr, n, min_cov, consumes, flags);
+  if (scan_temp) rc = exclusive_scan_u32_in(c, consumes, idx, n, scan_temp, scan_bytes);
+  else rc = exclusive_scan_u32(c, consumes, idx, n);
+  if (rc) return rc;
   Lcg48Jump j;
   const uint64_t M = (1ull << 48) - 1;
   uint64_t A = 0x5DEECE66Dull, C = 0xB;
@@ -3364,16 +3321,50 @@ int tk_select_pairs_d(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr,
     A = (A * A) & M;
   }
   const uint64_t state0 = ((uint64_t)12345 << 16) | 0x330Eull;
-  unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
-  TK_HIP(hipMemsetAsync(d, 0, 8, c->stream));
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(flags + 8);
   const int sgrid = (int)std::min<uint64_t>(ceil_div(n, 256), (uint64_t)std::max(c->num_cu, 1) * 16);
   TK_LAUNCH(c, "select", sgrid, 256, 0, k_select, dl, dr, n, consumes, idx, j, state0, max_cov,
             dsel, d);
-  uint64_t h = 0;
-  TK_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream));
+  TK_HIP(hipMemcpyAsync(c->h_flags, flags, 40, hipMemcpyDeviceToHost, c->stream));
   TK_HIP(hipStreamSynchronize(c->stream));
+  if (c->h_flags[0] & 1u)
+    return fail(TK_ERR_ARG, "select: a read's mean coverage is below 1 (stats computed with missing_value 0?); "
+                            "the merged-mean test is exact only for means >= 1");
+  if (c->h_flags[0] & 2u)
+    return fail(TK_ERR_ARG, "select: a pair with median 0 reaches the max_cov/median draw "
+                            "(nbkc_normalize.pl divides by zero there)");
+  uint64_t h = 0;
+  memcpy(&h, &c->h_flags[8], 8);  // the count k_select adds at d
   if (n_selected) *n_selected = h;
   return TK_OK;
+}
+
+int select_check(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint64_t n, int max_cov,
+                 int max_cv, uint32_t max_nwin, uint8_t* dsel) {
+  if (!c || (n && (!dl || !dr || !dsel))) return fail(TK_ERR_ARG, "NULL argument");
+  if (max_cov <= 0) return fail(TK_ERR_ARG, "max_cov must be positive");
+  if ((double)max_cv < 1.2 * std::sqrt((double)max_nwin) + 1.0)
+    return fail(TK_ERR_ARG, "max_cv=%d below the bound 1.2*sqrt(%u)+1 where the CV test is exact",
+                max_cv, max_nwin);
+  if (n > 0xFFFFFFFFull) return fail(TK_ERR_ARG, "too many pairs");
+  return TK_OK;
+}
+}  // namespace tk
+
+extern "C" {
+int tk_select_pairs_d(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint64_t n,
+                      int max_cov, int min_cov, int max_cv, uint32_t max_nwin, uint8_t* dsel,
+                      uint64_t* n_selected) {
+  int rc = select_check(c, dl, dr, n, max_cov, max_cv, max_nwin, dsel);
+  if (rc) return rc;
+  if (set_device(c->device)) return TK_ERR_HIP;
+  if (n_selected) *n_selected = 0;
+  if (!n) return TK_OK;
+  void* p = nullptr;
+  if ((rc = ensure_scratch(c, 1, n * 8 + 256, &p))) return rc;
+  uint32_t* consumes = (uint32_t*)p;
+  uint32_t* idx = consumes + ((n + 63) / 64) * 64;
+  return select_run(c, dl, dr, n, max_cov, min_cov, dsel, consumes, idx, nullptr, 0, c->d_flags, n_selected);
 }
 
 int tk_select_pairs(tk_ctx* c, const tk_read_stat* l, const tk_read_stat* r, uint64_t n, int max_cov,

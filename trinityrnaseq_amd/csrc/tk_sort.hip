@@ -71,6 +71,24 @@ static int scan_impl(tk_ctx* ctx, const T* d_in, T* d_out, uint64_t n, const cha
   return TK_OK;
 }
 
+// Exclusive sum into caller-provided temporary storage (allocates nothing).
+int exclusive_scan_u32_in(tk_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* temp,
+                          size_t temp_bytes) {
+  if (n == 0) return TK_OK;
+  if (n > 0x7FFFFFFFull) return fail(TK_ERR_ARG, "scan of %llu items too large", (unsigned long long)n);
+  size_t need = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_in, d_out, (int)n, ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "ExclusiveSum(size)");
+  if (need > temp_bytes)
+    return fail(TK_ERR_ARG, "scan needs %zu bytes of temporary storage, %zu reserved", need, temp_bytes);
+  hipEvent_t a;
+  timing_begin(ctx, &a);
+  e = hipcub::DeviceScan::ExclusiveSum(temp, need, d_in, d_out, (int)n, ctx->stream);
+  timing_end(ctx, "scan_u32", a);
+  if (e != hipSuccess) return hip_fail(e, "ExclusiveSum");
+  return TK_OK;
+}
+
 int exclusive_scan_u64(tk_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n) {
   return scan_impl<uint64_t>(ctx, d_in, d_out, n, "scan_u64");
 }
