@@ -64,3 +64,60 @@ def concat(seqs: list[str]) -> tuple[bytes, np.ndarray]:
     if bs:
         off[1:] = np.cumsum([len(b) for b in bs])
     return b"".join(bs), off
+
+
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+NBKC = GOLDEN / "nbkc"
+NBKC_SETTINGS = [(50, 0, 10000), (50, 1, 10000), (200, 0, 10000), (5, 3, 10000), (50, 0, 1), (30, 10, 2)]
+
+
+def nbkc_aggregates(side: str, dtype) -> tuple[list[str], np.ndarray]:
+    """Accessions and integer stats behind tests/golden/nbkc/<side>.stats.sort."""
+    lines = (NBKC / f"{side}.aggregates.tsv").read_text().splitlines()[1:]
+    accs = []
+    st = np.zeros(len(lines), dtype)
+    for i, ln in enumerate(lines):
+        f = ln.split("\t")
+        accs.append(f[0])
+        st[i] = (int(f[1]), int(f[2]), int(f[3]), int(f[4]), int(f[5]))
+    return accs, st
+
+
+def nbkc_pairs() -> list[list[str]]:
+    """Rows of the reference merge's output (pairs.stats), header dropped."""
+    return [ln.split("\t") for ln in (NBKC / "pairs.stats").read_text().splitlines()[1:]]
+
+
+def nbkc_selected(prefix: str, max_cov: int, min_cov: int, max_cv: int) -> list[str]:
+    return (NBKC / f"{prefix}.C{max_cov}.minC{min_cov}.maxCV{max_cv}.accs").read_text().split()
+
+
+def nbkc_stderr(prefix: str, max_cov: int, min_cov: int, max_cv: int) -> str:
+    return (NBKC / f"{prefix}.C{max_cov}.minC{min_cov}.maxCV{max_cv}.stderr").read_text()
+
+
+def histo_witness(name: str) -> list[tuple[int, int]]:
+    """(label, n_kmers) lines of a reference jellyfish-histo witness file."""
+    out = []
+    for ln in (GOLDEN / "histo" / f"kmer_histo.{name}.txt").read_text().splitlines():
+        a, b = ln.split()
+        out.append((int(a), int(b)))
+    return out
+
+
+def counts_for_histo_witness(name: str, seed: int = 3) -> tuple[np.ndarray, list[tuple[int, int]]]:
+    """A k-mer count multiset whose default histo has exactly the witness's
+    labels: n' = 1 + n % 5 k-mers per label (label 10001 taking counts above
+    10000, up to 2^32 - 1).  Returns the counts and the expected lines."""
+    rng = np.random.default_rng(seed)
+    vals, want = [], []
+    for lab, n in histo_witness(name):
+        m = 1 + n % 5
+        if lab == 10001:
+            vals += [int(x) for x in rng.choice([10001, 10002, 65535, 1 << 20, 2**32 - 1], m)]
+        else:
+            vals += [lab] * m
+        want.append((lab, m))
+    counts = np.array(vals, np.uint32)
+    rng.shuffle(counts)
+    return counts, want

@@ -421,6 +421,26 @@ def test_select_pairs_edge_rules(api, ctx):
         api.select_pairs(ctx, zero[:n], zero[n:], 50, 0, 10000)
 
 
+@pytest.mark.parametrize("max_cov,min_cov", [(50, 0), (50, 1), (200, 0), (5, 3)])
+def test_select_pairs_equals_reference_scripts(api, ctx, max_cov, min_cov):
+    """tk_select_pairs on the integer aggregates behind the committed stats
+    files selects exactly the pairs nbkc_normalize.pl selected (after
+    nbkc_merge_left_right_stats.pl --sorted) on their text."""
+    from data import nbkc_aggregates, nbkc_pairs, nbkc_selected
+    say("select_pairs_equals_reference_scripts", max_cov, min_cov)
+    la, ls = nbkc_aggregates("left", api.READ_STAT_DTYPE)
+    ra, rs = nbkc_aggregates("right", api.READ_STAT_DTYPE)
+    li = {a: i for i, a in enumerate(la)}
+    ri = {a: i for i, a in enumerate(ra)}
+    cores = [p[0] for p in nbkc_pairs()]
+    sl = ls[[li[c + "/1"] for c in cores]]
+    sr = rs[[ri[c + "/2"] for c in cores]]
+    sel, ns = api.select_pairs(ctx, sl, sr, max_cov, min_cov, 10000)
+    want = nbkc_selected("pairs", max_cov, min_cov, 10000)
+    assert [c for c, s in zip(cores, sel) if s] == want
+    assert ns == len(want)
+
+
 @pytest.mark.parametrize("chunk", ["", "1000", "65536"])
 def test_dump_in_key_range_chunks(api, ctx, small_se, chunk, monkeypatch):
     """A dump larger than one sort call takes goes out in key-range chunks,

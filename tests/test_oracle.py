@@ -215,3 +215,99 @@ def test_cpuref_matches_oracle():
         assert np.array_equal(got["median"], want["median"])
         assert np.array_equal(got["nwin"], want["nwin"])
         assert np.array_equal(got["sum"], want["sum"])
+
+
+# ------------------------------------------- golden vectors of the reference scripts
+
+from data import (NBKC, NBKC_SETTINGS, nbkc_aggregates, nbkc_pairs,  # noqa: E402
+                  nbkc_selected)
+
+
+def test_nbkc_fixture_text_is_the_aggregates_formatted():
+    """The stats text the reference scripts read is the oracle's row format
+    of the committed integer aggregates (which the GPU selection reads)."""
+    for side in ("left", "right"):
+        accs, st = nbkc_aggregates(side, O.READ_STAT_DTYPE)
+        rows = (NBKC / f"{side}.stats.sort").read_text().splitlines(keepends=True)[1:]
+        assert rows == [O.format_stats_row(a, s) for a, s in zip(accs, st)]
+        # C-locale order, as `sort -k1,1` leaves the files for --sorted
+        assert [a.encode() for a in accs] == sorted(a.encode() for a in accs)
+
+
+@pytest.mark.parametrize("max_cov,min_cov,max_cv", NBKC_SETTINGS)
+def test_select_matches_reference_scripts(max_cov, min_cov, max_cv):
+    """or_select_pairs on the left/right values of every pair the reference
+    merge kept selects exactly the accessions nbkc_normalize.pl printed."""
+    pairs = nbkc_pairs()
+    f = lambda i: [float(p[i]) for p in pairs]  # noqa: E731
+    sel = O.select_pairs(f(2), f(6), f(3), f(7), f(4), f(8), max_cov, min_cov, max_cv)
+    got = [p[0] for p, s in zip(pairs, sel) if s]
+    assert got == nbkc_selected("pairs", max_cov, min_cov, max_cv)
+
+
+def test_single_end_select_matches_reference_script():
+    """Single-end: nbkc_normalize.pl on one stats file (no merge).  Its
+    median is an integer and max_CV = 10000 leaves only mean > 0, so the pair
+    rule with the read on both sides makes the same decisions."""
+    rows = [ln.split("\t") for ln in (NBKC / "left.stats.sort").read_text().splitlines()[1:]]
+    f = lambda i: [float(r[i]) for r in rows]  # noqa: E731
+    sel = O.select_pairs(f(1), f(1), f(2), f(2), f(3), f(3), 50, 0, 10000)
+    got = [r[0][:-2] for r, s in zip(rows, sel) if s]
+    assert got == nbkc_selected("left", 50, 0, 10000)
+
+
+def test_merge_matches_reference_script():
+    """The pairs the reference merge keeps are the accessions present on both
+    sides; its median/mean/stdev columns are %.1f of the two sides' average."""
+    la, _ = nbkc_aggregates("left", O.READ_STAT_DTYPE)
+    ra, _ = nbkc_aggregates("right", O.READ_STAT_DTYPE)
+    both = sorted(set(a[:-2] for a in la) & set(a[:-2] for a in ra), key=lambda s: (s + "/1").encode())
+    pairs = nbkc_pairs()
+    assert [p[0] for p in pairs] == both
+    for p in pairs:
+        for c, (a, b) in zip((9, 10, 11), ((2, 6), (3, 7), (4, 8))):
+            assert float(p[c]) == O.perl_avg1(float(p[a]), float(p[b]))
+
+
+def test_stats_row_matches_textbook_mean_and_pstdev():
+    """mean = statistics.mean and stdev = statistics.pstdev (population) of
+    explicit per-window values, printed %g; median = upper middle value."""
+    import statistics
+    rng = np.random.default_rng(77)
+    for trial in range(400):
+        n = int(rng.integers(1, 127))
+        v = np.maximum(1, rng.poisson(float(np.exp(rng.normal(2, 2))), n)).astype(np.int64)
+        if trial % 7 == 0:
+            v[:] = int(v[0])  # constant: stdev exactly 0
+        st = np.zeros(1, O.READ_STAT_DTYPE)[0]
+        s = np.sort(v)
+        st["median"], st["nwin"], st["sum"] = int(s[n // 2]), n, int(v.sum())
+        sq = int((v * v).sum())
+        st["sumsq_lo"], st["sumsq_hi"] = sq & ((1 << 64) - 1), sq >> 64
+        f = O.format_stats_row("r", st).rstrip("\n").split("\t")
+        mean = statistics.mean(v.tolist())
+        sd = statistics.pstdev(v.tolist())
+        assert f[1] == str(int(s[n // 2]))
+        assert f[2] == "%g" % mean
+        assert f[3] == "%g" % sd
+        assert f[4] == "%g" % (sd / mean * 100)
+
+
+@pytest.mark.parametrize("name", ["all", "NormMaxKCov50"])
+def test_histo_default_shape_matches_reference_witness(name):
+    """jellyfish histo's default shape, witnessed by the reference's
+    kmer_histo files: labels from 1, the top label 10001 collecting every
+    count above 10000, and (NormMaxKCov50) bins with no k-mer omitted."""
+    from data import counts_for_histo_witness, histo_witness
+    w = histo_witness(name)
+    assert w[0][0] == 1 and all(n > 0 for _, n in w)
+    labels = [a for a, _ in w]
+    if name == "all":  # every bin filled, the last one the >10000 bin
+        assert labels == list(range(1, 10002))
+    else:  # after normalization: empty bins (and the empty top bin) omitted
+        assert labels[-1] < 10001 and len(labels) < labels[-1]
+    counts, want = counts_for_histo_witness(name)
+    bins = O.histo(counts)
+    base = O.histo_base(1, 1)
+    got = [(base + i, int(b)) for i, b in enumerate(bins) if b]
+    assert got == want

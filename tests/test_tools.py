@@ -232,3 +232,45 @@ def test_reads_to_transcripts(tmp_path, strand):
         if v["hits"] and 100 * int(v["hits"]) >= 10 * int(v["nwin"]):
             want.append(f"{int(v['comp'])}\t>{nm}\t{100.0 * int(v['hits']) / int(v['nwin']):g}\t{r.upper()}\n")
     assert (tmp_path / "r2c.out").read_text() == "".join(want)
+
+
+# ------------------------------------------- golden vectors of the reference scripts
+
+from data import NBKC, NBKC_SETTINGS, counts_for_histo_witness, nbkc_selected, nbkc_stderr  # noqa: E402
+
+
+@pytest.mark.parametrize("max_cov,min_cov,max_cv", NBKC_SETTINGS)
+def test_nbkc_normalize_equals_reference_scripts(tmp_path, max_cov, min_cov, max_cv):
+    """The CLI on the committed stats files writes byte for byte what
+    nbkc_merge_left_right_stats.pl --sorted and nbkc_normalize.pl wrote:
+    the merged pairs file, the selected accessions and the three summary
+    lines on stderr."""
+    r = run([BIN / "nbkc_normalize", "--left", NBKC / "left.stats.sort", "--right", NBKC / "right.stats.sort",
+             "--sorted", "--max_cov", max_cov, "--min_cov", min_cov, "--max_CV", max_cv,
+             "--merged_out", tmp_path / "pairs.stats"])
+    assert (tmp_path / "pairs.stats").read_bytes() == (NBKC / "pairs.stats").read_bytes()
+    assert r.stdout.split("\n")[:-1] == nbkc_selected("pairs", max_cov, min_cov, max_cv)
+    assert r.stderr.splitlines()[-3:] == nbkc_stderr("pairs", max_cov, min_cov, max_cv).splitlines()
+    r2 = run([BIN / "nbkc_normalize", "--stats_file", NBKC / "pairs.stats", "--max_cov", max_cov,
+              "--min_cov", min_cov, "--max_CV", max_cv])
+    assert r2.stdout == r.stdout
+    assert r2.stderr == nbkc_stderr("pairs", max_cov, min_cov, max_cv)
+
+
+def test_nbkc_normalize_single_end_equals_reference_script():
+    r = run([BIN / "nbkc_normalize", "--stats_file", NBKC / "left.stats.sort", "--max_cov", 50,
+             "--min_cov", 0, "--max_CV", 10000])
+    assert r.stdout.split("\n")[:-1] == nbkc_selected("left", 50, 0, 10000)
+    assert r.stderr == nbkc_stderr("left", 50, 0, 10000)
+
+
+@pytest.mark.parametrize("name", ["all", "NormMaxKCov50"])
+def test_jellyfish_histo_default_shape(tmp_path, name):
+    """`jellyfish histo` with default bounds prints the shape of the
+    reference's histogram witnesses (labels from 1, empty bins omitted,
+    10001 for counts above 10000)."""
+    counts, want = counts_for_histo_witness(name)
+    keys = np.arange(len(counts), dtype=np.uint64) * 7919
+    write_db(tmp_path / "w.jf", 25, 1, keys, counts)
+    run([BIN / "jellyfish", "histo", "-t", "2", "-o", tmp_path / "w.histo", tmp_path / "w.jf"])
+    assert (tmp_path / "w.histo").read_text() == "".join(f"{a} {b}\n" for a, b in want)
