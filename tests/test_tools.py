@@ -274,3 +274,115 @@ def test_jellyfish_histo_default_shape(tmp_path, name):
     write_db(tmp_path / "w.jf", 25, 1, keys, counts)
     run([BIN / "jellyfish", "histo", "-t", "2", "-o", tmp_path / "w.histo", tmp_path / "w.jf"])
     assert (tmp_path / "w.histo").read_text() == "".join(f"{a} {b}\n" for a, b in want)
+
+
+# ------------------------------------------- read ingest (FASTQ -> FASTA) and extraction
+
+import gzip  # noqa: E402
+import hashlib  # noqa: E402
+
+INGEST = Path(__file__).resolve().parent / "golden" / "ingest"
+SAMPLE = Path(__file__).resolve().parent / "golden" / "sample_data"
+
+
+@pytest.mark.parametrize("args,want", [
+    (["-I", "edge.fq"], "edge.fa"),
+    (["-I", "edge.fq", "-a", "1"], "edge.a1.fa"),
+    (["-I", "edge.fq", "--rev"], "edge.rev.fa"),
+    (["-I", "edge.fq, edge.fq"], "edge.twice.fa"),
+    (["-I", "edge_dirty.fq", "--ignore_dirty"], "edge_dirty.ignore.fa"),
+])
+def test_fastq_to_fasta_equals_reference_script(args, want):
+    """fastQ_to_fastA writes byte for byte what fastQ_to_fastA.pl wrote on
+    the committed edge-case FASTQ (Casava fields, /1 /2 suffixes, \\r\\n,
+    IUPAC reverse complement, dirty records skipped)."""
+    r = run([BIN / "fastQ_to_fastA", *args], cwd=INGEST)
+    assert r.stdout.encode() == (INGEST / want).read_bytes()
+
+
+def test_fastq_to_fasta_dirty_record_is_fatal():
+    r = run([BIN / "fastQ_to_fastA", "-I", "edge_dirty.fq"], cwd=INGEST, check=False)
+    assert r.returncode == int((INGEST / "edge_dirty.fatal.rc").read_text())
+    assert r.stdout.encode() == (INGEST / "edge_dirty.fatal.fa").read_bytes()
+    assert "improperly formatted entry" in r.stderr
+
+
+def test_sample_reads_to_fasta_equal_the_shipped_fasta(tmp_path):
+    """The reference's sample reads: the FASTA written from reads.*.fq.gz is
+    the reads.*.fa.gz shipped beside them, by both tool names; 30,575 records
+    per file and 61,150 in both.fa (test_prep.py:26-100); --rev and -a 2
+    outputs hash to what fastQ_to_fastA.pl wrote."""
+    both = b""
+    for side, rt in (("left", "1"), ("right", "2")):
+        want = gzip.decompress((SAMPLE / f"reads.{side}.fa.gz").read_bytes())
+        r1 = subprocess.run([BIN / "fastQ_to_fastA", "-I", SAMPLE / f"reads.{side}.fq.gz"], stdout=subprocess.PIPE,
+                            check=True)
+        assert r1.stdout == want
+        r2 = subprocess.run([BIN / "seqtk-trinity", "seq", "-A", "-R", rt, SAMPLE / f"reads.{side}.fq.gz"],
+                            stdout=subprocess.PIPE, check=True)
+        assert r2.stdout == want
+        assert want.count(b">") == 30575
+        both += want
+    assert both.count(b">") == 61150
+    sums = {}
+    for ln in (INGEST / "sample_sha256.txt").read_text().splitlines():
+        h, rest = ln.split("  ", 1)
+        f, *args = rest.split()
+        sums[f] = (h, args)
+    for f, (h, args) in sums.items():
+        r = subprocess.run([BIN / "fastQ_to_fastA", "-I", SAMPLE / f, *args], stdout=subprocess.PIPE, check=True)
+        assert hashlib.sha256(r.stdout).hexdigest() == h, f
+
+
+def test_seqtk_trinity_read_type_rule(tmp_path):
+    """seqtk-trinity -R N: names carrying /1, /2 or a Casava field keep that
+    read type, others get /N; -r reverse-complements (IUPAC table)."""
+    fq = tmp_path / "x.fq"
+    fq.write_text("@a/1\nACGTN\n+\nIIIII\n@b 2:N:0\nAACCG\n+\nIIIII\n@c\nRYKMB\n+\nIIIII\n")
+    r = run([BIN / "seqtk-trinity", "seq", "-A", "-R", "1", fq])
+    assert r.stdout == ">a/1\nACGTN\n>b/2\nAACCG\n>c/1\nRYKMB\n"
+    r = run([BIN / "seqtk-trinity", "seq", "-A", "-r", "-R", "2", fq])
+    assert r.stdout == ">a/1\nNACGT\n>b/2\nCGGTT\n>c/2\nVKMRY\n"
+
+
+def _fq_records(data: bytes):
+    lines = data.split(b"\n")
+    return [b"\n".join(lines[i:i + 4]) + b"\n" for i in range(0, len(lines) - 1, 4)]
+
+
+def test_extract_normalized_reads_fastq(tmp_path):
+    """make_normalized_reads_file on FASTQ: every record whose core name
+    (no '@', no /1 /2, no _forward/_reverse) is selected, as its 4 lines, in
+    file order; selected names may carry /1 or /2; a name no record has is
+    an error listed in <out>.missing_accs."""
+    left = gzip.decompress((SAMPLE / "reads.left.fq.gz").read_bytes())
+    recs = _fq_records(left)
+    names = [r.split(b"\n")[0][1:].split()[0].decode()[:-2] for r in recs]
+    rng = np.random.default_rng(4)
+    pick = sorted(set(rng.choice(len(names), 3000, replace=False).tolist()))
+    sel = tmp_path / "sel.accs"
+    sel.write_text("".join(names[i] + ("/1\n" if i % 5 == 0 else "\n") for i in reversed(pick)) + "\n  \n")
+    out = tmp_path / "left.norm.fq"
+    run([BIN / "extract_normalized_reads", "--seqType", "fq", "--selected", sel, "--output", out,
+         SAMPLE / "reads.left.fq.gz"])
+    assert out.read_bytes() == b"".join(recs[i] for i in pick)
+    # both sources in order: left then right records of the same pairs
+    right = _fq_records(gzip.decompress((SAMPLE / "reads.right.fq.gz").read_bytes()))
+    run([BIN / "extract_normalized_reads", "--seqType", "fq", "--selected", sel, "--output", out,
+         SAMPLE / "reads.left.fq.gz", SAMPLE / "reads.right.fq.gz"])
+    assert out.read_bytes() == b"".join(recs[i] for i in pick) + b"".join(right[i] for i in pick)
+    sel.write_text(names[0] + "\nnot_a_read\nalso_missing/2\n")
+    r = run([BIN / "extract_normalized_reads", "--seqType", "fq", "--selected", sel, "--output", out,
+             SAMPLE / "reads.left.fq.gz"], check=False)
+    assert r.returncode != 0 and "missing 2" in r.stderr
+    assert (tmp_path / "left.norm.fq.missing_accs").read_text() == "also_missing\nnot_a_read\n"
+
+
+def test_extract_normalized_reads_fasta(tmp_path):
+    fa = tmp_path / "x.fa"
+    fa.write_bytes(b"\n\n>a/1 first read\nACGT\nAC GT\n>b/2\t\nGGGG\n>c\nTT\x01TT\n>d/1\n\n")
+    sel = tmp_path / "s"
+    sel.write_text("a\nc\nd\n")
+    out = tmp_path / "o.fa"
+    run([BIN / "extract_normalized_reads", "--seqType", "fa", "--selected", sel, "--output", out, fa])
+    assert out.read_bytes() == b">a/1 first read\nACGTACGT\n>c\nTTTT\n>d/1\n\n"

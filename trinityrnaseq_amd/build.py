@@ -41,6 +41,9 @@ TOOLS = {
     "inchworm": ["tools/inchworm_main.cpp"],
     "ReadsToTranscripts": ["tools/ReadsToTranscripts_main.cpp"],
     "nbkc_normalize": ["tools/nbkc_normalize_main.cpp"],
+    "fastQ_to_fastA": ["tools/fastq_to_fasta_main.cpp"],
+    "seqtk-trinity": ["tools/fastq_to_fasta_main.cpp"],
+    "extract_normalized_reads": ["tools/extract_normalized_reads_main.cpp"],
 }
 TOOL_COMMON = ["tools/seq_io.cpp", "tools/tool_util.cpp"]
 
@@ -120,7 +123,7 @@ def build_tools(force: bool = True) -> list[Path]:
         deps = paths + list((CSRC / "tools").glob("*.h")) + [ROOT / "include" / "trinity_gpu.h"]
         if force or not _newer(out, deps):
             jobs.append([CXX, *CXX_FLAGS, *map(str, paths), "-o", str(out),
-                         f"-L{LIB}", "-ltrinity_gpu", "-Wl,-rpath,$ORIGIN/../lib"])
+                         f"-L{LIB}", "-ltrinity_gpu", "-lz", "-Wl,-rpath,$ORIGIN/../lib"])
     with ThreadPoolExecutor(max_workers=max(1, min(8, len(jobs)))) as ex:
         list(ex.map(_run, jobs))
     return outs

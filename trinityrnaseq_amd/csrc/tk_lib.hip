@@ -1651,11 +1651,13 @@ __global__ void k_dump(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi, 
 
 constexpr int kHistoLds = 4096;
 
-// Keys with count in [lo, hi] per key range: bin = key >> shift (< kHistoLds
-// bins), so a dump of more keys than one sort call takes can be cut into
-// key ranges that are sorted and downloaded one after another.
+// Keys with count in [lo, hi] and key in [klo, khi] per key range:
+// bin = (key - klo) >> shift (< kHistoLds bins), so that a dump of more keys
+// than one sort call takes can be cut into key ranges that are sorted and
+// downloaded one after another.
 __global__ __launch_bounds__(256) void k_dump_bins(TableView t, uint64_t n_slots, uint64_t lo, uint64_t hi,
-                                                   uint32_t shift, unsigned long long* __restrict__ bins) {
+                                                   uint64_t klo, uint64_t khi, uint32_t shift,
+                                                   unsigned long long* __restrict__ bins) {
   __shared__ uint32_t s_bins[kHistoLds];
   for (int i = threadIdx.x; i < kHistoLds; i += blockDim.x) s_bins[i] = 0;
   __syncthreads();
@@ -1665,7 +1667,8 @@ __global__ __launch_bounds__(256) void k_dump_bins(TableView t, uint64_t n_slots
     if (!s || (s >> t.cshift) == 0) continue;
     uint64_t v = slot_count(t, s);
     v = v > 0xFFFFFFFFull ? 0xFFFFFFFFull : v;
-    if (v >= lo && v <= hi) atomicAdd(&s_bins[(s & (t.occ - 1)) >> shift], 1u);
+    const uint64_t key = s & (t.occ - 1);
+    if (v >= lo && v <= hi && key >= klo && key <= khi) atomicAdd(&s_bins[(key - klo) >> shift], 1u);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kHistoLds; i += blockDim.x)
@@ -3075,58 +3078,81 @@ int tk_table_dump(tk_ctx* c, tk_table* t, uint32_t lo, uint32_t hi, uint64_t* ke
   if (!n) return TK_OK;
   const uint64_t limit = dump_chunk_limit();
   const uint64_t kmax = t->k >= 32 ? ~0ull : (1ull << (2 * t->k)) - 1;
-  // key ranges: one, or groups of consecutive bins of the key's top bits
-  std::vector<std::pair<uint64_t, uint64_t>> ranges;  // (first bin, entries), bins of `shift` low bits
-  uint32_t shift = 0;
-  std::vector<uint64_t> bins;
-  if (n <= limit) {
-    ranges.push_back({0, n});
-  } else {
-    const int kbits = 2 * t->k;
-    const int bbits = std::min(kbits, 12);  // kHistoLds bins
-    shift = (uint32_t)(kbits - bbits);
-    const uint64_t nbins = 1ull << bbits;
-    void* p = nullptr;
-    if ((rc = ensure_scratch(c, 1, nbins * 8, &p))) return rc;
-    TK_HIP(hipMemsetAsync(p, 0, nbins * 8, c->stream));
+  // Key ranges [klo, khi] of at most `limit` entries, ascending: a range over
+  // the limit is binned by up to 12 more key bits and its bins grouped; a
+  // bin still over the limit is binned again (distinct keys: a one-key bin
+  // always fits).
+  struct Range {
+    uint64_t klo, khi, n;
+  };
+  std::vector<Range> ranges;
+  std::vector<Range> todo{{0, kmax, n}};  // stack, top = next in key order
+  void* pbins = nullptr;
+  const uint64_t nbins_max = kHistoLds;
+  if (n > limit && (rc = ensure_scratch(c, 1, nbins_max * 8, &pbins))) return rc;
+  std::vector<uint64_t> bins(nbins_max);
+  while (!todo.empty()) {
+    const Range r = todo.back();
+    todo.pop_back();
+    if (r.n <= limit) {
+      ranges.push_back(r);
+      continue;
+    }
+    const uint64_t span = r.khi - r.klo;  // range holds span + 1 keys
+    int bits = 64 - __builtin_clzll(span | 1);
+    const uint32_t shift = (uint32_t)std::max(0, bits - 12);
+    const uint64_t nb = (span >> shift) + 1;
+    TK_HIP(hipMemsetAsync(pbins, 0, nb * 8, c->stream));
     TableView v = table_view(t);
     const uint64_t ns = t->nb * 8;
     TK_LAUNCH(c, "dump_bins", std::min<uint64_t>(ceil_div(ns, 256), 65536), 256, 0, k_dump_bins, v, ns,
-              (uint64_t)lo, (uint64_t)hi, shift, (unsigned long long*)p);
-    bins.resize(nbins);
-    TK_HIP(hipMemcpyAsync(bins.data(), p, nbins * 8, hipMemcpyDeviceToHost, c->stream));
+              (uint64_t)lo, (uint64_t)hi, r.klo, r.khi, shift, (unsigned long long*)pbins);
+    TK_HIP(hipMemcpyAsync(bins.data(), pbins, nb * 8, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
+    std::vector<Range> parts;  // this range's pieces in key order
     uint64_t first = 0, acc = 0, total = 0;
-    for (uint64_t b = 0; b < nbins; b++) {
-      if (bins[b] > limit)
-        return fail(TK_ERR_ARG, "dump: %llu keys share the top %d key bits (chunk limit %llu)",
-                    (unsigned long long)bins[b], bbits, (unsigned long long)limit);
+    auto bin_lo = [&](uint64_t b) { return r.klo + (b << shift); };
+    auto bin_hi = [&](uint64_t b) { return b + 1 == nb ? r.khi : r.klo + ((b + 1) << shift) - 1; };
+    for (uint64_t b = 0; b < nb; b++) {
+      total += bins[b];
+      if (bins[b] > limit) {
+        if (acc) parts.push_back({bin_lo(first), bin_hi(b - 1), acc});
+        parts.push_back({bin_lo(b), bin_hi(b), bins[b]});  // binned again
+        first = b + 1;
+        acc = 0;
+        continue;
+      }
       if (acc && acc + bins[b] > limit) {
-        ranges.push_back({first, acc});
+        parts.push_back({bin_lo(first), bin_hi(b - 1), acc});
         first = b;
         acc = 0;
       }
+      if (!acc) first = b;
       acc += bins[b];
-      total += bins[b];
     }
-    ranges.push_back({first, acc});
-    if (total != n) return fail(TK_ERR_STATE, "dump: binned %llu of %llu entries", (unsigned long long)total,
-                                (unsigned long long)n);
+    if (acc) parts.push_back({bin_lo(first), bin_hi(nb - 1), acc});
+    if (total != r.n)
+      return fail(TK_ERR_STATE, "dump: binned %llu of %llu entries", (unsigned long long)total,
+                  (unsigned long long)r.n);
+    if (shift == 0)
+      for (const Range& q : parts)
+        if (q.n > limit) return fail(TK_ERR_STATE, "dump: %llu entries share one key", (unsigned long long)q.n);
+    for (auto it = parts.rbegin(); it != parts.rend(); ++it) todo.push_back(*it);
   }
   uint64_t most = 0;
-  for (auto& r : ranges) most = std::max(most, r.second);
+  for (auto& r : ranges) most = std::max(most, r.n);
   uint64_t* dk = nullptr;
   uint32_t* dc = nullptr;
   if ((rc = dev_alloc((void**)&dk, most * 8))) return rc;
   if ((rc = dev_alloc((void**)&dc, most * 4))) { dev_free(dk, most * 8); return rc; }
   uint64_t at = 0;
   for (size_t i = 0; i < ranges.size() && !rc; i++) {
-    if (!ranges[i].second) continue;
-    const uint64_t klo = ranges.size() == 1 ? 0 : ranges[i].first << shift;
-    const uint64_t khi = (ranges.size() == 1 || i + 1 == ranges.size()) ? kmax : (ranges[i + 1].first << shift) - 1;
-    rc = dump_range(c, t, lo, hi, klo, khi, ranges[i].second, dk, dc, keys + at, counts + at);
-    at += ranges[i].second;
+    if (!ranges[i].n) continue;
+    rc = dump_range(c, t, lo, hi, ranges[i].klo, ranges[i].khi, ranges[i].n, dk, dc, keys + at, counts + at);
+    at += ranges[i].n;
   }
+  if (!rc && at != n)
+    rc = fail(TK_ERR_STATE, "dump: %llu of %llu entries in ranges", (unsigned long long)at, (unsigned long long)n);
   (void)hipStreamSynchronize(c->stream);
   dev_free(dk, most * 8);
   dev_free(dc, most * 4);

@@ -10,11 +10,43 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <zlib.h>
+
 #include <stdexcept>
 
 namespace tools {
 
-SeqReader::SeqReader(const std::string& path, bool keep_names) : keep_names_(keep_names) {
+static bool ends_with(const std::string& s, const char* suf) {
+  const size_t n = strlen(suf);
+  return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
+}
+
+TextSource::TextSource(const std::string& path) {
+  if (path == "-") {
+    char buf[1 << 16];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, stdin)) > 0) owned_.append(buf, n);
+    data_ = owned_.data();
+    size_ = owned_.size();
+    return;
+  }
+  if (ends_with(path, ".gz")) {
+    gzFile g = gzopen(path.c_str(), "rb");
+    if (!g) throw std::runtime_error("cannot open " + path);
+    gzbuffer(g, 1 << 20);
+    std::vector<char> buf(1 << 20);
+    int n;
+    while ((n = gzread(g, buf.data(), (unsigned)buf.size())) > 0) owned_.append(buf.data(), (size_t)n);
+    int err = 0;
+    const char* msg = gzerror(g, &err);
+    const bool bad = n < 0 || (err != Z_OK && err != Z_STREAM_END);
+    const std::string why = bad ? std::string(msg ? msg : "") : std::string();
+    gzclose(g);
+    if (bad) throw std::runtime_error(path + ": gzip error: " + why);
+    data_ = owned_.data();
+    size_ = owned_.size();
+    return;
+  }
   fd_ = open(path.c_str(), O_RDONLY);
   if (fd_ < 0) throw std::runtime_error("cannot open " + path);
   struct stat st;
@@ -25,7 +57,17 @@ SeqReader::SeqReader(const std::string& path, bool keep_names) : keep_names_(kee
     if (p == MAP_FAILED) throw std::runtime_error("cannot mmap " + path);
     madvise(p, size_, MADV_SEQUENTIAL);
     data_ = (const char*)p;
+    mapped_ = true;
   }
+}
+
+TextSource::~TextSource() {
+  if (mapped_) munmap((void*)data_, size_);
+  if (fd_ >= 0) close(fd_);
+}
+
+SeqReader::SeqReader(const std::string& path, bool keep_names)
+    : src_(path), data_(src_.data()), size_(src_.size()), keep_names_(keep_names) {
   // skip leading whitespace (Fasta_reader.pm:55-59)
   while (pos_ < size_ && (data_[pos_] == '\n' || data_[pos_] == '\r' || data_[pos_] == ' ' ||
                           data_[pos_] == '\t'))
@@ -37,10 +79,7 @@ SeqReader::SeqReader(const std::string& path, bool keep_names) : keep_names_(kee
   }
 }
 
-SeqReader::~SeqReader() {
-  if (data_) munmap((void*)data_, size_);
-  if (fd_ >= 0) close(fd_);
-}
+SeqReader::~SeqReader() {}
 
 static inline bool keep_seq_char(unsigned char c) {
   // drop whitespace and control characters (\000-\037, \177-\377)

@@ -5,6 +5,8 @@
 // whitespace-delimited token (Sequence::new, :110), the sequence is the
 // remaining lines joined with all whitespace and control characters removed.
 // FASTQ (4-line records, '@' header) is accepted as jellyfish does.
+// Every reader takes a plain file (memory-mapped), a gzip file (name ending
+// in .gz, inflated into memory) or "-" for standard input.
 #pragma once
 #include <stdint.h>
 
@@ -30,7 +32,26 @@ struct SeqSet {
   }
 };
 
-// Streaming reader over one file (mmap).  next_batch appends up to max_records
+// The bytes of one input file: mapped for plain files, inflated for .gz,
+// read whole for "-" (stdin).
+class TextSource {
+ public:
+  explicit TextSource(const std::string& path);
+  ~TextSource();
+  TextSource(const TextSource&) = delete;
+  TextSource& operator=(const TextSource&) = delete;
+  const char* data() const { return data_; }
+  uint64_t size() const { return size_; }
+
+ private:
+  const char* data_ = nullptr;
+  uint64_t size_ = 0;
+  bool mapped_ = false;
+  std::string owned_;
+  int fd_ = -1;
+};
+
+// Streaming reader over one file.  next_batch appends up to max_records
 // records (and stops once max_bases is exceeded); returns false at EOF with
 // nothing appended.
 class SeqReader {
@@ -41,12 +62,12 @@ class SeqReader {
   uint64_t file_size() const { return size_; }
 
  private:
+  TextSource src_;
   const char* data_ = nullptr;
   uint64_t size_ = 0;
   uint64_t pos_ = 0;
   bool fastq_ = false;
   bool keep_names_ = true;
-  int fd_ = -1;
 };
 
 // Read every record of a file.
