@@ -16,6 +16,8 @@
  *   jellyfish histo db       (Trinity:2630, :636) -> tk_table_histo
  *   inchworm --kmers dump --DS --min_any_entropy E [--minKmerCount m]
  *        Trinity:2654,2669-2711,1050             -> tk_table_load, tk_table_prune
+ *   inchworm --run_inchworm -L L (greedy contigs, Trinity:2654-2711)
+ *                                                -> tk_inchworm, tk_contigs_*
  *   fastaToKmerCoverageStats --reads --kmers --kmer_size --DS
  *        insilico_read_normalization.pl:837-845  -> tk_read_cov_stats
  *   nbkc_merge_left_right_stats.pl + nbkc_normalize.pl
@@ -23,6 +25,9 @@
  *                                                -> tk_select_pairs
  *   ReadsToTranscripts -i reads -f bundles -p P [-strand]
  *        Trinity:2238-2250                       -> tk_ctable_build, tk_r2t_vote
+ *   GraphFromFasta -k 24 -kk 48 (weld support, Trinity:2180)
+ *   QuantifyGraph -k 24 (per-component node weights, Trinity:2352)
+ *                                                -> tk_kset_create, tk_kset_count
  *
  * Conventions
  *   - Every function returns TK_OK (0) or a negative tk_status; the message
@@ -178,6 +183,53 @@ int tk_table_prune(tk_ctx* ctx, tk_table* t, uint32_t min_count, const uint8_t* 
 int tk_table_lookup(tk_ctx* ctx, tk_table* t, const uint64_t* keys, uint64_t n,
                     uint32_t* counts);
 int tk_table_free(tk_table* t);
+
+/* ---------------- counts of a fixed k-mer set ---------------- */
+/*
+ * Occurrences in the reads of a given set of keys, each 128 bits as two
+ * words (lo, hi), hi < 2^62, all distinct (TK_ERR_ARG otherwise):
+ *  - plain kk-mers, kk <= 64 (tags NULL): the key is the 2kk-bit k-mer
+ *    value, hi = its bits 64..127; GraphFromFasta's weld k-mers
+ *    (-k 24 -kk 48, Trinity:2180) counted in the reads;
+ *  - tagged k-mers, kk <= 32 (tags = one u32 per read): the key of a read's
+ *    window is (hi = tags[read], lo = k-mer); QuantifyGraph's node (k = 24)
+ *    and edge (k = 25) weights of each component from the reads
+ *    ReadsToTranscripts gave it (Trinity:2352).
+ * A window counts when all its bases are ACGT; canonical = min(k-mer,
+ * reverse complement) in numeric order.  counts[i] = windows equal to key i.
+ */
+typedef struct tk_kset tk_kset;
+int tk_kset_create(tk_ctx* ctx, const uint64_t* keys, uint64_t n_keys, tk_kset** out);
+int tk_kset_count(tk_ctx* ctx, tk_kset* s, const tk_reads* reads, int kk, int canonical,
+                  const uint32_t* tags, uint64_t* counts);
+int tk_kset_free(tk_kset* s);
+
+/* ---------------- Inchworm contigs ---------------- */
+/*
+ * Greedy contig construction over a (pruned) table, the assembly step of
+ * `inchworm --kmers <dump> --run_inchworm -K k -L L [--DS]`
+ * (Trinity:2654-2711; contigs read as ">a<N>;<cov>" by
+ * filter_iworm_by_min_length_or_cov.pl:22-26).  Seeds: keys with count >=
+ * min_seed_cov whose composition seed_keep accepts ((k+1)^3 bytes as in
+ * tk_table_prune; NULL accepts all), in order of count descending, then key
+ * ascending; a seed used by an earlier contig is skipped.  Each contig
+ * starts as its seed's forward string, extends to the right by the unused
+ * candidate k-mer (last k-1 bases + b) of largest count, ties to the
+ * smallest base, until none is left, then to the left the same way
+ * (b + first k-1 bases); lookups and use are by canonical form on a
+ * canonical table.  Contigs come in seed order, each with the sum of its
+ * k-mers' counts and its seed key; the table is not modified.  Tables of
+ * 2^32 or more slots are refused.
+ */
+typedef struct tk_contigs tk_contigs;
+int tk_inchworm(tk_ctx* ctx, tk_table* t, uint32_t min_seed_cov, const uint8_t* seed_keep,
+                tk_contigs** out);
+/* Number of contigs, total bases, and the rounds / walks the device ran. */
+int tk_contigs_info(const tk_contigs* c, uint64_t* n_contigs, uint64_t* total_len, uint64_t* rounds,
+                    uint64_t* walks);
+/* seq: total_len bytes (no separators); offsets: n+1; sums, seed_keys: n. */
+int tk_contigs_get(const tk_contigs* c, char* seq, uint64_t* offsets, uint64_t* sums, uint64_t* seed_keys);
+int tk_contigs_free(tk_contigs* c);
 
 /* ---------------- per-read k-mer coverage (fastaToKmerCoverageStats) ---------------- */
 enum { TK_MEDIAN_UPPER = 0, TK_MEDIAN_LOWER = 1, TK_MEDIAN_MEAN_FLOOR = 2 };

@@ -78,10 +78,13 @@ def revcomp(v: int, k: int) -> int:
     return lib().or_revcomp(v, k)
 
 
-def window_totals(seq: bytes, off, k: int) -> tuple[int, int]:
+def window_totals(seq, off, k: int) -> tuple[int, int]:
+    """(all windows, windows free of non-ACGT bytes); seq is bytes or a uint8 array."""
     off = _off(off)
     t = np.zeros(1, np.uint64)
     v = np.zeros(1, np.uint64)
+    if isinstance(seq, np.ndarray):
+        seq = C.cast(C.c_void_p(seq.ctypes.data), C.c_char_p)
     lib().or_window_totals(seq, _p(off), len(off) - 1, k, _p(t), _p(v))
     return int(t[0]), int(v[0])
 
@@ -205,16 +208,17 @@ def cpuref() -> C.CDLL:
         l.cr_count_stats.restype = C.c_int
         l.cr_count_stats.argtypes = [C.c_char_p, _P, C.c_uint64, C.c_int, C.c_int, C.c_uint32, C.c_int,
                                      C.c_uint64, _P, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                     C.POINTER(C.c_uint64)]
+                                     C.POINTER(C.c_uint64), _P, _P]
         _cpuref = l
     return _cpuref
 
 
 def cpuref_count_stats(seq, off, k: int = 25, canonical: bool = True, min_count: int = 2,
-                       threads: int = 1, expected_distinct: int = 0) -> dict:
+                       threads: int = 1, expected_distinct: int = 0, summary: bool = False) -> dict:
     """Count, then per-read (median, nwin, sum) with missing = 1, N-windows
     counted as missing and the upper median; also the wall-clock seconds of
-    the two passes and the distinct-key count."""
+    the two passes and the distinct-key count.  With `summary`, also the
+    table digest (table_digest) and the default-bin histogram."""
     off = _off(off)
     n = len(off) - 1
     med = np.zeros(max(n, 1), np.uint32)
@@ -226,9 +230,72 @@ def cpuref_count_stats(seq, off, k: int = 25, canonical: bool = True, min_count:
     else:
         sp = seq
     exp = expected_distinct or int(off[-1]) + 1024
+    dg = np.zeros(1, np.uint64)
+    hist = np.zeros(10001, np.uint64)
     rc = cpuref().cr_count_stats(sp, _p(off), n, k, int(canonical), min_count, threads, exp, _p(med),
-                                 _p(nwin), _p(sm), C.byref(tc), C.byref(ts), C.byref(nd))
+                                 _p(nwin), _p(sm), C.byref(tc), C.byref(ts), C.byref(nd),
+                                 _p(dg) if summary else None, _p(hist) if summary else None)
     if rc:
         raise MemoryError("cr_count_stats: out of memory")
-    return {"median": med[:n], "nwin": nwin[:n], "sum": sm[:n], "count_s": tc.value,
-            "stats_s": ts.value, "distinct": nd.value}
+    out = {"median": med[:n], "nwin": nwin[:n], "sum": sm[:n], "count_s": tc.value,
+           "stats_s": ts.value, "distinct": nd.value}
+    if summary:
+        out["digest"] = int(dg[0])
+        out["histo"] = hist
+    return out
+
+
+def table_digest(keys, counts) -> int:
+    """Order-independent digest of a (key, count) table: sum mod 2^64 of
+    mix64(key ^ count * 0x9E3779B97F4A7C15), as cr_count_stats computes it."""
+    with np.errstate(over="ignore"):
+        x = np.asarray(keys, np.uint64) ^ (np.asarray(counts, np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        x ^= x >> np.uint64(33)
+        x *= np.uint64(0xFF51AFD7ED558CCD)
+        x ^= x >> np.uint64(33)
+        x *= np.uint64(0xC4CEB9FE1A85EC53)
+        x ^= x >> np.uint64(33)
+        return int(x.sum(dtype=np.uint64))
+
+
+def inchworm(keys, counts, k: int = 25, ds: bool = True, min_seed_cov: int = 2,
+             seed_keep: np.ndarray | None = None):
+    """or_inchworm: contigs (list of str), per-contig count sums and seed keys, in seed order."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    n = len(keys)
+    cap = n * (k + 1) + 64
+    seq = C.create_string_buffer(cap)
+    off = np.zeros(n + 2, np.uint64)
+    sums = np.zeros(n + 1, np.uint64)
+    seeds = np.zeros(n + 1, np.uint64)
+    nc = C.c_uint64()
+    l = lib()
+    l.or_inchworm.restype = C.c_int64
+    l.or_inchworm.argtypes = [_P, _P, C.c_uint64, C.c_int, C.c_int, C.c_uint32, _P, C.c_char_p, C.c_uint64,
+                              _P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]
+    keep = None if seed_keep is None else np.ascontiguousarray(seed_keep, np.uint8)
+    rc = l.or_inchworm(_p(keys) if n else None, _p(counts) if n else None, n, k, int(ds), min_seed_cov,
+                       _p(keep) if keep is not None else None, seq, cap, _p(off), _p(sums), _p(seeds), n + 1,
+                       C.byref(nc))
+    assert rc >= 0, rc
+    m = nc.value
+    raw = seq.raw
+    contigs = [raw[int(off[i]):int(off[i + 1])].decode() for i in range(m)]
+    return contigs, sums[:m].copy(), seeds[:m].copy()
+
+
+def kset_count(keys, seq: bytes, off, kk: int, canonical: bool = True, tags=None) -> np.ndarray:
+    """or_kset_count: occurrences of each (lo, hi) key among the reads' windows."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, 2)
+    off = _off(off)
+    n = len(keys)
+    out = np.zeros(max(n, 1), np.uint64)
+    l = lib()
+    l.or_kset_count.restype = C.c_int
+    l.or_kset_count.argtypes = [_P, C.c_uint64, C.c_char_p, _P, C.c_uint64, C.c_int, C.c_int, _P, _P]
+    t = None if tags is None else np.ascontiguousarray(tags, dtype=np.uint32)
+    rc = l.or_kset_count(_p(keys) if n else None, n, seq, _p(off), len(off) - 1, kk, int(canonical),
+                         _p(t) if t is not None else None, _p(out))
+    assert rc == 0, rc
+    return out[:n]

@@ -104,12 +104,17 @@ static uint32_t cr_get(const cr_table* t, uint64_t key) {
  * Count the k-mers of reads [0, n) (offsets off[n+1] into seq) with `threads`
  * OpenMP threads, then the per-read median/nwin/sum of window values
  * (count if >= min_count, else 1).  Writes wall-clock seconds of each pass
- * and the number of distinct keys.  Returns 0, or -1 when memory runs out.
+ * and the number of distinct keys; with `digest` set, also an
+ * order-independent digest of the (key, count) table, the sum mod 2^64 of
+ * mix64(key ^ count * 0x9E3779B97F4A7C15) over every key (a table check that
+ * needs no sorted dump), and with `histo` set the counts' histogram in
+ * jellyfish histo's default bins (10,001 bins: counts 1..10000, then > 10000).
+ * Returns 0, or -1 when memory runs out.
  */
 extern "C" int cr_count_stats(const char* seq, const uint64_t* off, uint64_t n, int k, int canonical,
                    uint32_t min_count, int threads, uint64_t expected_distinct, uint32_t* median,
                    uint32_t* nwin, uint64_t* sum, double* t_count, double* t_stats,
-                   uint64_t* n_distinct) {
+                   uint64_t* n_distinct, uint64_t* digest, uint64_t* histo) {
     uint64_t cap = 1024;
     while (cap < 2 * expected_distinct) cap <<= 1;
     cr_table t;
@@ -134,6 +139,27 @@ extern "C" int cr_count_stats(const char* seq, const uint64_t* off, uint64_t n, 
 #pragma omp parallel for reduction(+ : nd)
     for (uint64_t i = 0; i < cap; i++) nd += t.keys[i] != 0;
     double c = omp_get_wtime();
+    if (digest) {
+        uint64_t dg = 0;
+#pragma omp parallel for reduction(+ : dg)
+        for (uint64_t i = 0; i < cap; i++)
+            if (t.keys[i]) dg += cr_mix64((t.keys[i] - 1) ^ ((uint64_t)t.counts[i] * 0x9E3779B97F4A7C15ULL));
+        *digest = dg;
+    }
+    if (histo) {
+        memset(histo, 0, 10001 * sizeof(uint64_t));
+#pragma omp parallel
+        {
+            uint64_t* h = (uint64_t*)calloc(10001, sizeof(uint64_t));
+#pragma omp for
+            for (uint64_t i = 0; i < cap; i++)
+                if (t.keys[i]) h[t.counts[i] > 10000 ? 10000 : t.counts[i] - 1]++;
+#pragma omp critical
+            for (int b = 0; b < 10001; b++) histo[b] += h[b];
+            free(h);
+        }
+    }
+    c = omp_get_wtime();
 #pragma omp parallel
     {
         uint32_t* vals = NULL;

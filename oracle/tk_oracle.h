@@ -143,6 +143,42 @@ void or_r2t_vote_reads(const char* seq, const uint64_t* off, uint64_t n_reads,
 /* pct >= min_pct test done exactly in integers: 100*hits >= min_pct*nwin. */
 int or_r2t_passes(const or_r2t_vote* v, uint32_t min_pct);
 
+/* ---- Inchworm greedy contig construction (tk_oracle_iworm.c) ---- */
+/*
+ * `inchworm --kmers <dump> --run_inchworm -K k -L L [--DS]` (Trinity:2654-2711)
+ * over a dictionary (keys ascending, canonical in DS mode; count 0 = pruned).
+ * Seeds: keys with count >= min_seed_cov (and seed_keep[composition], the
+ * seed-entropy table, when given), taken in order of count descending, then
+ * key ascending; a seed already used by an earlier contig is skipped.
+ * A contig starts as the seed's forward string (the key), then extends to the
+ * right: from its last k-mer x the candidates are x[1..k-1]+b, b = A,C,G,T; a
+ * candidate is available when it is in the dictionary with count > 0 and not
+ * used (DS: looked up and marked by canonical form); the one with the largest
+ * count is appended (ties: smallest base), marked used, and the walk goes on
+ * until no candidate is available.  Then the same to the left of the seed
+ * (candidates b+y[0..k-2], ties to the smallest base).  Every contig is
+ * returned in seed order, with the sum of its k-mers' counts and its seed key
+ * (the CLI reports those of length >= L as ">a<N>;<floor(sum/nkmers)>").
+ * seq_out gets the contigs' bases back to back, off_out[i..i+1] bounds
+ * contig i.  Returns the total length, or -(total+1) when a cap was short
+ * (n_contigs is still set).
+ */
+int64_t or_inchworm(const uint64_t* keys, const uint32_t* counts, uint64_t n, int k, int ds,
+                    uint32_t min_seed_cov, const uint8_t* seed_keep, char* seq_out, uint64_t seq_cap,
+                    uint64_t* off_out, uint64_t* sum_out, uint64_t* seed_out, uint64_t contig_cap,
+                    uint64_t* n_contigs);
+
+/*
+ * Counts of a fixed set of keys in the reads' windows (tk_oracle_iworm.c):
+ * keys[2i] = lo, keys[2i+1] = hi of key i.  Without tags a key is the
+ * 2kk-bit value of a kk-mer (kk <= 64); with tags (one per read, kk <= 32)
+ * the key of a read's window is (hi = tags[read], lo = k-mer).  Windows with
+ * a non-ACGT base do not count; canonical keys are min(k-mer, revcomp).
+ * Returns 0, -2 when the keys are not distinct.
+ */
+int or_kset_count(const uint64_t* keys, uint64_t n, const char* seq, const uint64_t* off, uint64_t n_reads,
+                  int kk, int canonical, const uint32_t* tags, uint64_t* counts);
+
 #ifdef __cplusplus
 }
 #endif

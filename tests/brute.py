@@ -153,3 +153,62 @@ def r2t(bundles, comps, reads, k: int, canonical=True):
             best = min(hits.items(), key=lambda kv: (-kv[1], kv[0]))
             res.append((best[0], best[1], nwin))
     return res
+
+
+def inchworm(d: dict, k: int, ds: bool, min_seed_cov: int, min_seed_entropy: float | None = None):
+    """Greedy contigs over a string dictionary {kmer: count} (canonical keys
+    when ds): seeds by count descending then string, each extended right then
+    left by the most abundant unused overlapping k-mer (ties: A<C<G<T)."""
+    canon = (lambda s: min(s, rc(s))) if ds else (lambda s: s)
+    used = set()
+    seeds = sorted((s for s, c in d.items()
+                    if c >= min_seed_cov and c > 0 and (min_seed_entropy is None or entropy(s) >= min_seed_entropy)),
+                   key=lambda s: (-d[s], s))
+    out = []
+    for s in seeds:
+        if s in used:
+            continue
+        used.add(s)
+        contig, total = s, d[s]
+        for side in ("right", "left"):
+            while True:
+                best = None
+                for b in "ACGT":
+                    y = canon(contig[-(k - 1):] + b if side == "right" else b + contig[:k - 1])
+                    c = d.get(y, 0)
+                    if c > 0 and y not in used and (best is None or c > best[0]):
+                        best = (c, b, y)
+                if best is None:
+                    break
+                used.add(best[2])
+                total += best[0]
+                contig = contig + best[1] if side == "right" else best[1] + contig
+        out.append((contig, total, s))
+    return out
+
+
+def kmer_value(s: str) -> int:
+    v = 0
+    for ch in s:
+        v = v * 4 + "ACGT".index(ch)
+    return v
+
+
+def kset_count(keys: list, reads: list, kk: int, canonical: bool, tags=None) -> list:
+    """Occurrences of each key string (or (tag, kmer string)) among the
+    reads' all-ACGT windows, by string slicing."""
+    index = {k: i for i, k in enumerate(keys)}
+    out = [0] * len(keys)
+    for r, s in enumerate(reads):
+        u = s.upper()
+        for i in range(len(u) - kk + 1):
+            w = u[i:i + kk]
+            if any(ch not in "ACGT" for ch in w):
+                continue
+            if canonical:
+                w = min(w, rc(w))
+            key = (tags[r], w) if tags is not None else w
+            j = index.get(key)
+            if j is not None:
+                out[j] += 1
+    return out

@@ -121,3 +121,26 @@ def counts_for_histo_witness(name: str, seed: int = 3) -> tuple[np.ndarray, list
     counts = np.array(vals, np.uint32)
     rng.shuffle(counts)
     return counts, want
+
+
+def sample_fasta(side: str) -> tuple[list[str], list[str]]:
+    """Names and sequences of the reference's sample reads (FASTA form)."""
+    import gzip
+    txt = gzip.decompress((GOLDEN / "sample_data" / f"reads.{side}.fa.gz").read_bytes()).decode()
+    lines = txt.split("\n")
+    return [ln[1:] for ln in lines[0::2] if ln], [ln for ln in lines[1::2]][:len(lines) // 2]
+
+
+def ref_seqs() -> list[str]:
+    """The 11 sample transcripts (refSeqs.fa), one sequence each."""
+    out, cur = [], []
+    for ln in (GOLDEN / "sample_data" / "refSeqs.fa").read_text().splitlines():
+        if ln.startswith(">"):
+            if cur:
+                out.append("".join(cur))
+            cur = []
+        else:
+            cur.append(ln.strip())
+    if cur:
+        out.append("".join(cur))
+    return out

@@ -502,6 +502,38 @@ def test_r2t_parity(api, ctx, canonical, ordered, monkeypatch):
     assert np.array_equal(got, want.astype(got.dtype))
 
 
+@pytest.mark.parametrize("canonical", [True, False])
+def test_sample_reads_parity(api, ctx, canonical):
+    """The reference's own sample reads (30,575 pairs of 76 nt, 1,190 + 627
+    reads with an N): count, dump, per-read stats (min_count 1 and 2) and the
+    vote against the 11 transcripts they were drawn from, each equal to the
+    oracle."""
+    from data import ref_seqs, sample_fasta
+    say("sample_reads_parity canonical", canonical)
+    seqs = sample_fasta("left")[1] + sample_fasta("right")[1]
+    seq, off = concat(seqs)
+    r, t, _, _ = _count(api, ctx, (seq, off), 25, canonical)
+    gk, gc = t.dump()
+    ok, oc = O.count(seq, off, 25, canonical)
+    assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+    total, valid = O.window_totals(seq, off, 25)
+    assert int(gc.sum()) == valid and total == 61150 * 52
+    for mc in (1, 2):
+        st = api.read_cov_stats(ctx, t, r, min_count=mc)
+        want = O.read_stats(seq, off, ok, oc, 25, canonical, mc, 1, 0, 0)
+        assert np.array_equal(st, want.astype(st.dtype)), mc
+    refs = ref_seqs()
+    bseq, boff = concat(refs)
+    comps = np.arange(len(refs), dtype=np.uint32)
+    ct = api.CTable(ctx, api.Reads(ctx, bseq, boff), comps, 25, canonical)
+    got = ct.vote(r)
+    keys, cv = O.r2t_build(bseq, boff, comps, 25, canonical)
+    want = O.r2t_vote(seq, off, keys, cv, 25, canonical)
+    assert np.array_equal(got, want.astype(got.dtype))
+    if canonical:  # about half of the reads come from these 11 transcripts
+        assert (got["hits"] > 0).mean() > 0.4
+
+
 @pytest.mark.parametrize("shift", [0, 1, 3])
 def test_reads_from_device_unaligned(api, ctx, shift):
     """Packing from device-resident ASCII at any byte alignment (and ending at
@@ -687,3 +719,129 @@ def test_server_results_equal_alone_and_in_the_mix(api, ctx):
                           [float(x[2]) for x in rows[:n]], [float(x[2]) for x in rows[n:]],
                           [float(x[3]) for x in rows[:n]], [float(x[3]) for x in rows[n:]], 50, 1, 10000)
     assert np.array_equal(alone[2][0], want)
+
+
+def _iworm_check(api, ctx, t, k, ds, msc, keep=None, keep_thr=None):
+    gk, gc = t.dump()
+    contigs, sums, seeds, info = api.inchworm(ctx, t, msc, keep)
+    want_c, want_s, want_seed = O.inchworm(gk, gc, k, ds, msc, keep)
+    assert len(contigs) == len(want_c)
+    assert contigs == want_c
+    assert np.array_equal(sums, want_s) and np.array_equal(seeds, want_seed)
+    return info
+
+
+@pytest.mark.parametrize("canonical", [True, False])
+def test_inchworm_parity_small_se(api, ctx, small_se, canonical):
+    """Greedy contigs over the 200k-read single-end table (the small
+    workload) equal the oracle's serial greedy, contig for contig, with seed
+    coverage 1 and 2."""
+    say("inchworm_parity_small_se canonical", canonical)
+    r, t, _, _ = _count(api, ctx, small_se, 25, canonical)
+    for msc in (2, 1):
+        info = _iworm_check(api, ctx, t, 25, canonical, msc)
+        say("  min_seed_cov", msc, info)
+
+
+@pytest.mark.parametrize("walks,lmax", [(1, 4096), (64, 16), (3, 8)])
+def test_inchworm_parity_forced_rounds(api, ctx, monkeypatch, walks, lmax):
+    """Tiny rounds (one walk: the serial order itself), and path buffers so
+    short that walks overflow and the first walk of a round regrows them."""
+    say("inchworm_parity_forced_rounds", walks, lmax)
+    monkeypatch.setenv("TK_IWORM_WALKS", str(walks))
+    monkeypatch.setenv("TK_IWORM_LMAX", str(lmax))
+    seqs = repetitive_reads(61, 3000, genome_len=2500, read_len=90, err=0.01) + edge_reads()
+    r, t, _, _ = _count(api, ctx, seqs, 25, True)
+    info = _iworm_check(api, ctx, t, 25, True, 2)
+    say("  ", info)
+
+
+@pytest.mark.parametrize("k", [13, 25, 31])
+def test_inchworm_parity_pruned_with_seed_entropy(api, ctx, k):
+    """After Inchworm's prune (min count 2, entropy 1.0), seeds filtered by a
+    seed entropy of 1.5; repeats and low-complexity reads make branches."""
+    say("inchworm_parity_pruned k", k)
+    seqs = repetitive_reads(70 + k, 4000, genome_len=1500, read_len=100, err=0.02) + \
+        ["A" * 60, "AC" * 40, "ACGTTGCA" * 12] * 20 + random_reads(71, 300, 30, 120)
+    r, t, _, _ = _count(api, ctx, seqs, k, True)
+    t.prune(2, api.entropy_keep_table(k, 1.0))
+    keep = api.entropy_keep_table(k, 1.5)
+    _iworm_check(api, ctx, t, k, True, 2, keep)
+    _iworm_check(api, ctx, t, k, True, 1, None)
+
+
+def test_inchworm_sample_reads_rebuild_transcripts(api, ctx):
+    """The reference's sample reads: contigs equal the oracle's, and the
+    longest contigs lie in the 11 transcripts the reads came from."""
+    from data import ref_seqs, sample_fasta
+    say("inchworm_sample_reads")
+    seq, off = concat(sample_fasta("left")[1] + sample_fasta("right")[1])
+    r, t, _, _ = _count(api, ctx, (seq, off), 25, True)
+    _iworm_check(api, ctx, t, 25, True, 2)
+    contigs, sums, seeds, info = api.inchworm(ctx, t, 2)
+    refs = ref_seqs()
+    big = sorted(contigs, key=len, reverse=True)[:5]
+    inside = [any(c in x or brute.rc(c) in x for x in refs) for c in big]
+    say("  longest", [len(c) for c in big], inside)
+    assert any(inside) and len(big[0]) > 1000
+
+
+def _kset_keys(strings, tags=None):
+    out = np.zeros((len(strings), 2), np.uint64)
+    for i, s in enumerate(strings):
+        v = brute.kmer_value(s)
+        out[i] = (v, tags[i]) if tags is not None else (v & ((1 << 64) - 1), v >> 64)
+    return out
+
+
+@pytest.mark.parametrize("kk,canonical", [(48, True), (48, False), (64, True), (40, True), (25, False)])
+def test_kset_count_parity(api, ctx, kk, canonical):
+    """Counts of a fixed set of long k-mers (weld k-mers are 48-mers) in the
+    reads equal the oracle's; keys are a mix of read windows and misses."""
+    say("kset_count kk", kk, "canonical", canonical)
+    import random
+    rnd = random.Random(kk)
+    seqs = repetitive_reads(kk, 20000, genome_len=5000, read_len=150, err=0.01) + edge_reads() + \
+        random_reads(kk + 1, 500, 40, 300, n_rate=0.03)
+    cand = set()
+    for s in seqs[:3000]:
+        u = s.upper()
+        for i in range(0, max(0, len(u) - kk + 1), 11):
+            w = u[i:i + kk]
+            if all(ch in "ACGT" for ch in w):
+                cand.add(min(w, brute.rc(w)) if canonical else w)
+    for _ in range(2000):
+        cand.add("".join(rnd.choice("ACGT") for _ in range(kk)))
+    keys = sorted(cand)
+    rnd.shuffle(keys)
+    kv = _kset_keys(keys)
+    seq, off = concat(seqs)
+    ks = api.KSet(ctx, kv)
+    got = ks.count(api.Reads(ctx, seq, off), kk, canonical)
+    want = O.kset_count(kv, seq, off, kk, canonical)
+    assert np.array_equal(got, want) and int(want.sum()) > 0
+    ks.free()
+
+
+def test_kset_count_tagged_and_errors(api, ctx):
+    say("kset_count tagged")
+    k = 24
+    seqs = repetitive_reads(5, 30000, genome_len=4000, read_len=150, err=0.01)
+    tags = np.array([i % 7 for i in range(len(seqs))], np.uint32)
+    keys = set()
+    for t in range(7):
+        for s in seqs[t:4000:7]:
+            for i in range(0, len(s) - k + 1, 9):
+                w = s[i:i + k]
+                keys.add((t, min(w, brute.rc(w))))
+    keys = sorted(keys)
+    kv = _kset_keys([w for _, w in keys], [t for t, _ in keys])
+    seq, off = concat(seqs)
+    ks = api.KSet(ctx, kv)
+    got = ks.count(api.Reads(ctx, seq, off), k, True, tags)
+    assert np.array_equal(got, O.kset_count(kv, seq, off, k, True, tags))
+    ks.free()
+    with pytest.raises(api.TkError, match="not distinct"):
+        api.KSet(ctx, np.concatenate([kv[:50], kv[10:11]]))
+    empty = api.KSet(ctx, np.zeros((0, 2), np.uint64))
+    assert len(empty.count(api.Reads(ctx, seq, off), k, True)) == 0

@@ -209,6 +209,19 @@ def test_cpuref_matches_oracle():
     seq, off = concat(seqs)
     ok, oc = O.count(seq, off, 25, True)
     want = O.read_stats(seq, off, ok, oc, 25, True, 2, 1, 0, 0)
+    seqs2 = repetitive_reads(9, 2000, genome_len=500)
+    s2, o2 = concat(seqs2)
+    k2, c2 = O.count(s2, o2, 25, True)
+    assert int(c2.max()) > 1
+    for threads in (1, 4):
+        got = O.cpuref_count_stats(s2, o2, 25, True, 2, threads, summary=True)
+        assert got["digest"] == O.table_digest(k2, c2)
+        assert np.array_equal(got["histo"], O.histo(c2).astype(np.uint64))
+        # the digest tells tables apart: one count off, or one key dropped
+        c3 = c2.copy()
+        c3[7] += 1
+        assert O.table_digest(k2, c3) != got["digest"]
+        assert O.table_digest(k2[1:], c2[1:]) != got["digest"]
     for threads in (1, 4):
         got = O.cpuref_count_stats(seq, off, 25, True, 2, threads)
         assert got["distinct"] == len(ok)
@@ -311,3 +324,130 @@ def test_histo_default_shape_matches_reference_witness(name):
     base = O.histo_base(1, 1)
     got = [(base + i, int(b)) for i, b in enumerate(bins) if b]
     assert got == want
+
+
+# ------------------------------------------- Inchworm greedy contigs
+
+def _dict_from_reads(seqs, k, ds):
+    seq, off = concat(seqs)
+    keys, counts = O.count(seq, off, k, ds)
+    return keys, counts
+
+
+@pytest.mark.parametrize("k,ds", [(5, True), (11, True), (25, True), (11, False), (25, False)])
+def test_inchworm_matches_brute_force(k, ds):
+    """or_inchworm equals the string brute force on dictionaries with
+    branches (repeats, errors) and low-count tails; seed coverage 1 and 2."""
+    seqs = repetitive_reads(40 + k, 600, genome_len=700, read_len=60, err=0.01) + random_reads(41, 60, 30, 80)
+    keys, counts = _dict_from_reads(seqs, k, ds)
+    d = _as_dict(keys, counts, k)
+    for msc in (1, 2):
+        contigs, sums, seeds = O.inchworm(keys, counts, k, ds, msc)
+        want = brute.inchworm(d, k, ds, msc)
+        assert [c for c, _, _ in want] == contigs
+        assert [t for _, t, _ in want] == sums.tolist()
+        assert [s for _, _, s in want] == [brute.kmer_of(int(x), k) for x in seeds]
+
+
+def test_inchworm_seed_entropy_and_pruned_entries():
+    k = 11
+    seqs = repetitive_reads(3, 400, genome_len=300, read_len=50) + ["A" * 40, "AC" * 25] * 5
+    keys, counts = _dict_from_reads(seqs, k, True)
+    counts = counts.copy()
+    counts[::7] = 0  # pruned entries stay in the table with count 0
+    d = {brute.kmer_of(int(a), k): int(b) for a, b in zip(keys, counts)}
+    keep = O.entropy_keep_table(k, 1.5)
+    contigs, sums, seeds = O.inchworm(keys, counts, k, True, 2, keep)
+    want = brute.inchworm(d, k, True, 2, 1.5)
+    assert [c for c, _, _ in want] == contigs and [t for _, t, _ in want] == sums.tolist()
+    for s in seeds:
+        assert brute.entropy(brute.kmer_of(int(s), k)) >= 1.5
+
+
+def test_inchworm_reassembles_a_transcript():
+    """Error-free reads tiling one repeat-free transcript at depth >= 2: one
+    contig, the transcript or its reverse complement (DS), every k-mer used
+    once, cov = total count / k-mers."""
+    import random
+    rnd = random.Random(5)
+    tx = "".join(rnd.choice("ACGT") for _ in range(3000))
+    k = 25
+    reads = [tx[i:i + 100] for i in range(0, len(tx) - 100 + 1, 7)] + [tx[-100:], tx[-50:]]
+    reads = [r if j % 2 else brute.rc(r) for j, r in enumerate(reads)]
+    keys, counts = _dict_from_reads(reads, k, True)
+    assert len(keys) == len(tx) - k + 1
+    contigs, sums, _ = O.inchworm(keys, counts, k, True, 2)
+    assert len([c for c in contigs if len(c) > k]) == 1
+    main = max(contigs, key=len)
+    # k-mers at the transcript's ends are covered once and cannot seed but are reached by extension
+    assert main in (tx, brute.rc(tx))
+    assert int(sums[contigs.index(main)]) == int(counts.sum())
+
+
+def test_inchworm_uses_each_kmer_once():
+    k = 25
+    seqs = repetitive_reads(77, 2000, genome_len=3000, read_len=100, err=0.005)
+    keys, counts = _dict_from_reads(seqs, k, True)
+    contigs, sums, seeds = O.inchworm(keys, counts, k, True, 1)
+    seen = set()
+    d = _as_dict(keys, counts, k)
+    for c, t in zip(contigs, sums):
+        ks = [min(c[i:i + k], brute.rc(c[i:i + k])) for i in range(len(c) - k + 1)]
+        assert not (set(ks) & seen) and len(set(ks)) == len(ks)
+        seen |= set(ks)
+        assert sum(d[x] for x in ks) == int(t)
+    assert seen == set(d)  # seed coverage 1: every k-mer ends in some contig
+
+
+# ------------------------------------------- counts of a fixed k-mer set
+
+def _keys_of(strings, tags=None):
+    out = np.zeros((len(strings), 2), np.uint64)
+    for i, s in enumerate(strings):
+        v = brute.kmer_value(s)
+        if tags is not None:
+            out[i] = (v, tags[i])
+        else:
+            out[i] = (v & ((1 << 64) - 1), v >> 64)
+    return out
+
+
+@pytest.mark.parametrize("kk,canonical", [(48, True), (48, False), (64, True), (33, True), (25, True)])
+def test_kset_count_matches_brute_force(kk, canonical):
+    import random
+    rnd = random.Random(kk)
+    seqs = repetitive_reads(kk, 800, genome_len=600, read_len=120, err=0.01) + edge_reads() + \
+        random_reads(kk + 1, 50, 40, 150, n_rate=0.03)
+    # keys: windows of the genome-like reads (hits) and random strings (misses), canonical forms
+    cand = set()
+    for s in seqs[:300]:
+        u = s.upper()
+        for i in range(0, max(0, len(u) - kk + 1), 7):
+            w = u[i:i + kk]
+            if all(ch in "ACGT" for ch in w):
+                cand.add(min(w, brute.rc(w)) if canonical else w)
+    for _ in range(200):
+        cand.add("".join(rnd.choice("ACGT") for _ in range(kk)))
+    keys = sorted(cand)
+    rnd.shuffle(keys)
+    seq, off = concat(seqs)
+    got = O.kset_count(_keys_of(keys), seq, off, kk, canonical)
+    want = brute.kset_count(keys, seqs, kk, canonical)
+    assert got.tolist() == want
+    assert sum(want) > 0
+
+
+def test_kset_count_tagged_matches_brute_force():
+    k = 24
+    seqs = repetitive_reads(4, 900, genome_len=500, read_len=100, err=0.01)
+    tags = [i % 3 for i in range(len(seqs))]
+    keys = []
+    for t in range(3):
+        for s in seqs[t:300:3]:
+            for i in range(0, len(s) - k + 1, 5):
+                w = s[i:i + k]
+                keys.append((t, min(w, brute.rc(w))))
+    keys = sorted(set(keys))
+    seq, off = concat(seqs)
+    got = O.kset_count(_keys_of([w for _, w in keys], [t for t, _ in keys]), seq, off, k, True, tags)
+    assert got.tolist() == brute.kset_count(keys, seqs, k, True, tags)

@@ -183,6 +183,17 @@ def test_fastaToKmerCoverageStats_matches_oracle(sample):
     assert out2[1:] == [O.format_stats_row(n, s) for n, s in zip(names, st2)]
 
 
+def iworm_text(contigs, sums, k, min_len=25, min_cov=1):
+    out, n = [], 0
+    for c, t in zip(contigs, sums.tolist()):
+        cov = t // (len(c) - k + 1)
+        if len(c) < min_len or cov < min_cov:
+            continue
+        n += 1
+        out.append(f">a{n};{cov} K: {k} length: {len(c)}\n{c}\n")
+    return "".join(out)
+
+
 @pytest.mark.gpu
 def test_inchworm_load_prune(sample):
     d, fa, names, seqs, seq, off, keys, counts = sample
@@ -190,12 +201,21 @@ def test_inchworm_load_prune(sample):
     dump.write_text(dump_text(keys, counts, 25))
     r = run([BIN / "inchworm", "--kmers", dump, "--run_inchworm", "-K", "25", "--monitor", "1", "--DS",
              "--num_threads", "6", "--PARALLEL_IWORM", "--min_any_entropy", "1.5", "-L", "25",
-             "--no_prune_error_kmers", "--minKmerCount", "2", "--dump_pruned", d / "pruned.fa"],
-            cwd=d, check=False)
-    assert r.returncode == 2 and "contig extension" in r.stderr
+             "--no_prune_error_kmers", "--minKmerCount", "2", "--dump_pruned", d / "pruned.fa"], cwd=d)
     pk, pc = O.prune(keys, counts, 25, 2, 1.5)
     assert (d / "inchworm.kmer_count").read_text().strip() == str(len(pk))
     assert (d / "pruned.fa").read_text() == dump_text(pk, pc, 25)
+    contigs, sums, _ = O.inchworm(pk, pc, 25, True, 2, O.entropy_keep_table(25, 1.5))
+    assert r.stdout == iworm_text(contigs, sums, 25)
+    assert r.stdout.startswith(">a1;")
+    # -L and --min_assembly_coverage filter the report; Inchworm's error pruning is refused
+    r2 = run([BIN / "inchworm", "--kmers", dump, "--run_inchworm", "-K", "25", "--DS", "-L", "60",
+              "--min_assembly_coverage", "3", "--min_seed_entropy", "0", "--min_seed_coverage", "1",
+              "--no_prune_error_kmers"], cwd=d)
+    contigs, sums, _ = O.inchworm(keys, counts, 25, True, 1, None)
+    assert r2.stdout == iworm_text(contigs, sums, 25, 60, 3)
+    r3 = run([BIN / "inchworm", "--kmers", dump, "--run_inchworm", "-K", "25", "--DS"], cwd=d, check=False)
+    assert r3.returncode != 0 and "no_prune_error_kmers" in r3.stderr
 
 
 @pytest.mark.gpu
@@ -386,3 +406,212 @@ def test_extract_normalized_reads_fasta(tmp_path):
     out = tmp_path / "o.fa"
     run([BIN / "extract_normalized_reads", "--seqType", "fa", "--selected", sel, "--output", out, fa])
     assert out.read_bytes() == b">a/1 first read\nACGTACGT\n>c\nTTTT\n>d/1\n\n"
+
+
+@pytest.mark.gpu
+def test_normalization_chain_on_sample_reads(tmp_path):
+    """insilico_read_normalization.pl's PE chain on the reference's sample
+    reads, tool by tool (FASTQ->FASTA, jellyfish count + dump -L 2, per-read
+    stats of each side, C-locale sort, pair merge + selection at max_cov 30,
+    extraction of the selected FASTQ records), against the oracle's count,
+    stats and selection."""
+    from data import sample_fasta
+    d = tmp_path
+    for side in ("left", "right"):
+        with open(d / f"{side}.fa", "wb") as f:
+            subprocess.run([BIN / "fastQ_to_fastA", "-I", SAMPLE / f"reads.{side}.fq.gz"], stdout=f, check=True)
+    (d / "both.fa").write_bytes((d / "left.fa").read_bytes() + (d / "right.fa").read_bytes())
+    run([BIN / "jellyfish", "count", "-t", "4", "-m", "25", "-s", "10000000", "--canonical", d / "both.fa"], cwd=d)
+    dump = d / "jellyfish.K25.min2.kmers.fa"
+    dump.write_text(run([BIN / "jellyfish", "dump", "-L", "2", d / "mer_counts.jf"]).stdout)
+    names, seqs = {}, {}
+    for side in ("left", "right"):
+        names[side], seqs[side] = sample_fasta(side)
+        out = run([BIN / "fastaToKmerCoverageStats", "--reads", d / f"{side}.fa", "--kmers", dump, "--kmer_size",
+                   "25", "--num_threads", "4", "--DS"]).stdout.splitlines(True)
+        (d / f"{side}.stats.sort").write_text(out[0] + "".join(sorted(out[1:], key=lambda x: x.encode())))
+    r = run([BIN / "nbkc_normalize", "--left", d / "left.stats.sort", "--right", d / "right.stats.sort",
+             "--sorted", "--max_cov", "30", "--min_cov", "0", "--max_CV", "10000"])
+    (d / "pairs.accs").write_text(r.stdout)
+    for side in ("left", "right"):
+        run([BIN / "extract_normalized_reads", "--seqType", "fq", "--selected", d / "pairs.accs", "--output",
+             d / f"{side}.norm.fq", SAMPLE / f"reads.{side}.fq.gz"])
+    # the same chain through the oracle
+    seq, off = concat(seqs["left"] + seqs["right"])
+    keys, counts = O.count(seq, off, 25, True)
+    m = counts >= 2
+    rows = {}
+    for side in ("left", "right"):
+        s1, o1 = concat(seqs[side])
+        st = O.read_stats(s1, o1, keys[m], counts[m], 25, True, 1, 1, 0, 0)
+        rows[side] = {n[:-2]: O.format_stats_row(n, x).split("\t") for n, x in zip(names[side], st)}
+    cores = sorted(rows["left"], key=lambda c: (c + "/1").encode())
+    L = [rows["left"][c] for c in cores]
+    R = [rows["right"][c] for c in cores]
+    f = lambda rs, i: [float(x[i]) for x in rs]  # noqa: E731
+    sel = O.select_pairs(f(L, 1), f(R, 1), f(L, 2), f(R, 2), f(L, 3), f(R, 3), 30, 0, 10000)
+    want = [c for c, s in zip(cores, sel) if s]
+    assert r.stdout.split() == want
+    assert 0 < len(want) < len(cores)
+    wanted = set(want)
+    for side in ("left", "right"):
+        recs = _fq_records(gzip.decompress((SAMPLE / f"reads.{side}.fq.gz").read_bytes()))
+        exp = b"".join(x for x in recs if x.split(b"\n")[0][1:].split()[0].decode()[:-2] in wanted)
+        assert (d / f"{side}.norm.fq").read_bytes() == exp
+
+
+# ------------------------------------------- Chrysalis: welds, de Bruijn graphs, node weights
+
+def _debruijn_expected(records, k, ss=False):
+    out = []
+    for name, s in records:
+        out.append(f"Component {name[2:] if name.startswith('s_') else name}\n")
+        ids, order, preds = {}, [], {}
+        for part in "".join(ch if ch.upper() in "ACGT" else " " for ch in s).split():
+            part = part.upper()
+            prev = None
+            for i in range(len(part) - k + 1):
+                w = part[i:i + k]
+                key = w if ss else min(w, brute.rc(w))
+                if key not in ids:
+                    ids[key] = len(order)
+                    order.append(key)
+                    preds[key] = set()
+                if prev is not None:
+                    preds[key].add(ids[prev])
+                prev = key
+        for key in order:
+            p = ",".join(str(x) for x in sorted(preds[key])) or "-1"
+            out.append(f"{ids[key]}\t{key}\t{p}\n")
+    return "".join(out)
+
+
+@pytest.mark.parametrize("ss", [False, True])
+def test_fasta_to_debruijn(tmp_path, ss):
+    import random
+    rnd = random.Random(3)
+    recs = []
+    for c in range(6):
+        seqs = ["".join(rnd.choice("ACGT") for _ in range(rnd.randint(30, 300))) for _ in range(rnd.randint(1, 4))]
+        if c == 2:
+            seqs.append(seqs[0][10:80])  # repeated k-mers: shared nodes, several predecessors
+        recs.append((f"s_{c * 7}", "X".join(seqs)))
+    recs.append(("s_99", "ACGTN" * 3))  # no k-mer
+    write_fasta(tmp_path / "b.fa", [n for n, _ in recs], [s for _, s in recs])
+    args = [BIN / "FastaToDeBruijn", "--fasta", tmp_path / "b.fa", "-K", "24", "--graph_per_record", "--threads", "2"]
+    if ss:
+        args.append("--SS")
+    assert run(args).stdout == _debruijn_expected(recs, 24, ss)
+
+
+def _weld_expected(contigs, reads, k, kk, min_glue, strand):
+    flank = (kk - k) // 2
+    canon = (lambda s: s) if strand else (lambda s: min(s, brute.rc(s)))
+    occ = {}
+    for i, c in enumerate(contigs):
+        for p in range(len(c) - k + 1):
+            w = c[p:p + k]
+            if strand or w <= brute.rc(w):
+                occ.setdefault(w, []).append((i, c, p))
+            else:
+                rc = brute.rc(c)
+                occ.setdefault(brute.rc(w), []).append((i, rc, len(c) - p - k))
+    cands = []
+    for w, v in occ.items():
+        for (ia, sa, pa) in v:
+            for (ib, sb, pb) in v:
+                if ia == ib or pa < flank or pb + k + flank > len(sb):
+                    continue
+                weld = sa[pa - flank:pa] + w + sb[pb + k:pb + k + flank]
+                cands.append((min(ia, ib), max(ia, ib), canon(weld), w))
+    welds = sorted({c[2] for c in cands})
+    counts = brute.kset_count(welds, reads, kk, not strand)
+    sup = dict(zip(welds, counts))
+    best = {}
+    for a, b, weld, w in cands:
+        s = sup[weld]
+        if s < min_glue:
+            continue
+        cur = best.get((a, b))
+        if cur is None or (s, [-ord(ch) for ch in weld], [-ord(ch) for ch in w]) > \
+                (cur[0], [-ord(ch) for ch in cur[1]], [-ord(ch) for ch in cur[2]]):
+            best[(a, b)] = (s, weld, w)
+    return "".join(f"{a} -> {b} weld: {best[(a, b)][1]} k-mer: {best[(a, b)][2]} support: {best[(a, b)][0]}\n"
+                   for a, b in sorted(best))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("strand", [False, True])
+def test_graph_from_fasta_weld_support(tmp_path, strand):
+    """Contigs cut from transcripts with overlaps of 30-60 bases (so pairs
+    share 24-mers), some reverse-complemented; reads tiled over the
+    transcripts: the welded pairs and their best weld k-mer equal the
+    restatement's, with the weld counts taken from the reads."""
+    import random
+    rnd = random.Random(11 + strand)
+    contigs, reads = [], []
+    for t in range(12):
+        tx = "".join(rnd.choice("ACGT") for _ in range(rnd.randint(400, 900)))
+        cut = rnd.randint(150, len(tx) - 150)
+        ov = rnd.randint(30, 60)
+        a, b = tx[:cut + ov], tx[cut:]
+        if t % 3 == 0 and not strand:
+            b = brute.rc(b)
+        contigs += [a, b]
+        for i in range(0, len(tx) - 100, rnd.randint(3, 9)):
+            r = tx[i:i + 100]
+            reads.append(brute.rc(r) if (i % 2 and not strand) else r)
+    contigs.append("ACGT" * 10)
+    write_fasta(tmp_path / "iw.fa", [f"a{i + 1};10" for i in range(len(contigs))], contigs)
+    write_fasta(tmp_path / "r.fa", [f"r{i}" for i in range(len(reads))], reads)
+    args = [BIN / "GraphFromFasta", "-i", tmp_path / "iw.fa", "-r", tmp_path / "r.fa", "-min_contig_length", "1",
+            "-min_glue", "2", "-glue_factor", "0.05", "-min_iso_ratio", "0.05", "-t", "4", "-k", "24", "-kk", "48"]
+    if strand:
+        args.append("-strand")
+    out = run(args).stdout
+    assert out == _weld_expected(contigs, reads, 24, 48, 2, strand)
+    assert len(out.splitlines()) >= 10
+
+
+@pytest.mark.gpu
+def test_quantify_graph_single_and_batched(tmp_path):
+    """Node weights of each component = the windows of its reads equal to the
+    node (canonical), one component per call and all components of a listing
+    in one call; both equal the brute-force count."""
+    import random
+    rnd = random.Random(5)
+    recs, comp_reads = [], {}
+    for c in (3, 8, 21):
+        tx = "".join(rnd.choice("ACGT") for _ in range(rnd.randint(300, 700)))
+        recs.append((f"s_{c}", tx[:200] + "X" + tx[150:]))
+        rs = []
+        for i in range(0, len(tx) - 76, 4):
+            r = tx[i:i + 76]
+            rs.append(brute.rc(r) if i % 8 else r)
+        comp_reads[c] = rs
+    write_fasta(tmp_path / "b.fa", [n for n, _ in recs], [s for _, s in recs])
+    graph = run([BIN / "FastaToDeBruijn", "--fasta", tmp_path / "b.fa", "-K", "24", "--graph_per_record"]).stdout
+    blocks = graph.split("Component ")[1:]
+    listing = []
+    for blk in blocks:
+        cid = int(blk.split("\n", 1)[0])
+        base = tmp_path / f"c{cid}"
+        (tmp_path / f"c{cid}.graph.tmp").write_text("Component " + blk)
+        (tmp_path / f"c{cid}.reads.tmp").write_text("".join(f">r{j} 100\n{r}\n" for j, r in enumerate(comp_reads[cid])))
+        listing.append(f"{cid}\t{base}\n")
+    (tmp_path / "listing.txt").write_text("".join(listing))
+
+    def expected(cid):
+        lines = (tmp_path / f"c{cid}.graph.tmp").read_text().splitlines()
+        kmers = [ln.split("\t")[1] for ln in lines[1:]]
+        cnt = brute.kset_count(kmers, comp_reads[cid], 24, True)
+        return lines[0] + "\n" + "".join(f"{ln}\t{n}\n" for ln, n in zip(lines[1:], cnt))
+
+    for cid in (3, 8, 21):
+        run([BIN / "QuantifyGraph", "-g", tmp_path / f"c{cid}.graph.tmp", "-i", tmp_path / f"c{cid}.reads.tmp",
+             "-o", tmp_path / f"c{cid}.graph.out", "-max_reads", "1000000", "-k", "24"])
+        assert (tmp_path / f"c{cid}.graph.out").read_text() == expected(cid)
+        (tmp_path / f"c{cid}.graph.out").unlink()
+    run([BIN / "QuantifyGraph", "--batch", tmp_path / "listing.txt", "-k", "24"])
+    for cid in (3, 8, 21):
+        assert (tmp_path / f"c{cid}.graph.out").read_text() == expected(cid)

@@ -58,6 +58,13 @@ SIGNATURES = {
     "tk_table_dump_size": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _U64P]),
     "tk_table_dump": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint64, _U64P]),
     "tk_table_histo": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64, C.c_uint64, _P, C.c_uint64]),
+    "tk_inchworm": (C.c_int, [_P, _P, C.c_uint32, _P, C.POINTER(_P)]),
+    "tk_kset_create": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(_P)]),
+    "tk_kset_count": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, _P, _P]),
+    "tk_kset_free": (C.c_int, [_P]),
+    "tk_contigs_info": (C.c_int, [_P, _U64P, _U64P, _U64P, _U64P]),
+    "tk_contigs_get": (C.c_int, [_P, _P, _P, _P, _P]),
+    "tk_contigs_free": (C.c_int, [_P]),
     "tk_table_load": (C.c_int, [_P, _P, _P, _P, C.c_uint64]),
     "tk_table_prune": (C.c_int, [_P, _P, C.c_uint32, _P, _U64P]),
     "tk_table_lookup": (C.c_int, [_P, _P, _P, C.c_uint64, _P]),
@@ -437,6 +444,59 @@ def select_pairs(ctx: Context, left: np.ndarray, right: np.ndarray, max_cov: int
     _check(lib().tk_select_pairs(ctx.h, _ptr(left), _ptr(right), len(left), max_cov, min_cov, max_cv,
                                  _ptr(sel), C.byref(ns)), "tk_select_pairs")
     return sel, ns.value
+
+
+class KSet:
+    """A fixed set of 128-bit keys (tk_kset): keys is a (n, 2) uint64 array of (lo, hi)."""
+
+    def __init__(self, ctx: Context, keys: np.ndarray):
+        self.ctx = ctx
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, 2)
+        self.n = len(keys)
+        self.h = C.c_void_p()
+        _check(lib().tk_kset_create(ctx.h, _ptr(keys) if self.n else None, self.n, C.byref(self.h)),
+               "tk_kset_create")
+
+    def count(self, reads: Reads, kk: int, canonical: bool = True, tags: np.ndarray | None = None) -> np.ndarray:
+        out = np.zeros(max(self.n, 1), np.uint64)
+        t = None if tags is None else np.ascontiguousarray(tags, dtype=np.uint32)
+        _check(lib().tk_kset_count(self.ctx.h, self.h, reads.h, kk, int(canonical),
+                                   _ptr(t) if t is not None else None, _ptr(out)), "tk_kset_count")
+        return out[:self.n]
+
+    def free(self):
+        if self.h:
+            lib().tk_kset_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def inchworm(ctx: Context, table: "Table", min_seed_cov: int = 2, seed_keep: np.ndarray | None = None):
+    """Greedy contigs over the table (tk_inchworm): (list of str, count sums,
+    seed keys, {"rounds", "walks"})."""
+    h = C.c_void_p()
+    keep = None if seed_keep is None else np.ascontiguousarray(seed_keep, dtype=np.uint8)
+    _check(lib().tk_inchworm(ctx.h, table.h, min_seed_cov, _ptr(keep) if keep is not None else None,
+                             C.byref(h)), "tk_inchworm")
+    try:
+        n, tot, rounds, walks = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().tk_contigs_info(h, C.byref(n), C.byref(tot), C.byref(rounds), C.byref(walks)),
+               "tk_contigs_info")
+        seq = np.zeros(max(tot.value, 1), np.uint8)
+        off = np.zeros(n.value + 1, np.uint64)
+        sums = np.zeros(max(n.value, 1), np.uint64)
+        seeds = np.zeros(max(n.value, 1), np.uint64)
+        _check(lib().tk_contigs_get(h, _ptr(seq), _ptr(off), _ptr(sums), _ptr(seeds)), "tk_contigs_get")
+    finally:
+        lib().tk_contigs_free(h)
+    raw = seq.tobytes()
+    contigs = [raw[int(off[i]):int(off[i + 1])].decode() for i in range(n.value)]
+    return contigs, sums[:n.value], seeds[:n.value], {"rounds": rounds.value, "walks": walks.value}
 
 
 class CTable:

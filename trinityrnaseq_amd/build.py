@@ -33,7 +33,7 @@ HIP_FLAGS = [
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-pthread", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["tk_lib.hip", "tk_sort.hip", "tk_batch.hip"]
+HIP_SOURCES = ["tk_lib.hip", "tk_sort.hip", "tk_batch.hip", "tk_iworm.hip", "tk_kset.hip"]
 HOST_SOURCES = ["tk_server.cpp"]
 TOOLS = {
     "jellyfish": ["tools/jellyfish_main.cpp"],
@@ -44,6 +44,9 @@ TOOLS = {
     "fastQ_to_fastA": ["tools/fastq_to_fasta_main.cpp"],
     "seqtk-trinity": ["tools/fastq_to_fasta_main.cpp"],
     "extract_normalized_reads": ["tools/extract_normalized_reads_main.cpp"],
+    "GraphFromFasta": ["tools/GraphFromFasta_main.cpp"],
+    "FastaToDeBruijn": ["tools/FastaToDeBruijn_main.cpp"],
+    "QuantifyGraph": ["tools/QuantifyGraph_main.cpp"],
 }
 TOOL_COMMON = ["tools/seq_io.cpp", "tools/tool_util.cpp"]
 

@@ -111,6 +111,9 @@ struct tk_ctable {
 
 namespace tk {
 TableView table_view(const tk_table* t);
+// Clear a pending table and fold its side table into final counts (every
+// entry point that reads slots calls this first).
+int table_finalize(tk_ctx* ctx, tk_table* t);
 // Sorting/scan helpers implemented with hipcub in tk_sort.hip.
 int sort_pairs_u64_u32(tk_ctx* ctx, uint64_t* d_keys, uint32_t* d_vals, uint64_t n,
                        int key_bits);

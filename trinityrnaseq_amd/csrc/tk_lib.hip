@@ -1579,6 +1579,8 @@ static int table_set_final(tk_ctx* ctx, tk_table* t, bool final_) {
   return TK_OK;
 }
 
+int table_finalize(tk_ctx* ctx, tk_table* t) { return table_set_final(ctx, t, true); }
+
 // Thread per (key,count) pair.
 __global__ void k_load(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ counts,
                        uint64_t n, TableView t, uint32_t* __restrict__ flags) {

@@ -10,9 +10,18 @@
 // reads), drops k-mers with count < --minKmerCount or base-composition
 // entropy < --min_any_entropy, and writes inchworm.kmer_count (the number
 // of k-mers kept; read back at Trinity:3189).  --dump_pruned FILE writes the
-// kept dictionary as a jellyfish-style FASTA dump.  Greedy contig extension
-// (--run_inchworm's assembly) is not part of this build: with --run_inchworm
-// the tool reports that and exits 2 after writing the dictionary outputs.
+// kept dictionary as a jellyfish-style FASTA dump.  With --run_inchworm the
+// greedy contigs (tk_inchworm) go to stdout, those of length >= -L and
+// coverage >= --min_assembly_coverage, as
+//   >a<N>;<cov> K: <k> length: <len>
+//   <sequence>
+// N counting the reported contigs from 1 in seed order and cov = the mean
+// count of the contig's k-mers rounded down (scaffold_iworm_contigs.pl:67
+// and filter_iworm_by_min_length_or_cov.pl:22-26 read a/N;cov).  Seeds need
+// count >= --min_seed_coverage (2) and composition entropy >=
+// --min_seed_entropy (1.5).  Inchworm's error-k-mer pruning (the default
+// when --no_prune_error_kmers is absent) is not part of this build: the tool
+// refuses to assemble without --no_prune_error_kmers.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -55,22 +64,23 @@ int main(int argc, char** argv) {
   const double min_entropy = a.real({"--min_any_entropy"}, 0.0);
   const long long min_count = a.integer({"--minKmerCount"}, 1);
   const bool run = a.flag({"--run_inchworm"});
-  // accepted for command-line compatibility; they steer contig extension
+  const long long min_len = a.integer({"-L"}, 25);
+  const long long min_asm_cov = a.integer({"--min_assembly_coverage"}, 1);
+  const double min_seed_entropy = a.real({"--min_seed_entropy"}, 1.5);
+  const long long min_seed_cov = a.integer({"--min_seed_coverage"}, 2);
+  const bool no_error_prune = a.flag({"--no_prune_error_kmers"});
+  // accepted for command-line compatibility (threads, logging, scratch files)
   a.integer({"--num_threads"}, 1);
   a.integer({"--monitor"}, 0);
-  a.integer({"-L"}, 25);
   a.flag({"--PARALLEL_IWORM"});
-  a.flag({"--no_prune_error_kmers"});
   a.flag({"--keep_tmp_files"});
-  a.flag({"--make_super_reads"});
-  std::string ignored;
-  a.value({"--min_assembly_coverage"}, ignored);
-  a.value({"--min_seed_entropy"}, ignored);
-  a.value({"--min_seed_coverage"}, ignored);
+  if (a.flag({"--make_super_reads"})) die("inchworm: --make_super_reads is not part of this build");
   auto unk = a.unknown_options();
   if (!unk.empty()) die("inchworm: unknown option %s", unk[0].c_str());
   if (kmers.empty() == reads.empty()) die("inchworm: need exactly one of --kmers / --reads");
   if (k < 1 || k > 31) die("inchworm: -K must be in [1,31]");
+  if (run && !no_error_prune)
+    die("inchworm: error-k-mer pruning is not part of this build; run with --no_prune_error_kmers");
 
   try {
     tk_ctx* ctx = open_ctx();
@@ -124,13 +134,38 @@ int main(int argc, char** argv) {
       fclose(o);
       if (rename(tmp.c_str(), dump_out.c_str()) != 0) die("cannot rename %s", tmp.c_str());
     }
+    if (run) {
+      std::vector<uint8_t> seed_keep = entropy_keep(k, min_seed_entropy);
+      tk_contigs* cg = nullptr;
+      check(tk_inchworm(ctx, table, min_seed_cov < 1 ? 1 : (uint32_t)min_seed_cov,
+                        min_seed_entropy > 0 ? seed_keep.data() : nullptr, &cg),
+            "tk_inchworm");
+      uint64_t n = 0, total = 0, rounds = 0, walks = 0;
+      check(tk_contigs_info(cg, &n, &total, &rounds, &walks), "tk_contigs_info");
+      std::string seq(total, '\0');
+      std::vector<uint64_t> off(n + 1), sums(n), seeds(n);
+      check(tk_contigs_get(cg, &seq[0], off.data(), sums.data(), seeds.data()), "tk_contigs_get");
+      tk_contigs_free(cg);
+      Out w(stdout);
+      uint64_t reported = 0;
+      char hdr[128];
+      for (uint64_t i = 0; i < n; i++) {
+        const uint64_t L = off[i + 1] - off[i];
+        const uint64_t cov = sums[i] / (L - (uint64_t)k + 1);
+        if ((long long)L < min_len || (long long)cov < min_asm_cov) continue;
+        reported++;
+        const int hl = snprintf(hdr, sizeof hdr, ">a%llu;%llu K: %d length: %llu\n", (unsigned long long)reported,
+                                (unsigned long long)cov, k, (unsigned long long)L);
+        w.put(hdr, (size_t)hl);
+        w.put(seq.data() + off[i], L);
+        w.putc('\n');
+      }
+      w.flush();
+      fprintf(stderr, "-inchworm: %llu contigs (%llu reported), %llu rounds, %llu walks\n", (unsigned long long)n,
+              (unsigned long long)reported, (unsigned long long)rounds, (unsigned long long)walks);
+    }
     tk_table_free(table);
     tk_ctx_free(ctx);
-    if (run) {
-      fprintf(stderr, "inchworm: greedy contig extension is not part of this build "
-                      "(dictionary load and prune done)\n");
-      return 2;
-    }
   } catch (const std::exception& e) {
     die("%s", e.what());
   }
