@@ -771,19 +771,22 @@ def test_inchworm_parity_pruned_with_seed_entropy(api, ctx, k):
 
 
 def test_inchworm_sample_reads_rebuild_transcripts(api, ctx):
-    """The reference's sample reads: contigs equal the oracle's, and the
-    longest contigs lie in the 11 transcripts the reads came from."""
+    """The reference's sample reads: contigs equal the oracle's, and every
+    25-mer of the 11 transcripts the reads came from lies in some contig."""
     from data import ref_seqs, sample_fasta
     say("inchworm_sample_reads")
     seq, off = concat(sample_fasta("left")[1] + sample_fasta("right")[1])
     r, t, _, _ = _count(api, ctx, (seq, off), 25, True)
     _iworm_check(api, ctx, t, 25, True, 2)
     contigs, sums, seeds, info = api.inchworm(ctx, t, 2)
-    refs = ref_seqs()
-    big = sorted(contigs, key=len, reverse=True)[:5]
-    inside = [any(c in x or brute.rc(c) in x for x in refs) for c in big]
-    say("  longest", [len(c) for c in big], inside)
-    assert any(inside) and len(big[0]) > 1000
+    in_contigs = set()
+    for c in contigs:
+        for i in range(len(c) - 24):
+            w = c[i:i + 25]
+            in_contigs.add(min(w, brute.rc(w)))
+    for ref in ref_seqs():
+        ks = [min(ref[i:i + 25], brute.rc(ref[i:i + 25])) for i in range(len(ref) - 24)]
+        assert all(k in in_contigs for k in ks)
 
 
 def _kset_keys(strings, tags=None):
