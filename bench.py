@@ -85,7 +85,8 @@ def cpu_baseline(tx, n_pairs_sample: int, max_cov: int) -> dict:
             "count_s": r["count_s"], "stats_s": r["stats_s"]}
 
 
-def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict, steps: int) -> dict:
+def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict, steps: int,
+                          ops: int | None = None) -> dict:
     """Rates of random 64-byte line reads and random 8-byte atomics (add, CAS,
     workgroup-scope add) over a buffer the size of the step's count table and
     over smaller ones (tk_micro.hip), the XCD each workgroup runs on, and the
@@ -109,7 +110,7 @@ def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict,
         best = 0.0
         for _ in range(3):
             ms, done = ctypes.c_float(), ctypes.c_uint64()
-            rc = lib.tkm_run(kind, ctypes.c_void_p(buf.ptr), nb, wins_per_step, ctypes.byref(ms),
+            rc = lib.tkm_run(kind, ctypes.c_void_p(buf.ptr), nb, ops or wins_per_step, ctypes.byref(ms),
                              ctypes.byref(done))
             if rc:
                 raise RuntimeError(f"tkm_run({name}) failed: {rc}")
@@ -381,6 +382,10 @@ def bench_stress(args) -> dict:
         b.free()
     contigs.free()
     api.thread_cache(0)
+    if args.ceiling:
+        # random 64-B line reads / atomics over a buffer the size of this job's count table
+        roofline["ceiling"] = random_access_ceiling(ctx, info["capacity"] * 8, wins, kern, args.steps,
+                                                    ops=wins // 8)
     ctx.close()
     cpu = None
     if not args.no_cpu:
@@ -418,14 +423,17 @@ def percentiles(xs) -> dict:
 
 def cpu_baseline_stress(tx, n_pairs_sample: int) -> dict:
     """Count + coverage stats of the first n pairs of the stressed stream with
-    the OpenMP CPU restatement (oracle/tk_cpuref.cpp) on every host core this
-    process may use; the median of three runs (warm page cache)."""
+    the OpenMP CPU restatement (oracle/tk_cpuref.cpp) on the host cores this
+    process is given (OMP_NUM_THREADS when set, else every core of its
+    affinity mask); the median of three runs (warm page cache)."""
     from oracle import oracle as O
     from trinityrnaseq_amd import synth
     buf = np.empty(2 * n_pairs_sample * READ_LEN, np.uint8)
     synth.pe_reads(tx, n_pairs_sample, READ_LEN, seed=1005, sub_rate=0.015, out=buf)
     off = np.arange(2 * n_pairs_sample + 1, dtype=np.uint64) * np.uint64(READ_LEN)
     threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     wins = 2 * n_pairs_sample * (READ_LEN - K + 1)
     runs = []
     for _ in range(3):
@@ -641,7 +649,10 @@ def main() -> int:
                     help="pairs of the CPU baseline's sample (median of 3 runs on all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ceiling", action="store_true",
-                    help="skip the random-access ceiling microbenchmarks after the timed steps")
+                    help="large_pe: skip the random-access ceiling microbenchmarks after the timed steps")
+    ap.add_argument("--ceiling", action="store_true",
+                    help="stress: run the random-access ceiling microbenchmarks over a buffer the size of "
+                         "the count table after the timed steps")
     ap.add_argument("--copy-ref-gb", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GB (HBM counter reference)")
     ap.add_argument("--workload", choices=["stress", "large_pe", "components", "serving", "inchworm"],

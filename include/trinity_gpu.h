@@ -188,7 +188,7 @@ int tk_table_free(tk_table* t);
 /*
  * Occurrences in the reads of a given set of keys, each 128 bits as two
  * words (lo, hi), hi < 2^62, all distinct (TK_ERR_ARG otherwise):
- *  - plain kk-mers, kk <= 64 (tags NULL): the key is the 2kk-bit k-mer
+ *  - plain kk-mers, kk <= 63 (tags NULL): the key is the 2kk-bit k-mer
  *    value, hi = its bits 64..127; GraphFromFasta's weld k-mers
  *    (-k 24 -kk 48, Trinity:2180) counted in the reads;
  *  - tagged k-mers, kk <= 32 (tags = one u32 per read): the key of a read's

@@ -677,6 +677,62 @@ def test_batch_components_match_per_component_oracle(api, ctx, use_graph, packed
             assert np.array_equal(vo[a:z], wv.astype(vo.dtype)), (rep, c)
 
 
+def _check_batch(api, ctx, comps, b, ro):
+    n = int(ro[-1])
+    d_st = api.DeviceBuffer(ctx, max(n, 1) * 32)
+    d_vo = api.DeviceBuffer(ctx, max(n, 1) * 12)
+    b.run_d(d_st.ptr, d_vo.ptr, min_count=2)
+    st = np.zeros(n, api.READ_STAT_DTYPE)
+    vo = np.zeros(n, api.R2T_VOTE_DTYPE)
+    d_st.download(st)
+    d_vo.download(vo)
+    for c, (rs, cs) in enumerate(comps):
+        if not rs:
+            continue
+        seq, off = concat(rs)
+        ok, oc = O.count(seq, off, 25, True)
+        a, z = int(ro[c]), int(ro[c + 1])
+        assert np.array_equal(st[a:z], O.read_stats(seq, off, ok, oc, 25, True, 2, 1, 0, 0).astype(st.dtype)), c
+        if cs:
+            bseq, boff = concat(cs)
+            keys, cv = O.r2t_build(bseq, boff, np.arange(len(cs)), 25, True)
+            assert np.array_equal(vo[a:z], O.r2t_vote(seq, off, keys, cv, 25, True).astype(vo.dtype)), c
+    d_st.free()
+    d_vo.free()
+
+
+@pytest.mark.parametrize("window_load,waves", [(0.35, 2), (0.7, 4), (0.0, 0)])
+def test_batch_layouts_autotune_and_large_component(api, ctx, window_load, waves):
+    """Explicit table load and waves per component (both corners of the
+    autotuner's grid), and the autotuned layout, on a batch that holds a
+    component of 60,000 reads beside small ones: every component equal to
+    the oracle run on it alone."""
+    from trinityrnaseq_amd import synth
+    say("batch layouts", window_load, waves)
+    comps = _components(23, 12)
+    tx = synth.Transcriptome(777, 8, 0, 1500, 4000)
+    s, o = synth.se_reads(tx, 60000, 150, seed=778, sub_rate=0.01)
+    big = [s[int(o[i]):int(o[i + 1])].tobytes().decode() for i in range(60000)]
+    comps.insert(4, (big, [tx.seq(j) for j in range(tx.n)]))
+    all_reads = [r for rs, _ in comps for r in rs]
+    all_contigs = [c for _, cs in comps for c in cs]
+    ro = np.cumsum([0] + [len(rs) for rs, _ in comps]).astype(np.uint64)
+    co = np.cumsum([0] + [len(cs) for _, cs in comps]).astype(np.uint64)
+    reads = api.Reads.from_list(ctx, all_reads)
+    contigs = api.Reads.from_list(ctx, all_contigs)
+    b = api.Batch(ctx, reads, ro, contigs, co, window_load=window_load, stats_waves=waves, use_graph=True)
+    if window_load == 0.0:
+        n = len(all_reads)
+        d_st = api.DeviceBuffer(ctx, n * 32)
+        d_vo = api.DeviceBuffer(ctx, n * 12)
+        ms = b.autotune(d_st.ptr, d_vo.ptr, min_count=2)
+        say("  autotuned", b.info(), f"{ms:.3f} ms")
+        d_st.free()
+        d_vo.free()
+    _check_batch(api, ctx, comps, b, ro)
+    b.free()
+
+
 def _plain_job(api, ctx, seq, off, n_pairs, max_cov=50):
     r = api.Reads(ctx, seq.tobytes(), off)
     t = api.Table(ctx, 25, True, int(api.estimate_distinct(ctx, r, 25, True) * 1.05) + 4096)
@@ -797,7 +853,7 @@ def _kset_keys(strings, tags=None):
     return out
 
 
-@pytest.mark.parametrize("kk,canonical", [(48, True), (48, False), (64, True), (40, True), (25, False)])
+@pytest.mark.parametrize("kk,canonical", [(48, True), (48, False), (63, True), (40, True), (25, False)])
 def test_kset_count_parity(api, ctx, kk, canonical):
     """Counts of a fixed set of long k-mers (weld k-mers are 48-mers) in the
     reads equal the oracle's; keys are a mix of read windows and misses."""

@@ -9,7 +9,7 @@
 //    (its 24-mers) and edges (25-mers) weighted by the windows of the reads
 //    ReadsToTranscripts assigned to that component.
 // Both are one table of 128-bit keys (hi, lo) built once and probed by every
-// read window: a plain kk-mer (kk <= 64) is its 2kk-bit value split in two
+// read window: a plain kk-mer (kk <= 63) is its 2kk-bit value split in two
 // words; a tagged k-mer (k <= 32) is (tag of the read, k-mer), so one table
 // holds the nodes of every component of a batch.
 //
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_ks_count(ReadsView rv, int kk, int cano
                                                   const uint64_t* __restrict__ slots, uint64_t nb,
                                                   unsigned long long* __restrict__ counts) {
   typedef unsigned __int128 u128;
-  const u128 mask = kk == 64 ? ~(u128)0 : (((u128)1 << (2 * kk)) - 1);
+  const u128 mask = ((u128)1 << (2 * kk)) - 1;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rv.n_reads;
        r += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t len = rv.len[r];
@@ -254,8 +254,8 @@ int tk_kset_count(tk_ctx* c, tk_kset* s, const tk_reads* reads, int kk, int cano
                   uint64_t* counts) {
   using namespace tk;
   if (!c || !s || !reads || (s->n && !counts)) return fail(TK_ERR_ARG, "NULL argument");
-  if (kk < 1 || kk > 64 || (tags && kk > 32))
-    return fail(TK_ERR_ARG, "tk_kset_count: kk=%d (1..64, 1..32 with tags)", kk);
+  if (kk < 1 || kk > 63 || (tags && kk > 32))
+    return fail(TK_ERR_ARG, "tk_kset_count: kk=%d (1..63, 1..32 with tags)", kk);
   if (hipSetDevice(c->device) != hipSuccess) return fail(TK_ERR_HIP, "hipSetDevice");
   const uint64_t ns = s->nb * 4;
   uint32_t* dtags = nullptr;

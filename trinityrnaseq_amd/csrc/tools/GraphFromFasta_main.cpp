@@ -106,7 +106,7 @@ int main(int argc, char** argv) {
   a.flag({"-debug_weld_all"});
   a.flag({"-report_welds"});
   if (!a.unknown_options().empty()) die("GraphFromFasta: unknown option %s", a.unknown_options()[0].c_str());
-  if (k < 1 || k > 32 || kk <= k || kk > 64 || (kk - k) % 2) die("GraphFromFasta: need 1 <= k <= 32, k < kk <= 64, kk - k even");
+  if (k < 1 || k > 32 || kk <= k || kk > 63 || (kk - k) % 2) die("GraphFromFasta: need 1 <= k <= 32, k < kk <= 63, kk - k even");
   const int flank = (kk - k) / 2;
   try {
     SeqSet contigs = read_all(iworm, false);
