@@ -273,6 +273,19 @@ int tk_select_pairs_d(tk_ctx* ctx, const tk_read_stat* d_left, const tk_read_sta
 int tk_select_pairs(tk_ctx* ctx, const tk_read_stat* left, const tk_read_stat* right,
                     uint64_t n_pairs, int max_cov, int min_cov, int max_cv,
                     uint8_t* selected, uint64_t* n_selected);
+/*
+ * tk_select_pairs_d without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as tk_select_pairs_d, for at most 2^24 pairs (more fail
+ * with TK_ERR_ARG); its working memory (flags, per-pair draw counts and
+ * ranks, scan storage: 132 MB) is a static device array that each GPU holds
+ * until the process ends.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (other threads, other contexts) race.
+ */
+int tk_select_pairs_d_noalloc(tk_ctx* ctx, const tk_read_stat* d_left, const tk_read_stat* d_right,
+                              uint64_t n_pairs, int max_cov, int min_cov, int max_cv,
+                              uint32_t max_nwin, uint8_t* d_selected, uint64_t* n_selected);
 
 /* ---------------- ReadsToTranscripts ---------------- */
 /*

@@ -71,6 +71,8 @@ SIGNATURES = {
     "tk_table_free": (C.c_int, [_P]),
     "tk_read_cov_stats": (C.c_int, [_P, _P, _P, _P, _P]),
     "tk_read_cov_stats_d": (C.c_int, [_P, _P, _P, _P, _P]),
+    "tk_select_pairs_d_noalloc": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_uint32, _P,
+                                           _U64P]),
     "tk_select_pairs_d": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_int, C.c_int, C.c_int,
                                     C.c_uint32, _P, _U64P]),
     "tk_select_pairs": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_int, C.c_int, C.c_int, _P, _U64P]),

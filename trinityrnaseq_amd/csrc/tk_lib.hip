@@ -2025,6 +2025,39 @@ using namespace tk;
 // =====================================================================
 // C ABI
 // =====================================================================
+namespace tk {
+// Device scratch of the allocation-free pair selection
+// (tk_select_pairs_d_noalloc): 256 bytes of flags, the consumes and idx words
+// of up to kNoAllocPairs pairs, and the scan's temporary storage.  Each GPU
+// the library runs on holds one copy until the process ends.  Host code never
+// uses the symbol's name as a device address: tk_init takes the device
+// address with hipGetSymbolAddress once per device (g_scratch_ptr).
+constexpr uint64_t kNoAllocPairs = 1ull << 24;
+constexpr size_t kNoAllocScanBytes = 4u << 20;
+constexpr size_t kNoAllocBytes = 256 + 2 * kNoAllocPairs * 4 + kNoAllocScanBytes;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];  // per device, set once with that device current
+
+static int scratch_setup(int device) {
+  if (device < 0 || device >= 64) return fail(TK_ERR_ARG, "device %d beyond the scratch table", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return TK_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr[device] = p;
+  return TK_OK;
+}
+
+static void* scratch_ptr(int device) {
+  if (device < 0 || device >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+}  // namespace tk
+
 extern "C" {
 
 const char* tk_last_error(void) { return tk::g_err.c_str(); }
@@ -2038,6 +2071,7 @@ int tk_init(int device, tk_ctx** out) {
   if (e != hipSuccess || n == 0) return fail(TK_ERR_HIP, "no HIP device available");
   if (device < 0 || device >= n) return fail(TK_ERR_ARG, "device %d out of range [0,%d)", device, n);
   TK_HIP(hipSetDevice(device));
+  if (int rc0 = scratch_setup(device)) return rc0;
   tk_ctx* c = new tk_ctx();
   c->device = device;
   hipDeviceProp_t prop;
@@ -3393,6 +3427,27 @@ int tk_select_pairs_d(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr,
   uint32_t* consumes = (uint32_t*)p;
   uint32_t* idx = consumes + ((n + 63) / 64) * 64;
   return select_run(c, dl, dr, n, max_cov, min_cov, dsel, consumes, idx, nullptr, 0, c->d_flags, n_selected);
+}
+
+int tk_select_pairs_d_noalloc(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint64_t n,
+                              int max_cov, int min_cov, int max_cv, uint32_t max_nwin, uint8_t* dsel,
+                              uint64_t* n_selected) {
+  int rc = select_check(c, dl, dr, n, max_cov, max_cv, max_nwin, dsel);
+  if (rc) return rc;
+  if (n > kNoAllocPairs)
+    return fail(TK_ERR_ARG, "tk_select_pairs_d_noalloc: %llu pairs exceed the %llu its preallocated scratch holds",
+                (unsigned long long)n, (unsigned long long)kNoAllocPairs);
+  if (set_device(c->device)) return TK_ERR_HIP;
+  if (n_selected) *n_selected = 0;
+  if (!n) return TK_OK;
+  uint8_t* base = (uint8_t*)scratch_ptr(c->device);
+  if (!base) return fail(TK_ERR_STATE, "tk_select_pairs_d_noalloc: no scratch address for device %d", c->device);
+  uint32_t* flags = (uint32_t*)base;
+  uint32_t* consumes = (uint32_t*)(base + 256);
+  uint32_t* idx = consumes + kNoAllocPairs;
+  void* temp = (void*)(idx + kNoAllocPairs);
+  return select_run(c, dl, dr, n, max_cov, min_cov, dsel, consumes, idx, temp, kNoAllocScanBytes, flags,
+                    n_selected);
 }
 
 int tk_select_pairs(tk_ctx* c, const tk_read_stat* l, const tk_read_stat* r, uint64_t n, int max_cov,
