@@ -378,6 +378,31 @@ def bench_stress(args) -> dict:
         log(f"host-input job {i}: {host_lat[-1]:.3f}s")
         if (kept_h, assigned_h, info_h["capacity"]) != (kept, assigned, info["capacity"]):
             raise RuntimeError("the host-input job's results differ from the device job's")
+    dump = None
+    if not args.no_dump_check:
+        # `jellyfish count` then `dump -L 1` of the whole table (Trinity:2612-2625): more keys
+        # than one device sort call takes, so the dump goes out in sorted key ranges
+        t = time.perf_counter()
+        reads = api.Reads.from_device(ctx, d_seq.ptr, d_off.ptr, n_reads)
+        table = api.Table(ctx, K, True, int(est * 1.05) + 4096, 0.5)
+        table.count(reads)
+        n_dump = table.dump_size()
+        hist = table.histo()
+        keys, counts = table.dump()
+        dump_s = time.perf_counter() - t
+        ok = len(keys) == n_dump == int(hist.sum())
+        step_ = 1 << 27
+        for a in range(0, len(keys), step_):
+            z = min(len(keys), a + step_ + 1)
+            ok = ok and bool(np.all(keys[a + 1:z] > keys[a:z - 1]))
+        dump = {"entries": int(n_dump), "sorted_and_complete": ok, "count_dump_histo_s": round(dump_s, 3),
+                "counts_sum": int(counts.sum(dtype=np.uint64))}
+        log(f"dump check: {dump}")
+        del keys, counts
+        table.free()
+        reads.free()
+        if not ok:
+            raise RuntimeError("the full dump is not complete and sorted")
     for b in (d_seq, d_off, d_stats):
         b.free()
     contigs.free()
@@ -408,7 +433,7 @@ def bench_stress(args) -> dict:
             "k-mers/s_p50": wins / percentiles(host_lat)["p50"] if host_lat else None,
             "input": "pageable host memory (staged through pinned buffers): upload + pack inside the job"}),
         "kernels": kern, "roofline": roofline, "cpu_baseline": cpu,
-        "distinct_estimate": est, "kept_after_prune": kept,
+        "distinct_estimate": est, "kept_after_prune": kept, "full_dump": dump,
         "table_capacity": info["capacity"], "reads_assigned": assigned,
     }
 
@@ -453,7 +478,7 @@ def cpu_baseline_stress(tx, n_pairs_sample: int) -> dict:
 def bench_inchworm(args) -> dict:
     """Inchworm's greedy contigs (tk_inchworm) over the dictionary of the
     small workload (500 transcripts, 200k 76-nt single-end reads, 0.5% errors)
-    and of a 2M-pair slice of the 50k-transcript stream, after Trinity's
+    and of a 500k-pair slice of the 50k-transcript stream, after Trinity's
     phase-1 prune (entropy >= 1.0); seeds need count >= 2 and entropy >= 1.5.
     Each run is checked against the oracle's serial greedy (which is also the
     single-thread CPU figure)."""
@@ -465,12 +490,12 @@ def bench_inchworm(args) -> dict:
         return synth.se_reads(synth.small_transcriptome(1001), 200_000, 76, seed=1001)
 
     def large():
-        n = 2_000_000
+        n = 500_000
         buf = np.empty(2 * n * READ_LEN, np.uint8)
         synth.pe_reads(synth.large_transcriptome(1002), n, READ_LEN, seed=1002, out=buf)
         return buf, np.arange(2 * n + 1, dtype=np.uint64) * np.uint64(READ_LEN)
 
-    cases = [("small_se_200k_reads", small), ("large_pe_2M_pairs", large)]
+    cases = [("small_se_200k_reads", small), ("large_pe_500k_pairs", large)]
     keep1 = api.entropy_keep_table(K, 1.0)
     keep15 = api.entropy_keep_table(K, 1.5)
     for name, gen in cases:
@@ -488,6 +513,7 @@ def bench_inchworm(args) -> dict:
             t = time.perf_counter()
             contigs, sums, seeds, info = api.inchworm(ctx, table, 2, keep15)
             lat.append(time.perf_counter() - t)
+        log(f"inchworm {name}: GPU p50 {percentiles(lat)['p50']:.3f}s, {len(keys)} k-mers; serial oracle next")
         t = time.perf_counter()
         oc, osum, oseeds = O.inchworm(keys, counts, K, True, 2, keep15)
         cpu_s = time.perf_counter() - t
@@ -504,7 +530,7 @@ def bench_inchworm(args) -> dict:
         table.free()
         reads.free()
     ctx.close()
-    first = out["large_pe_2M_pairs"]
+    first = out["large_pe_500k_pairs"]
     return {"metric": METRIC + " [inchworm contigs]", "value": first["gpu_kmers_per_s"], "unit": "contig k-mers/s",
             "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(1000 * first["gpu_s_p50"], 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
@@ -650,6 +676,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ceiling", action="store_true",
                     help="large_pe: skip the random-access ceiling microbenchmarks after the timed steps")
+    ap.add_argument("--no-dump-check", action="store_true",
+                    help="stress: skip the full sorted dump (dump -L 1) after the timed steps")
     ap.add_argument("--ceiling", action="store_true",
                     help="stress: run the random-access ceiling microbenchmarks over a buffer the size of "
                          "the count table after the timed steps")

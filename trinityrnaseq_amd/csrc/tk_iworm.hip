@@ -230,12 +230,25 @@ struct WalkParams {
   int canonical;
 };
 
+constexpr uint32_t kDeps = 8;  // recorded dependencies per walk (more: the walk is redone)
+
+// Record that walk w treated a k-mer claimed by earlier walk `dep` of this
+// round as used (the decision holds only if that walk commits a contig).
+__device__ __forceinline__ void iw_dep(uint32_t* __restrict__ deps, uint32_t* __restrict__ dep_n, uint32_t w,
+                                       uint32_t dep, uint32_t& last) {
+  if (dep == last) return;
+  last = dep;
+  const uint32_t j = atomicAdd(&dep_n[w], 1u);
+  if (j < kDeps) deps[w * kDeps + j] = dep;
+}
+
 // One four-lane group per walk: claim the seed, extend right, then left.
 __global__ __launch_bounds__(256) void k_iw_walk(WalkParams p, const uint32_t* __restrict__ picked_slot,
                                                  uint32_t* __restrict__ claim, uint32_t* __restrict__ lost,
                                                  uint32_t* __restrict__ status, uint32_t* __restrict__ lens,
                                                  uint64_t* __restrict__ sums, uint32_t* __restrict__ path_slot,
-                                                 uint8_t* __restrict__ path_base) {
+                                                 uint8_t* __restrict__ path_base, uint32_t* __restrict__ deps,
+                                                 uint32_t* __restrict__ dep_n) {
   const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
   if (w >= p.n) return;  // whole groups leave together
   const int lane = threadIdx.x & 63;
@@ -248,10 +261,15 @@ __global__ __launch_bounds__(256) void k_iw_walk(WalkParams p, const uint32_t* _
   const uint64_t seed_key = seed_word & (t.occ - 1);
   uint64_t sum = slot_count(t, seed_word);
   uint32_t st = kRunning;
+  uint32_t last_dep = 0xFFFFFFFFu;
   if (b == 0) {
     const uint32_t old = atomicMin(&claim[seed_slot], r);
-    if (old < r) st = kSkip;
-    else if (old != kFree && old != r) atomicOr(&lost[old - p.S - 1], 1u);
+    if (old < r) {
+      st = kSkip;
+      if (old > p.S) iw_dep(deps, dep_n, w, old - p.S - 1, last_dep);
+    } else if (old != kFree && old != r) {
+      atomicOr(&lost[old - p.S - 1], 1u);
+    }
   }
   st = __shfl(st, g0);
   uint32_t lens_side[2] = {0, 0};
@@ -269,6 +287,7 @@ __global__ __launch_bounds__(256) void k_iw_walk(WalkParams p, const uint32_t* _
       if (c > 0) {
         const uint32_t cl = __hip_atomic_load(&claim[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         avail = cl > r;  // free or held by a later walk
+        if (cl > p.S && cl < r) iw_dep(deps, dep_n, w, cl - p.S - 1, last_dep);
       }
       bool took = false;
       uint32_t best_b = 4;
@@ -293,7 +312,10 @@ __global__ __launch_bounds__(256) void k_iw_walk(WalkParams p, const uint32_t* _
           took = true;
           break;
         }
-        if ((uint32_t)b == best_b) avail = false;  // taken by an earlier walk (or our own)
+        if ((uint32_t)b == best_b) {  // taken by an earlier walk (or our own)
+          avail = false;
+          if (old > p.S && old < r) iw_dep(deps, dep_n, w, old - p.S - 1, last_dep);
+        }
       }
       if (!took) break;
       const uint32_t bslot = (uint32_t)__shfl((int)slot, g0 | (int)best_b);
@@ -324,17 +346,24 @@ __global__ __launch_bounds__(256) void k_iw_walk(WalkParams p, const uint32_t* _
   }
 }
 
-// Walks from `first` on give their claims back (seed and path).
-__global__ void k_iw_release(uint32_t first, uint32_t n, uint32_t S, uint32_t lmax,
-                             const uint32_t* __restrict__ picked_slot, const uint32_t* __restrict__ status,
-                             const uint32_t* __restrict__ lens, const uint32_t* __restrict__ path_slot,
-                             uint32_t* __restrict__ claim) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)(n - first) * 2;
+// The seed slot's final claim of every walk (its own rank unless an earlier
+// walk took the seed).
+__global__ void k_iw_owner(const uint32_t* __restrict__ picked_slot, uint32_t n, const uint32_t* __restrict__ claim,
+                           uint32_t* __restrict__ owner) {
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x)
+    owner[w] = claim[picked_slot[w]];
+}
+
+// Flagged walks give their claims back (seed and path).
+__global__ void k_iw_release(uint32_t n, uint32_t S, uint32_t lmax, const uint32_t* __restrict__ picked_slot,
+                             const uint32_t* __restrict__ release, const uint32_t* __restrict__ lens,
+                             const uint32_t* __restrict__ path_slot, uint32_t* __restrict__ claim) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * 2;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t w = first + (uint32_t)(i >> 1);
+    const uint32_t w = (uint32_t)(i >> 1);
     const int side = (int)(i & 1);
     const uint32_t r = S + 1 + w;
-    if (status[w] == kSkip) continue;
+    if (!release[w]) continue;
     if (side == 0) atomicCAS(&claim[picked_slot[w]], r, kFree);
     const uint32_t* ps = path_slot + ((uint64_t)w * 2 + side) * lmax;
     for (uint32_t j = 0; j < lens[2 * w + side]; j++) atomicCAS(&claim[ps[j]], r, kFree);
@@ -433,10 +462,13 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
   };
   uint32_t Bcap = cap_for(lmax);
   uint32_t B = std::min<uint32_t>(1024, Bcap);
-  DevBuf dpick, dpos, dout, dlost, dstat, dlens, dsums, dpslot, dpbase, dwalks, dwoff, dchars;
+  DevBuf dpick, dpos, dout, dlost, dstat, dlens, dsums, dpslot, dpbase, dwalks, dwoff, dchars, ddeps, ddepn,
+      downer, drel;
   if ((rc = dpick.alloc(Bmax * 4)) || (rc = dpos.alloc(Bmax * 8)) || (rc = dout.alloc(16)) ||
       (rc = dlost.alloc(Bmax * 4)) || (rc = dstat.alloc(Bmax * 4)) || (rc = dlens.alloc(Bmax * 8)) ||
-      (rc = dsums.alloc(Bmax * 8)) || (rc = dwalks.alloc(Bmax * 4)) || (rc = dwoff.alloc(Bmax * 8 + 8)))
+      (rc = dsums.alloc(Bmax * 8)) || (rc = dwalks.alloc(Bmax * 4)) || (rc = dwoff.alloc(Bmax * 8 + 8)) ||
+      (rc = ddeps.alloc((size_t)Bmax * kDeps * 4)) || (rc = ddepn.alloc(Bmax * 4)) || (rc = downer.alloc(Bmax * 4)) ||
+      (rc = drel.alloc(Bmax * 4)))
     return rc;
   auto alloc_paths = [&]() -> int {
     int r2 = dpslot.alloc((size_t)Bcap * 2 * lmax * 4);
@@ -444,7 +476,9 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
     return r2;
   };
   if ((rc = alloc_paths())) return rc;
-  std::vector<uint32_t> h_stat(Bmax), h_lost(Bmax), h_lens(2 * Bmax);
+  std::vector<uint32_t> h_stat(Bmax), h_lost(Bmax), h_lens(2 * Bmax), h_depn(Bmax), h_deps((size_t)Bmax * kDeps),
+      h_owner(Bmax), h_rel(Bmax);
+  std::vector<uint8_t> late(Bmax);
   std::vector<uint64_t> h_sums(Bmax), h_pos(Bmax), h_woff(Bmax + 1);
   std::vector<uint32_t> h_walks(Bmax);
   uint64_t cursor = 0;
@@ -466,6 +500,7 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
     res->rounds++;
     res->walks += n;
     IW_LAUNCH(c, "iworm_fill", cdiv(n, 256), 256, k_iw_fill, dlost.as<uint32_t>(), (uint64_t)n, 0u);
+    IW_LAUNCH(c, "iworm_fill", cdiv(n, 256), 256, k_iw_fill, ddepn.as<uint32_t>(), (uint64_t)n, 0u);
     WalkParams wp;
     wp.t = v;
     wp.kmask = k >= 32 ? ~0ull : ((1ull << (2 * k)) - 1);
@@ -476,25 +511,63 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
     wp.canonical = t->canonical;
     IW_LAUNCH(c, "iworm_walk", cdiv((uint64_t)n * 4, 256), 256, k_iw_walk, wp, dpick.as<uint32_t>(),
               dclaim.as<uint32_t>(), dlost.as<uint32_t>(), dstat.as<uint32_t>(), dlens.as<uint32_t>(),
-              dsums.as<uint64_t>(), dpslot.as<uint32_t>(), dpbase.as<uint8_t>());
+              dsums.as<uint64_t>(), dpslot.as<uint32_t>(), dpbase.as<uint8_t>(), ddeps.as<uint32_t>(),
+              ddepn.as<uint32_t>());
+    IW_LAUNCH(c, "iworm_owner", cdiv(n, 256), 256, k_iw_owner, dpick.as<uint32_t>(), n, dclaim.as<uint32_t>(),
+              downer.as<uint32_t>());
+    TK_HIP(hipMemcpyAsync(h_depn.data(), ddepn.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_deps.data(), ddeps.p, (size_t)n * kDeps * 4, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_owner.data(), downer.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipMemcpyAsync(h_stat.data(), dstat.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipMemcpyAsync(h_lost.data(), dlost.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipMemcpyAsync(h_lens.data(), dlens.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipMemcpyAsync(h_sums.data(), dsums.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipMemcpyAsync(h_pos.data(), dpos.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
-    // committed prefix: finished walks that lost nothing, and skipped seeds
+    // Committed prefix, in rank order.  A walk whose seed an earlier walk of
+    // the round took ("late skip") is the serial skip of that seed when the
+    // taker commits a contig; its own claims are void.  Any decision that
+    // treated a k-mer as used because an earlier walk of the round held it
+    // (the dependencies) holds only if that walk commits a contig, i.e. is
+    // not a late skip.  Seeds claimed at the start by an earlier walk are
+    // skips under the same condition.  The first walk that lost a k-mer
+    // without losing its seed, outgrew its buffers or recorded too many
+    // dependencies ends the prefix.
     uint32_t P = n;
     for (uint32_t i = 0; i < n; i++) {
-      if (h_stat[i] == kSkip) continue;
-      if (h_stat[i] == kDone && !h_lost[i]) continue;
+      late[i] = 0;
+      bool deps_ok = h_depn[i] <= kDeps;
+      for (uint32_t j = 0; deps_ok && j < h_depn[i]; j++) {
+        const uint32_t d = h_deps[(size_t)i * kDeps + j];
+        deps_ok = d < i && !late[d];
+      }
+      const uint32_t r = S + 1 + i;
+      if (h_stat[i] == kSkip) {
+        if (!deps_ok) { P = i; break; }
+        continue;
+      }
+      if (h_owner[i] != r) {  // seed taken during the round by walk h_owner[i] - S - 1 < i
+        const uint32_t o = h_owner[i] - S - 1;
+        if (h_owner[i] <= S || o >= i || late[o]) { P = i; break; }
+        late[i] = 1;
+        continue;
+      }
+      if (h_stat[i] == kDone && !h_lost[i] && deps_ok) continue;
       P = i;
       break;
     }
-    if (P < n)
-      IW_LAUNCH(c, "iworm_release", std::min<uint64_t>(cdiv((uint64_t)(n - P) * 2, 256), 4096), 256, k_iw_release,
-                P, n, S, lmax, dpick.as<uint32_t>(), dstat.as<uint32_t>(), dlens.as<uint32_t>(),
-                dpslot.as<uint32_t>(), dclaim.as<uint32_t>());
+    // release: late skips of the prefix and every walk past it
+    bool any_rel = false;
+    for (uint32_t i = 0; i < n; i++) {
+      h_rel[i] = (i >= P && h_stat[i] != kSkip) || (i < P && late[i]) ? 1u : 0u;
+      any_rel = any_rel || h_rel[i];
+    }
+    if (any_rel) {
+      TK_HIP(hipMemcpyAsync(drel.p, h_rel.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+      IW_LAUNCH(c, "iworm_release", std::min<uint64_t>(cdiv((uint64_t)n * 2, 256), 4096), 256, k_iw_release, n, S,
+                lmax, dpick.as<uint32_t>(), drel.as<uint32_t>(), dlens.as<uint32_t>(), dpslot.as<uint32_t>(),
+                dclaim.as<uint32_t>());
+    }
     if (P == 0) {
       // the first walk outgrew its path buffers: larger buffers, fewer walks
       if (h_stat[0] != kOverflow) return fail(TK_ERR_STATE, "inchworm: first walk of a round did not commit");
@@ -511,7 +584,7 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
     uint32_t nw = 0;
     uint64_t total = 0;
     for (uint32_t i = 0; i < P; i++) {
-      if (h_stat[i] != kDone) continue;
+      if (h_stat[i] != kDone || late[i]) continue;
       h_walks[nw] = i;
       h_woff[nw] = total;
       const uint64_t L = (uint64_t)h_lens[2 * i] + h_lens[2 * i + 1] + k;
