@@ -15,10 +15,15 @@
  *   selection     nbkc_normalize.pl:67-127 (srand 12345, rand(1) <= max_cov/median)
  *   R2T vote      ReadsToTranscripts -p P                  Trinity:2238-2250 ; Changelog.txt:467,871-873
  *
- * Parity pins (tests/test_oracle_*.py): brute-force string-based Python
+ * Parity pins (tests/test_oracle.py): brute-force string-based Python
  * counters/voters on small inputs, closed forms (canon(rc(w)) = canon(w),
  * sum of counts = valid windows, histogram sums), Perl's own rand() stream
- * (tests/golden/perl_rand_12345.txt), and the reference's sample reads.
+ * (tests/golden/perl_rand_12345.txt), golden vectors the reference's
+ * nbkc_merge_left_right_stats.pl / nbkc_normalize.pl wrote
+ * (tests/golden/nbkc/), the reference's kmer_histo witness files, and
+ * textbook mean / population stdev for the stats row.  The reference's
+ * sample reads (tests/golden/sample_data/) are run through the oracle and
+ * the GPU path side by side; they check the GPU, not the oracle.
  * Conventions the reference tree does not fix (median of an even-length
  * vector, N-containing windows, stdev form, R2T tie rule and pct format)
  * are parameters; their default settings are parity unpinned.

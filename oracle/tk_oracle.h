@@ -9,9 +9,19 @@
  * Every function is plain, single-threaded C99 over host arrays and cites
  * the reference call site / consumer whose behaviour it restates.  The tools
  * themselves (jellyfish 2.x, Inchworm's fastaToKmerCoverageStats and
- * inchworm, Chrysalis' ReadsToTranscripts) are absent from the reference
- * tree, so conventions the reference tree cannot fix are explicit
- * parameters (see the *_policy arguments).
+ * inchworm, Chrysalis' ReadsToTranscripts, GraphFromFasta, QuantifyGraph)
+ * are absent from the reference tree, so conventions the reference tree
+ * cannot fix are explicit parameters (see the *_policy arguments) or stated
+ * at the function.  Pins (tests/test_oracle.py): string brute force for
+ * counting, histo, prune, stats, R2T, Inchworm and the k-mer set counts;
+ * Perl's own rand() and golden vectors written by the reference's Perl
+ * scripts for the selection; the reference's histogram files for histo's
+ * default shape; textbook mean / population stdev for the stats row.
+ * Parity unpinned (no reference binary or test fixes them): the stats
+ * tool's columns and number format, even-length median and N-window
+ * conventions; R2T's pct format and tie rule; or_inchworm's seed and base
+ * tie orders, seed thresholds and the cov rounding of its callers; the weld
+ * rule GraphFromFasta applies on top of or_kset_count.
  */
 #ifndef TK_ORACLE_H
 #define TK_ORACLE_H
