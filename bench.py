@@ -56,8 +56,11 @@ VOTE_B_PER_WIN, VOTE_B_PER_READ = 64.375, 24.0
 # process, so the profiled figures are carried here for roofline.traffic.
 TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (211.8 + 1.3) / 5.04}
 # The same for the stressed sample's kernels (37.8 G windows per step), from
-# profiles/stress_150M_pairs_trace.md; filled from that profile's counter runs.
-STRESS_TRAFFIC_B_PER_WIN: dict = {}
+# profiles/stress_150M_pairs_trace.md: FETCH_SIZE + WRITE_SIZE of one job at
+# face value (their random 64-B line reads read as 64 B each, atomics as 32 B
+# of WRITE_SIZE).
+STRESS_TRAFFIC_B_PER_WIN = {"count": (880.30 + 574.82) / 37.8, "stats": (2483.85 + 9.63) / 37.8,
+                            "vote": (2243.57 + 129.77) / 37.8}
 
 
 def log(*a):
@@ -86,7 +89,7 @@ def cpu_baseline(tx, n_pairs_sample: int, max_cov: int) -> dict:
 
 
 def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict, steps: int,
-                          ops: int | None = None) -> dict:
+                          n_ops: int | None = None) -> dict:
     """Rates of random 64-byte line reads and random 8-byte atomics (add, CAS,
     workgroup-scope add) over a buffer the size of the step's count table and
     over smaller ones (tk_micro.hip), the XCD each workgroup runs on, and the
@@ -110,7 +113,7 @@ def random_access_ceiling(ctx, table_bytes: int, wins_per_step: int, kern: dict,
         best = 0.0
         for _ in range(3):
             ms, done = ctypes.c_float(), ctypes.c_uint64()
-            rc = lib.tkm_run(kind, ctypes.c_void_p(buf.ptr), nb, ops or wins_per_step, ctypes.byref(ms),
+            rc = lib.tkm_run(kind, ctypes.c_void_p(buf.ptr), nb, n_ops or wins_per_step, ctypes.byref(ms),
                              ctypes.byref(done))
             if rc:
                 raise RuntimeError(f"tkm_run({name}) failed: {rc}")
@@ -298,10 +301,17 @@ def bench_stress(args) -> dict:
     d_seq.upload(host)
     d_off.upload(off)
     d_stats = api.DeviceBuffer(ctx, n_reads * api.READ_STAT_DTYPE.itemsize)
+    d_votes = api.DeviceBuffer(ctx, n_reads * api.R2T_VOTE_DTYPE.itemsize)
     keep = api.entropy_keep_table(K, 1.5)
     contigs = api.Reads(ctx, cseq, coff)
     wins = n_reads * (READ_LEN - K + 1)
     log("inputs resident in HBM")
+
+    def assigned_reads() -> int:
+        """Reads the vote assigned to a component (outside the timed region)."""
+        v = np.empty(n_reads, api.R2T_VOTE_DTYPE)
+        d_votes.download(v)
+        return int((v["comp"] != 0xFFFFFFFF).sum())
 
     def step(reads_from_host: bool):
         if reads_from_host:
@@ -314,31 +324,33 @@ def bench_stress(args) -> dict:
         api.read_cov_stats_d(ctx, table, reads, d_stats.ptr, min_count=2)
         kept = table.prune(2, keep)  # Inchworm's dictionary, after the stats used the full counts
         ct = api.CTable(ctx, contigs, comps, K, True)
-        votes = ct.vote(reads)
+        ct.vote_d(reads, d_votes.ptr)  # results stay in HBM, like the stats
+        ctx.sync()
         info = table.info()
         ct.free()
         table.free()
         reads.free()
-        return est, kept, info, int((votes["comp"] != 0xFFFFFFFF).sum())
+        return est, kept, info
 
     # one job after another in one process: each job's table and read set take the blocks the
     # previous job freed (no hipFree + hipMalloc of a 65 GB table per job)
     api.thread_cache(200 << 30)
     for i in range(args.warmup):
         t = time.perf_counter()
-        est, kept, info, assigned = step(False)
+        est, kept, info = step(False)
         log(f"warmup {i}: {time.perf_counter() - t:.3f}s est_distinct={est:.3e} kept={kept} "
-            f"capacity={info['capacity']} assigned={assigned}")
+            f"capacity={info['capacity']}")
     ctx.timing(True)
     ctx.timing_reset()
     lat = []
     ctx.sync()
     for i in range(args.steps):
         t = time.perf_counter()
-        est, kept, info, assigned = step(False)
+        est, kept, info = step(False)
         lat.append(time.perf_counter() - t)
         log(f"step {i}: {lat[-1]:.3f}s")
     ctx.sync()
+    assigned = assigned_reads()
     kern = {}
     for name in ["pack", "read_words", "hll", "sort_pairs", "count", "count_merge", "count_hist", "count_scatter",
                  "count_regions", "stats", "prune", "ctable_build", "vote", "ovf_fold"]:
@@ -373,10 +385,10 @@ def bench_stress(args) -> dict:
     host_lat = []
     for i in range(args.host_steps):
         t = time.perf_counter()
-        est_h, kept_h, info_h, assigned_h = step(True)
+        est_h, kept_h, info_h = step(True)
         host_lat.append(time.perf_counter() - t)
         log(f"host-input job {i}: {host_lat[-1]:.3f}s")
-        if (kept_h, assigned_h, info_h["capacity"]) != (kept, assigned, info["capacity"]):
+        if (kept_h, assigned_reads(), info_h["capacity"]) != (kept, assigned, info["capacity"]):
             raise RuntimeError("the host-input job's results differ from the device job's")
     dump = None
     if not args.no_dump_check:
@@ -403,14 +415,14 @@ def bench_stress(args) -> dict:
         reads.free()
         if not ok:
             raise RuntimeError("the full dump is not complete and sorted")
-    for b in (d_seq, d_off, d_stats):
+    for b in (d_seq, d_off, d_stats, d_votes):
         b.free()
     contigs.free()
     api.thread_cache(0)
     if args.ceiling:
         # random 64-B line reads / atomics over a buffer the size of this job's count table
         roofline["ceiling"] = random_access_ceiling(ctx, info["capacity"] * 8, wins, kern, args.steps,
-                                                    ops=wins // 8)
+                                                    n_ops=wins // 8)
     ctx.close()
     cpu = None
     if not args.no_cpu:

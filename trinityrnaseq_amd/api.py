@@ -521,6 +521,10 @@ class CTable:
         _check(lib().tk_r2t_vote_reads(self.ctx.h, self.h, reads.h, _ptr(out)), "tk_r2t_vote_reads")
         return out
 
+    def vote_d(self, reads: Reads, d_out: int) -> None:
+        """Votes into a device buffer of 12 bytes per read (asynchronous)."""
+        _check(lib().tk_r2t_vote_reads_d(self.ctx.h, self.h, reads.h, C.c_void_p(d_out)), "tk_r2t_vote_reads_d")
+
     def free(self):
         if self.h:
             lib().tk_ctable_free(self.h)

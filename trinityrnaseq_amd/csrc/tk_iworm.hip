@@ -279,21 +279,24 @@ __global__ __launch_bounds__(256) void k_iw_walk(WalkParams p, const uint32_t* _
     uint32_t* pslot = path_slot + ((uint64_t)w * 2 + side) * p.lmax;
     uint8_t* pbase = path_base + ((uint64_t)w * 2 + side) * p.lmax;
     for (;;) {
+      // this walk's lost flag, loaded first so that its latency hides behind the lookup
+      uint32_t l = 0;
+      if (b == 0) l = __hip_atomic_load(&lost[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint64_t y = side == 0 ? (((x << 2) | (uint64_t)b) & p.kmask)
                                    : (((uint64_t)b << (2 * (p.k - 1))) | (x >> 2));
       uint32_t slot;
       const uint64_t c = table_find(t, canon_key(y, p.k, p.canonical), &slot);
-      bool avail = false;
-      if (c > 0) {
-        const uint32_t cl = __hip_atomic_load(&claim[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        avail = cl > r;  // free or held by a later walk
-        if (cl > p.S && cl < r) iw_dep(deps, dep_n, w, cl - p.S - 1, last_dep);
-      }
+      // Candidates are tried by count (then base) with the claim itself: a
+      // claim that returns an earlier rank (committed, this walk's own, or an
+      // earlier walk of the round) drops the candidate, so the k-mer taken is
+      // the best one no earlier walk holds -- one round trip per step instead
+      // of reading the claim word first.
+      bool avail = c > 0;
       bool took = false;
       uint32_t best_b = 4;
       uint64_t best_c = 0;
       for (int attempt = 0; attempt < 4; attempt++) {
-        // best available candidate of the group: count, then smallest base
+        // best remaining candidate of the group: count, then smallest base
         uint64_t key = avail ? (((c < (1ull << 61) ? c : (1ull << 61)) << 2) | (uint64_t)(3 - b)) : 0;
         key = std::max(key, shfl_xor_u64(key, 1));
         key = std::max(key, shfl_xor_u64(key, 2));
@@ -312,7 +315,7 @@ __global__ __launch_bounds__(256) void k_iw_walk(WalkParams p, const uint32_t* _
           took = true;
           break;
         }
-        if ((uint32_t)b == best_b) {  // taken by an earlier walk (or our own)
+        if ((uint32_t)b == best_b) {  // held by an earlier rank (or this walk)
           avail = false;
           if (old > p.S && old < r) iw_dep(deps, dep_n, w, old - p.S - 1, last_dep);
         }
@@ -328,8 +331,6 @@ __global__ __launch_bounds__(256) void k_iw_walk(WalkParams p, const uint32_t* _
       sum += best_c;
       x = side == 0 ? (((x << 2) | (uint64_t)best_b) & p.kmask)
                     : (((uint64_t)best_b << (2 * (p.k - 1))) | (x >> 2));
-      uint32_t l = 0;
-      if (b == 0) l = __hip_atomic_load(&lost[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       l = __shfl(l, g0);
       if (l) {
         st = kLost;
