@@ -595,6 +595,8 @@ def bench_serving(args) -> dict:
         synth.pe_reads(tx, int(n), READ_LEN, seed=2000 + j, out=buf)
         off = np.arange(2 * int(n) + 1, dtype=np.uint64) * np.uint64(READ_LEN)
         samples.append((buf, off, int(n), pinned))
+        if (j + 1) % 16 == 0:
+            log(f"generated {j + 1} of {args.samples} samples")
     total_pairs = int(sizes.sum())
     log(f"generated {args.samples} samples, {total_pairs} pairs in {time.perf_counter() - t:.1f}s")
     # plain per-job GPU form, one sample at a time; like the server's workers it keeps the
@@ -676,6 +678,190 @@ def bench_serving(args) -> dict:
     }
 
 
+def _write_fastq(path: str, reads: np.ndarray, tag: bytes, mate: int) -> None:
+    """Reads (n x READ_LEN ASCII) as FASTQ records "@<tag><i:09d>/<mate>", quality 'I'."""
+    n = reads.shape[0]
+    head = np.frombuffer(b"@" + tag, np.uint8)
+    h = len(head)
+    width = h + 9 + 3 + READ_LEN + 3 + READ_LEN + 1
+    rec = np.empty((n, width), np.uint8)
+    rec[:, :h] = head
+    x = np.arange(n, dtype=np.int64)
+    for d in range(8, -1, -1):
+        rec[:, h + d] = 48 + (x % 10)
+        x //= 10
+    p = h + 9
+    rec[:, p:p + 3] = np.frombuffer(b"/%d\n" % mate, np.uint8)
+    p += 3
+    rec[:, p:p + READ_LEN] = reads
+    p += READ_LEN
+    rec[:, p:p + 3] = np.frombuffer(b"\n+\n", np.uint8)
+    p += 3
+    rec[:, p:p + READ_LEN] = ord("I")
+    rec[:, -1] = ord("\n")
+    with open(path, "wb") as f:
+        f.write(rec.tobytes())
+
+
+def bench_cli(args) -> dict:
+    """The per-sample normalization as the Perl driver runs it, from FASTQ
+    files to normalized FASTQ files (util/insilico_read_normalization.pl
+    --pairs_together): (a) tool by tool, one process per step, the script's
+    own commands with this build's binaries; (b) bin/insilico_read_normalization,
+    the whole job in one process; (c) a stream of samples through the resident
+    service with concurrent clients against (b) run one sample after another.
+    Every output pair is compared with (a)'s / (b)'s."""
+    import shutil
+    import subprocess
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+    from trinityrnaseq_amd import synth
+    bin_ = os.path.join(os.path.dirname(os.path.abspath(__file__)), "trinityrnaseq_amd", "bin")
+    tool = lambda name: os.path.join(bin_, name)  # noqa: E731
+    work = tempfile.mkdtemp(prefix="tk_cli_")
+    env = dict(os.environ, LC_ALL="C")
+
+    def sh(cmd: str, cwd: str) -> None:
+        r = subprocess.run(["bash", "-c", cmd], cwd=cwd, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+        if r.returncode:
+            raise RuntimeError(f"{cmd} -> {r.returncode}: {r.stderr.decode()[-2000:]}")
+
+    def sample(j: int, n_pairs: int) -> tuple[str, str]:
+        tx = synth.large_transcriptome(3000 + j)
+        buf = np.empty(2 * n_pairs * READ_LEN, np.uint8)
+        synth.pe_reads(tx, n_pairs, READ_LEN, seed=3000 + j, out=buf)
+        r = buf.reshape(2 * n_pairs, READ_LEN)
+        lp, rp = os.path.join(work, f"s{j}_1.fq"), os.path.join(work, f"s{j}_2.fq")
+        _write_fastq(lp, r[:n_pairs], b"s%d_" % j, 1)
+        _write_fastq(rp, r[n_pairs:], b"s%d_" % j, 2)
+        return lp, rp
+
+    irn_opts = "--seqType fq --JM 20G --max_cov {c} --min_cov 1 --CPU 16 --max_CV 10000 --pairs_together " \
+               "--PARALLEL_STATS".format(c=args.max_cov)
+    try:
+        t = time.perf_counter()
+        L, R = sample(0, args.cli_pairs)
+        log(f"cli: wrote {args.cli_pairs} pairs of FASTQ in {time.perf_counter() - t:.1f}s")
+        # (a) tool by tool (the script's commands, :274-452)
+        da = os.path.join(work, "chain")
+        os.makedirs(da)
+        steps = [
+            ("seqtk-trinity", f"{tool('seqtk-trinity')} seq -A -R 1 {L} >> left.fa & "
+                              f"{tool('seqtk-trinity')} seq -A -R 2 {R} >> right.fa & wait"),
+            ("cat", "cat left.fa right.fa > both.fa"),
+            ("jellyfish count", f"{tool('jellyfish')} count -t 16 -m 25 -s 100000000 --canonical both.fa"),
+            ("jellyfish histo", f"{tool('jellyfish')} histo -t 16 -o jellyfish.K25.min2.kmers.fa.histo mer_counts.jf"),
+            ("jellyfish dump", f"{tool('jellyfish')} dump -L 2 mer_counts.jf > jellyfish.K25.min2.kmers.fa"),
+            ("fastaToKmerCoverageStats", " & ".join(
+                f"{tool('fastaToKmerCoverageStats')} --reads {x}.fa --kmers jellyfish.K25.min2.kmers.fa "
+                f"--kmer_size 25 --num_threads 8 --DS > {x}.fa.K25.stats" for x in ("left", "right")) + " & wait"),
+            ("sort", " & ".join(
+                f"(head -n1 {x}.fa.K25.stats > {x}.fa.K25.stats.sort && tail -n +2 {x}.fa.K25.stats | "
+                f"sort --parallel=8 -k1,1 -T . -S 8G >> {x}.fa.K25.stats.sort)" for x in ("left", "right"))
+             + " & wait"),
+            ("nbkc_normalize", f"{tool('nbkc_normalize')} --left left.fa.K25.stats.sort --right "
+                               f"right.fa.K25.stats.sort --sorted --max_cov {args.max_cov} --min_cov 1 "
+                               f"--max_CV 10000 --merged_out pairs.K25.stats > pairs.accs"),
+            ("extract", " & ".join(
+                f"{tool('extract_normalized_reads')} --seqType fq --selected pairs.accs --output {x}.norm.fq {f}"
+                for x, f in (("left", L), ("right", R))) + " & wait"),
+        ]
+        chain = {}
+        t0 = time.perf_counter()
+        for name, cmd in steps:
+            t = time.perf_counter()
+            sh(cmd, da)
+            chain[name] = round(time.perf_counter() - t, 3)
+            log(f"cli chain: {name} {chain[name]:.2f}s")
+        chain_s = time.perf_counter() - t0
+        # (b) one process
+        db = os.path.join(work, "one")
+        t = time.perf_counter()
+        sh(f"{tool('insilico_read_normalization')} {irn_opts} --left {L} --right {R} --output {db}", work)
+        one_s = time.perf_counter() - t
+        log(f"cli one process: {one_s:.2f}s (chain {chain_s:.2f}s)")
+        tag = f"normalized_K25_maxC{args.max_cov}_minC1_maxCV10000.fq"
+        same = all(open(os.path.join(da, f"{x}.norm.fq"), "rb").read() ==
+                   open(os.path.join(db, f"s0_{m}.fq.{tag}"), "rb").read() for x, m in (("left", 1), ("right", 2)))
+        if not same:
+            raise RuntimeError("the one-process outputs differ from the tool-by-tool chain's")
+        n_sel = sum(1 for _ in open(os.path.join(da, "pairs.accs")))
+        for f in (L, R):
+            os.remove(f)
+        shutil.rmtree(da)
+        shutil.rmtree(db)
+        # (c) a stream of samples: one process per sample, one after another, vs the service
+        S, P = args.cli_samples, args.cli_stream_pairs
+        files = []
+        t = time.perf_counter()
+        for j in range(S):
+            files.append(sample(1 + j, P))
+        log(f"cli: wrote {S} samples of {P} pairs in {time.perf_counter() - t:.1f}s")
+        lat_seq = []
+        t0 = time.perf_counter()
+        for j, (l_, r_) in enumerate(files):
+            t = time.perf_counter()
+            sh(f"{tool('insilico_read_normalization')} {irn_opts} --left {l_} --right {r_} "
+               f"--output {work}/seq{j}", work)
+            lat_seq.append(time.perf_counter() - t)
+        seq_s = time.perf_counter() - t0
+        log(f"cli stream, one process per sample: {seq_s:.2f}s")
+        sock = os.path.join(work, "norm.sock")
+        srv = subprocess.Popen([tool("insilico_read_normalization"), "--serve", sock, "--slots", str(args.slots),
+                                "--hbm_budget_gb", str(args.hbm_budget_gb)], stdout=subprocess.PIPE,
+                               stderr=subprocess.PIPE)
+        try:
+            t = time.perf_counter()
+            while not os.path.exists(sock):
+                if srv.poll() is not None or time.perf_counter() - t > 120:
+                    raise RuntimeError("the normalization service did not start")
+                time.sleep(0.1)
+            start_s = time.perf_counter() - t
+
+            def client(j):
+                t1 = time.perf_counter()
+                l_, r_ = files[j]
+                sh(f"{tool('insilico_read_normalization')} --server {sock} {irn_opts} --left {l_} --right {r_} "
+                   f"--output {work}/srv{j}", work)
+                return time.perf_counter() - t1
+
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(args.cli_clients) as ex:
+                lat_srv = list(ex.map(client, range(S)))
+            srv_s = time.perf_counter() - t0
+            sh(f"{tool('insilico_read_normalization')} --server {sock} --shutdown", work)
+            srv.wait(timeout=120)
+        finally:
+            if srv.poll() is None:
+                srv.kill()
+                srv.wait()
+        log(f"cli stream through the service: {srv_s:.2f}s")
+        for j in range(S):
+            for m in (1, 2):
+                a_ = open(os.path.join(work, f"seq{j}", f"s{1 + j}_{m}.fq.{tag}"), "rb").read()
+                b_ = open(os.path.join(work, f"srv{j}", f"s{1 + j}_{m}.fq.{tag}"), "rb").read()
+                if a_ != b_:
+                    raise RuntimeError(f"sample {j}: the service's output differs from the one-process run's")
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return {
+        "metric": METRIC + " [CLI: FASTQ files to normalized FASTQ]", "value": 2 * args.cli_pairs / one_s,
+        "unit": "reads/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(one_s * 1000, 3),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic FASTQ files written to local disk before each timed run",
+        "config": {"workload": f"insilico_read_normalization --pairs_together --max_cov {args.max_cov} on "
+                               f"{args.cli_pairs} 2x150 pairs (one sample); stream: {S} samples of {P} pairs",
+                   "slots": args.slots, "clients": args.cli_clients},
+        "one_process_s": round(one_s, 3), "chain_s": round(chain_s, 3), "chain_steps_s": chain,
+        "one_process_over_chain": chain_s / one_s, "selected_pairs": n_sel,
+        "outputs_match": "one-process FASTQ == tool-by-tool FASTQ; service FASTQ == one-process FASTQ",
+        "stream": {"samples": S, "pairs_each": P, "one_process_each_s": round(seq_s, 3),
+                   "service_s": round(srv_s, 3), "service_over_one_process_each": seq_s / srv_s,
+                   "service_start_s": round(start_s, 3),
+                   "one_process_latency": percentiles(lat_seq), "service_latency": percentiles(lat_srv)},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -695,7 +881,7 @@ def main() -> int:
                          "the count table after the timed steps")
     ap.add_argument("--copy-ref-gb", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GB (HBM counter reference)")
-    ap.add_argument("--workload", choices=["stress", "large_pe", "components", "serving", "inchworm"],
+    ap.add_argument("--workload", choices=["stress", "large_pe", "components", "serving", "inchworm", "cli"],
                     default="stress")
     ap.add_argument("--host-steps", type=int, default=2,
                     help="stress: jobs run from pageable host memory after the timed steps")
@@ -710,6 +896,10 @@ def main() -> int:
     ap.add_argument("--serve-pinned", action="store_true",
                     help="serving: hold the samples in page-locked host memory")
     ap.add_argument("--hbm-budget-gb", type=float, default=96.0)
+    ap.add_argument("--cli-pairs", type=int, default=2_000_000, help="cli: pairs of the one-sample comparison")
+    ap.add_argument("--cli-samples", type=int, default=8, help="cli: samples of the stream")
+    ap.add_argument("--cli-stream-pairs", type=int, default=1_000_000, help="cli: pairs per stream sample")
+    ap.add_argument("--cli-clients", type=int, default=4, help="cli: concurrent clients of the service")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -726,6 +916,9 @@ def main() -> int:
         return 0
     if args.workload == "inchworm":
         print(json.dumps(bench_inchworm(args)), flush=True)
+        return 0
+    if args.workload == "cli":
+        print(json.dumps(bench_cli(args)), flush=True)
         return 0
 
     from trinityrnaseq_amd import api, synth

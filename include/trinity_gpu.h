@@ -386,6 +386,42 @@ int tk_server_create(const tk_server_opts* opts, tk_server** out);
  * pairs nbkc_normalize.pl keeps.  Buffers must stay valid until tk_server_wait. */
 int tk_server_submit(tk_server* s, const char* seq, const uint64_t* offsets, uint64_t n_pairs,
                      uint8_t* selected, uint64_t* job_id);
+/*
+ * The job of insilico_read_normalization.pl in general form
+ * (util/insilico_read_normalization.pl:270-464): every read of seq/offsets
+ * (the both.fa or single.fa order) is counted (canonical unless
+ * strand-specific: :617-622) and gets its coverage statistics
+ * (fastaToKmerCoverageStats: :837-845); then one or two selections.  Unit u
+ * of a selection is the pair (left[u], right[u]) of read indices, in the
+ * merged-row order of nbkc_merge_left_right_stats.pl --sorted
+ * (--pairs_together: :951-987), or the single read left[u] when right is
+ * NULL (nbkc_normalize.pl on one stats file: single reads, and each side of
+ * paired reads normalized separately: :917-944).  Parameters are per job, so
+ * one server takes jobs of any settings; its opts give min_count only.  The
+ * job struct and every buffer it names stay valid until tk_server_wait;
+ * n_selected is set when the job ends.
+ */
+typedef struct {
+    const uint64_t* left;
+    const uint64_t* right;     /* NULL: single-read units                      */
+    uint64_t n_units;
+    uint8_t* selected;         /* n_units bytes: 1 = kept                      */
+    uint64_t n_selected;       /* out                                          */
+} tk_server_selection;
+
+typedef struct {
+    const char* seq;
+    const uint64_t* offsets;   /* n_reads + 1                                  */
+    uint64_t n_reads;
+    int32_t k;                 /* --KMER_SIZE (<= 31)                          */
+    int32_t canonical;         /* 0 with --SS_lib_type                         */
+    int32_t max_cov;           /* nbkc_normalize.pl --max_cov                  */
+    int32_t min_cov;           /* --min_cov                                    */
+    int32_t max_cv;            /* --max_CV                                     */
+    int32_t n_selections;      /* 1 or 2                                       */
+    tk_server_selection sel[2];
+} tk_server_job;
+int tk_server_submit_job(tk_server* s, tk_server_job* job, uint64_t* job_id);
 int tk_server_wait(tk_server* s, uint64_t job_id, tk_job_result* result);
 int tk_server_free(tk_server* s);
 

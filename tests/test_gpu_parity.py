@@ -169,6 +169,35 @@ def test_count_resident_crowded_regions(api, ctx, region_kb, tile, monkeypatch):
     assert np.array_equal(gk, ok) and np.array_equal(gc, 3 * oc)
 
 
+@pytest.mark.parametrize("mode", ["reads", "words"])
+def test_count_resident_in_chunks(api, ctx, mode, monkeypatch):
+    """Region-resident count in read chunks (the form of counts whose word
+    streams do not fit beside the table): each chunk merges, sorts and adds
+    its words onto the regions the earlier chunks wrote.  "reads": chunks of
+    2,500 reads, every word in the streams; "words": streams of 200,000
+    words, shorter than a chunk's words, so flushes past their end are added
+    with table atomics.  A second count adds onto the non-empty table."""
+    say("count_resident_in_chunks", mode)
+    monkeypatch.setenv("TK_PARTITION", "0")
+    monkeypatch.setenv("TK_ORDER", "1")
+    monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
+    monkeypatch.setenv("TK_RESIDENT", "1")
+    monkeypatch.setenv("TK_REGION_KB", "4")
+    if mode == "reads":
+        monkeypatch.setenv("TK_RES_CHUNK_READS", "2500")
+    else:
+        monkeypatch.setenv("TK_RES_STREAM_WORDS", "200000")
+    seqs = random_reads(17, 20000, 100, 150, n_rate=0.001) + repetitive_reads(25, 3000)
+    r, t, seq, off = _count(api, ctx, seqs, 25, True)
+    ok, oc = O.count(seq, off, 25, True)
+    assert ctx.count_words() > 0
+    gk, gc = t.dump()
+    assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+    t.count(r)
+    gk, gc = t.dump()
+    assert np.array_equal(gk, ok) and np.array_equal(gc, 2 * oc)
+
+
 @pytest.mark.parametrize("wpr", ["1", "2"])
 def test_count_resident_merge_windows_per_round(api, ctx, wpr, monkeypatch):
     """The region-resident merge with one or two windows per thread between
@@ -740,6 +769,51 @@ def _plain_job(api, ctx, seq, off, n_pairs, max_cov=50):
     st = api.read_cov_stats(ctx, t, r, min_count=2)
     sel, ns = api.select_pairs(ctx, st[:n_pairs], st[n_pairs:], max_cov, 1, 10000)
     return sel, ns
+
+
+def test_server_general_job_selections(api, ctx):
+    """tk_server_submit_job: all reads counted and scored (strand-specific
+    here, plus unpaired reads that count but are never selected), then a pair
+    selection over units given as read indices in a shuffled order and a
+    single-read selection; each equals tk_select_pairs on the stats of the
+    same job run alone, gathered in the same order."""
+    say("server_general_job")
+    from trinityrnaseq_amd import synth
+    tx = synth.large_transcriptome(3100, 800)
+    n = 40000
+    buf = np.empty(2 * n * 150, np.uint8)
+    synth.pe_reads(tx, n, 150, seed=3100, out=buf)
+    extra, _ = synth.se_reads(tx, 3000, 120, seed=3101)
+    seq = np.concatenate([buf, extra])
+    off = np.concatenate([np.arange(2 * n + 1, dtype=np.uint64) * np.uint64(150),
+                          np.uint64(2 * n * 150) + np.arange(1, 3001, dtype=np.uint64) * np.uint64(120)])
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(n - 100)  # 100 pairs never selected
+    left, right = perm.astype(np.uint64), (perm + n).astype(np.uint64)
+    singles = rng.permutation(2 * n + 3000)[:50000].astype(np.uint64)
+    r = api.Reads(ctx, seq.tobytes(), off)
+    t = api.Table(ctx, 25, False, int(api.estimate_distinct(ctx, r, 25, False) * 1.05) + 4096)
+    t.count(r)
+    st = api.read_cov_stats(ctx, t, r, min_count=2)
+    want_p, nw_p = api.select_pairs(ctx, st[left], st[right], 30, 1, 10000)
+    want_s, nw_s = api.select_pairs(ctx, st[singles], st[singles], 20, 1, 10000)
+    t.free()
+    r.free()
+    srv = api.Server(slots=2, hbm_budget_bytes=1 << 30)
+    try:
+        a = srv.submit_job(seq, off, [(left, right)], canonical=False, max_cov=30)
+        b = srv.submit_job(seq, off, [(singles, None), (left, right)], canonical=False, max_cov=20)
+        res, (sel,) = srv.wait(a)
+        assert res.n_selected == nw_p and np.array_equal(sel, want_p)
+        res, (sel_s, sel_p2) = srv.wait(b)
+        assert np.array_equal(sel_s, want_s)
+        want_p2, _ = api.select_pairs(ctx, st[left], st[right], 20, 1, 10000)
+        assert np.array_equal(sel_p2, want_p2)
+        assert res.n_selected == nw_s + int(want_p2.sum())
+        with pytest.raises(api.TkError):  # a unit index past the reads
+            srv.submit_job(seq, off, [(np.array([len(off) - 1], np.uint64), None)])
+    finally:
+        srv.close()
 
 
 def test_server_results_equal_alone_and_in_the_mix(api, ctx):

@@ -408,6 +408,55 @@ def test_extract_normalized_reads_fasta(tmp_path):
     assert out.read_bytes() == b">a/1 first read\nACGTACGT\n>c\nTTTT\n>d/1\n\n"
 
 
+def _rc_iupac(s: str) -> str:
+    """seqtk-trinity -r / revcomp_fasta.pl: reverse, IUPAC complement."""
+    return s.translate(str.maketrans("ACGTUMRWSYKVHDBNacgtumrwsykvhdbn", "TGCAAKYWSRMBDHVNtgcaakywsrmbdhvn"))[::-1]
+
+
+def _oracle_normalization(mode: str, max_cov: int, min_cov: int = 0, max_cv: int = 10000, ss: str = ""):
+    """insilico_read_normalization.pl on the reference's sample reads, step by
+    step through the oracle: mode "pairs" (--pairs_together), "separate"
+    (each side selected on its own, :917-944) or "single" (the left reads as
+    single-end).  Returns the expected normalized FASTQ bytes per side and
+    the selected core names per side."""
+    from data import sample_fasta
+    sides = ("left",) if mode == "single" else ("left", "right")
+    canonical = not ss
+    kind = {"left": ss[:1], "right": ss[1:2]} if mode != "single" else {"left": ""}
+    names, seqs = {}, {}
+    for side in sides:
+        n, q = sample_fasta(side)
+        names[side], seqs[side] = n, [_rc_iupac(x) for x in q] if kind[side] == "R" else q
+    seq, off = concat(sum((seqs[x] for x in sides), []))
+    keys, counts = O.count(seq, off, 25, canonical)
+    m = counts >= 2  # dump -L 2
+    rows = {}
+    for side in sides:
+        s1, o1 = concat(seqs[side])
+        st = O.read_stats(s1, o1, keys[m], counts[m], 25, canonical, 1, 1, 0, 0)
+        rows[side] = sorted(((n, O.format_stats_row(n, x).split("\t")) for n, x in zip(names[side], st)),
+                            key=lambda t: t[0].encode())  # sort -k1,1 in the C locale
+    f = lambda rs, i: [float(r[1][i]) for r in rs]  # noqa: E731
+    wanted = {}
+    if mode == "pairs":
+        L, R = rows["left"], rows["right"]
+        assert [a[:-2] for a, _ in L] == [b[:-2] for b, _ in R]  # the sample is fully paired
+        sel = O.select_pairs(f(L, 1), f(R, 1), f(L, 2), f(R, 2), f(L, 3), f(R, 3), max_cov, min_cov, max_cv)
+        w = [a[:-2] for (a, _), x in zip(L, sel) if x]
+        wanted = {"left": w, "right": w}
+    else:
+        for side in sides:
+            S = rows[side]
+            sel = O.select_pairs(f(S, 1), f(S, 1), f(S, 2), f(S, 2), f(S, 3), f(S, 3), max_cov, min_cov, max_cv)
+            wanted[side] = [a[:-2] for (a, _), x in zip(S, sel) if x]
+    out = {}
+    for side in sides:
+        recs = _fq_records(gzip.decompress((SAMPLE / f"reads.{side}.fq.gz").read_bytes()))
+        ws = set(wanted[side])
+        out[side] = b"".join(x for x in recs if x.split(b"\n")[0][1:].split()[0].decode()[:-2] in ws)
+    return out, wanted
+
+
 @pytest.mark.gpu
 def test_normalization_chain_on_sample_reads(tmp_path):
     """insilico_read_normalization.pl's PE chain on the reference's sample
@@ -415,7 +464,6 @@ def test_normalization_chain_on_sample_reads(tmp_path):
     stats of each side, C-locale sort, pair merge + selection at max_cov 30,
     extraction of the selected FASTQ records), against the oracle's count,
     stats and selection."""
-    from data import sample_fasta
     d = tmp_path
     for side in ("left", "right"):
         with open(d / f"{side}.fa", "wb") as f:
@@ -424,9 +472,7 @@ def test_normalization_chain_on_sample_reads(tmp_path):
     run([BIN / "jellyfish", "count", "-t", "4", "-m", "25", "-s", "10000000", "--canonical", d / "both.fa"], cwd=d)
     dump = d / "jellyfish.K25.min2.kmers.fa"
     dump.write_text(run([BIN / "jellyfish", "dump", "-L", "2", d / "mer_counts.jf"]).stdout)
-    names, seqs = {}, {}
     for side in ("left", "right"):
-        names[side], seqs[side] = sample_fasta(side)
         out = run([BIN / "fastaToKmerCoverageStats", "--reads", d / f"{side}.fa", "--kmers", dump, "--kmer_size",
                    "25", "--num_threads", "4", "--DS"]).stdout.splitlines(True)
         (d / f"{side}.stats.sort").write_text(out[0] + "".join(sorted(out[1:], key=lambda x: x.encode())))
@@ -436,28 +482,157 @@ def test_normalization_chain_on_sample_reads(tmp_path):
     for side in ("left", "right"):
         run([BIN / "extract_normalized_reads", "--seqType", "fq", "--selected", d / "pairs.accs", "--output",
              d / f"{side}.norm.fq", SAMPLE / f"reads.{side}.fq.gz"])
-    # the same chain through the oracle
-    seq, off = concat(seqs["left"] + seqs["right"])
-    keys, counts = O.count(seq, off, 25, True)
-    m = counts >= 2
-    rows = {}
+    want, wanted = _oracle_normalization("pairs", 30)
+    assert r.stdout.split() == wanted["left"]
+    assert 0 < len(wanted["left"]) < 30575
     for side in ("left", "right"):
-        s1, o1 = concat(seqs[side])
-        st = O.read_stats(s1, o1, keys[m], counts[m], 25, True, 1, 1, 0, 0)
-        rows[side] = {n[:-2]: O.format_stats_row(n, x).split("\t") for n, x in zip(names[side], st)}
-    cores = sorted(rows["left"], key=lambda c: (c + "/1").encode())
-    L = [rows["left"][c] for c in cores]
-    R = [rows["right"][c] for c in cores]
-    f = lambda rs, i: [float(x[i]) for x in rs]  # noqa: E731
-    sel = O.select_pairs(f(L, 1), f(R, 1), f(L, 2), f(R, 2), f(L, 3), f(R, 3), 30, 0, 10000)
-    want = [c for c, s in zip(cores, sel) if s]
-    assert r.stdout.split() == want
-    assert 0 < len(want) < len(cores)
-    wanted = set(want)
-    for side in ("left", "right"):
-        recs = _fq_records(gzip.decompress((SAMPLE / f"reads.{side}.fq.gz").read_bytes()))
-        exp = b"".join(x for x in recs if x.split(b"\n")[0][1:].split()[0].decode()[:-2] in wanted)
-        assert (d / f"{side}.norm.fq").read_bytes() == exp
+        assert (d / f"{side}.norm.fq").read_bytes() == want[side]
+
+
+# ------------------------------------------- insilico_read_normalization in one process
+
+IRN = BIN / "insilico_read_normalization"
+
+
+def _irn_args(mode: str, max_cov: int, ss: str = ""):
+    args = ["--seqType", "fq", "--JM", "2G", "--CPU", "4", "--max_cov", max_cov, "--min_cov", "0",
+            "--max_CV", "10000"]
+    if mode == "single":
+        args += ["--single", SAMPLE / "reads.left.fq.gz"]
+    else:
+        args += ["--left", SAMPLE / "reads.left.fq.gz", "--right", SAMPLE / "reads.right.fq.gz"]
+    if mode == "pairs":
+        args += ["--pairs_together", "--PARALLEL_STATS"]
+    if ss:
+        args += ["--SS_lib_type", ss]
+    return [str(a) for a in args]
+
+
+def _check_irn_outputs(out: Path, mode: str, max_cov: int, want):
+    for side, exp in want.items():
+        base = f"reads.{side}.fq.gz.normalized_K25_maxC{max_cov}_minC0_maxCV10000.fq"
+        assert (out / base).read_bytes() == exp, (mode, side)
+        link = out / f"{'single' if mode == 'single' else side}.norm.fq"
+        assert link.is_symlink() and link.resolve() == (out / base).resolve()
+        assert (out / (base + ".ok")).exists()
+    assert not (out / "tmp_normalized_reads").exists()
+
+
+def test_insilico_normalization_plan_matches_reference_merge(tmp_path):
+    """The selection units of --pairs_together are the rows
+    nbkc_merge_left_right_stats.pl --sorted wrote for the golden stats files
+    (unpaired accessions on either side skipped), whatever the reads' input
+    order; without --pairs_together each side is its own sorted list."""
+    import random
+    accs = {x: [ln.split("\t")[0] for ln in (NBKC / f"{x}.stats.sort").read_text().splitlines()[1:]]
+            for x in ("left", "right")}
+    rnd = random.Random(1)
+    for side, a in accs.items():
+        a = a[:]
+        rnd.shuffle(a)
+        (tmp_path / f"{side}.fa").write_text("".join(f">{x}\nACGTACGTAC\n" for x in a))
+    base = [IRN, "--seqType", "fa", "--JM", "1G", "--max_cov", "30", "--left", tmp_path / "left.fa", "--right",
+            tmp_path / "right.fa", "--plan_only"]
+    got = run(base + ["--pairs_together"]).stdout
+    rows = [ln.split("\t") for ln in (NBKC / "pairs.stats").read_text().splitlines()[1:]]
+    assert got == "# selection 0\n" + "".join(f"{r[1]}\t{r[5]}\n" for r in rows)
+    got = run(base).stdout.split("# selection 1\n")
+    assert got[0] == "# selection 0\n" + "".join(x + "\n" for x in sorted(accs["left"], key=str.encode))
+    assert got[1] == "".join(x + "\n" for x in sorted(accs["right"], key=str.encode))
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["--seqType", "fx", "--JM", "1G", "--max_cov", "30", "--single", "x.fa"], "set --seqType to 'fq' or 'fa'"),
+    (["--seqType", "fa", "--JM", "1", "--max_cov", "30", "--single", "x.fa"], "must set --JM"),
+    (["--seqType", "fa", "--JM", "1.5Gb", "--max_cov", "30", "--single", "x.fa"], "must set --JM"),
+    (["--seqType", "fa", "--JM", "10Gb", "--max_cov", "30", "--single", "x.fa"], "cannot parse max_memory"),
+    (["--seqType", "fa", "--JM", "1G", "--max_cov", "1", "--single", "x.fa"], "--max_cov at least 2"),
+    (["--seqType", "fa", "--JM", "1G", "--max_cov", "9", "--left", "x.fa"], "need either options 'left' and"),
+    (["--seqType", "fa", "--JM", "1G", "--max_cov", "9", "--single", "x.fa", "--pairs_together"],
+     "must use the --left and --right"),
+    (["--seqType", "fa", "--JM", "1G", "--max_cov", "9", "--single", "x.fa", "--SS_lib_type", "RR"],
+     "unrecognized SS_lib_type"),
+    (["--seqType", "fa", "--JM", "1G", "--max_cov", "9", "--single", "x.fa", "--bogus"], "do not understand"),
+    (["--server", "/nonexistent/norm.sock", "--seqType", "fa", "--JM", "1G", "--max_cov", "9", "--single", "x.fa"],
+     "no normalization service"),
+])
+def test_insilico_normalization_option_errors(tmp_path, args, msg):
+    """The script's option checks (:184-256), before any device work; an
+    explicit --server that nobody serves is an error, not a local run."""
+    r = run([IRN, *args], cwd=tmp_path, check=False)
+    assert r.returncode != 0 and msg in r.stderr, r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,ss,max_cov", [("pairs", "", 30), ("separate", "", 20), ("single", "", 20),
+                                             ("pairs", "RF", 30)])
+def test_insilico_normalization_cli(tmp_path, mode, ss, max_cov):
+    """The whole per-sample job in one process (count, stats and selection on
+    the device through the normalization service) writes the normalized FASTQ
+    the oracle's step-by-step chain gives: pairs together, each side on its
+    own, single-end, and a strand-specific RF library (left reads
+    reverse-complemented, k-mers not canonical); outputs named and linked as
+    the Perl script does, its tmp directory removed."""
+    want, wanted = _oracle_normalization(mode, max_cov, 0, 10000, ss)
+    assert all(0 < len(w) < 30575 for w in wanted.values())
+    r = run([IRN, *_irn_args(mode, max_cov, ss), "--output", tmp_path / "norm"])
+    assert "Normalization complete" in r.stderr
+    _check_irn_outputs(tmp_path / "norm", mode, max_cov, want)
+
+
+@pytest.mark.gpu
+def test_insilico_normalization_service(tmp_path):
+    """The resident service: jobs sent by concurrent clients (relative
+    --output resolved in each client's directory) give the outputs of the
+    one-process runs; a job with bad options fails alone; --shutdown stops
+    the service."""
+    import threading
+    import time
+    sock = tmp_path / "norm.sock"
+    srv = subprocess.Popen([str(IRN), "--serve", str(sock), "--slots", "2", "--hbm_budget_gb", "16"],
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    try:
+        t0 = time.time()
+        while not sock.exists():
+            assert srv.poll() is None, srv.communicate()[1]
+            assert time.time() - t0 < 120, "service did not start"
+            time.sleep(0.2)
+        cases = [("pairs", 30), ("separate", 20), ("single", 20)]
+        results = {}
+
+        def client(mode, max_cov):
+            d = tmp_path / mode
+            d.mkdir()
+            results[mode] = run([IRN, "--server", sock, *_irn_args(mode, max_cov), "--output", "out"], cwd=d,
+                                check=False, timeout=600)
+
+        th = [threading.Thread(target=client, args=c) for c in cases]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for mode, max_cov in cases:
+            r = results[mode]
+            assert r.returncode == 0, r.stderr
+            assert "Normalization complete" in r.stderr
+            want, _ = _oracle_normalization(mode, max_cov)
+            _check_irn_outputs(tmp_path / mode / "out", mode, max_cov, want)
+        bad = run([IRN, "--server", sock, "--seqType", "fq", "--JM", "1G", "--max_cov", "9", "--single",
+                   tmp_path / "missing.fq", "--output", tmp_path / "bad"], check=False, timeout=120)
+        assert bad.returncode != 0 and "missing.fq" in bad.stderr
+        # a job the device part refuses (max_CV below the bound where the CV
+        # test is exact) fails alone too
+        bad = run([IRN, "--server", sock, *_irn_args("single", 20), "--max_CV", "1", "--output", tmp_path / "bad2"],
+                  check=False, timeout=300)
+        assert bad.returncode != 0 and "max_cv" in bad.stderr, bad.stderr
+        assert srv.poll() is None
+        run([IRN, "--server", sock, "--shutdown"], timeout=120)
+        assert srv.wait(timeout=120) == 0
+        assert not sock.exists()
+    finally:
+        if srv.poll() is None:
+            srv.kill()
+            srv.wait()
 
 
 # ------------------------------------------- Chrysalis: welds, de Bruijn graphs, node weights

@@ -90,6 +90,7 @@ SIGNATURES = {
     "tk_server_create": (C.c_int, [_P, C.POINTER(_P)]),
     "tk_server_submit": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _U64P]),
     "tk_server_wait": (C.c_int, [_P, C.c_uint64, _P]),
+    "tk_server_submit_job": (C.c_int, [_P, _P, _U64P]),
     "tk_server_free": (C.c_int, [_P]),
 }
 
@@ -113,6 +114,17 @@ class JobResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("n_selected", C.c_uint64), ("distinct_estimate", C.c_double),
                 ("wait_s", C.c_double), ("run_s", C.c_double), ("latency_s", C.c_double),
                 ("message", C.c_char * 256)]
+
+
+class ServerSelection(C.Structure):
+    _fields_ = [("left", C.c_void_p), ("right", C.c_void_p), ("n_units", C.c_uint64), ("selected", C.c_void_p),
+                ("n_selected", C.c_uint64)]
+
+
+class ServerJob(C.Structure):
+    _fields_ = [("seq", C.c_void_p), ("offsets", C.c_void_p), ("n_reads", C.c_uint64), ("k", C.c_int32),
+                ("canonical", C.c_int32), ("max_cov", C.c_int32), ("min_cov", C.c_int32), ("max_cv", C.c_int32),
+                ("n_selections", C.c_int32), ("sel", ServerSelection * 2)]
 
 
 class TkError(RuntimeError):
@@ -626,12 +638,43 @@ class Server:
         self._keep[jid.value] = (seq, offsets, sel)
         return jid.value
 
+    def submit_job(self, seq: np.ndarray, offsets: np.ndarray, selections, k: int = 25, canonical: bool = True,
+                   max_cov: int = 50, min_cov: int = 1, max_cv: int = 10000) -> int:
+        """The general job (tk_server_submit_job): every read counted and scored,
+        then one or two selections, each a (left, right) pair of read-index
+        arrays in unit order (right None: single-read units)."""
+        seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        assert 1 <= len(selections) <= 2
+        job = ServerJob()
+        job.seq, job.offsets, job.n_reads = _ptr(seq), _ptr(offsets), len(offsets) - 1
+        job.k, job.canonical, job.max_cov, job.min_cov, job.max_cv = k, int(canonical), max_cov, min_cov, max_cv
+        job.n_selections = len(selections)
+        keep = {"buffers": [seq, offsets], "job": job, "selected": []}
+        for q, (left, right) in enumerate(selections):
+            left = np.ascontiguousarray(left, dtype=np.uint64)
+            right = None if right is None else np.ascontiguousarray(right, dtype=np.uint64)
+            sel = np.zeros(max(len(left), 1), np.uint8)
+            keep["buffers"] += [left, right]
+            keep["selected"].append(sel)
+            job.sel[q] = ServerSelection(_ptr(left), _ptr(right) if right is not None else None, len(left),
+                                         _ptr(sel), 0)
+        jid = C.c_uint64()
+        _check(lib().tk_server_submit_job(self.h, C.byref(job), C.byref(jid)), "tk_server_submit_job")
+        self._keep[jid.value] = keep
+        return jid.value
+
     def wait(self, job_id: int):
+        """(result, selected flags); for a general job the flags are a list, one array per selection."""
         r = JobResult()
         _check(lib().tk_server_wait(self.h, job_id, C.byref(r)), "tk_server_wait")
-        seq, off, sel = self._keep.pop(job_id)
+        kept = self._keep.pop(job_id)
         if r.status:
             raise TkError(r.status, "server job", r.message.decode(errors="replace"))
+        if isinstance(kept, dict):
+            job = kept["job"]
+            return r, [x[: job.sel[q].n_units] for q, x in enumerate(kept["selected"])]
+        seq, off, sel = kept
         return r, (sel[: (len(off) - 1) // 2] if sel is not None else None)
 
     def close(self):

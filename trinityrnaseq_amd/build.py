@@ -47,8 +47,9 @@ TOOLS = {
     "GraphFromFasta": ["tools/GraphFromFasta_main.cpp"],
     "FastaToDeBruijn": ["tools/FastaToDeBruijn_main.cpp"],
     "QuantifyGraph": ["tools/QuantifyGraph_main.cpp"],
+    "insilico_read_normalization": ["tools/insilico_read_normalization_main.cpp"],
 }
-TOOL_COMMON = ["tools/seq_io.cpp", "tools/tool_util.cpp"]
+TOOL_COMMON = ["tools/seq_io.cpp", "tools/tool_util.cpp", "tools/ingest.cpp", "tools/extract.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -1850,6 +1850,18 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 // bit 0 for a read whose mean is below 1 (its printed mean could round the
 // merged mean to "0.0"), bit 1 for a drawn pair whose median is 0 (the
 // script divides max_cov by it and dies).
+// out[i] = stats[idx[i]]: the rows of one selection's units in their order
+// (the merged-row order of a pair selection).  Indices are checked on the
+// host; one out of range gives an empty row.
+__global__ void k_gather_stats(const tk_read_stat* __restrict__ stats, uint64_t n_reads,
+                               const uint64_t* __restrict__ idx, uint64_t n, tk_read_stat* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t j = idx[i];
+    out[i] = j < n_reads ? stats[j] : tk_read_stat{0, 0, 0, 0, 0};
+  }
+}
+
 __global__ void k_select_flags(const tk_read_stat* __restrict__ L, const tk_read_stat* __restrict__ R,
                                uint64_t n, int min_cov, uint32_t* __restrict__ consumes,
                                uint32_t* __restrict__ err) {
@@ -2710,6 +2722,16 @@ static int merge_wpr() {
   return e && atoi(e) == 1 ? 1 : 2;
 }
 
+// Words a merge block reserves in the stream at a time, for a pass over
+// `windows` windows with `mgrid` blocks.
+static uint64_t res_echunk(uint64_t windows, uint64_t mgrid) {
+  return std::min<uint64_t>(65536, std::max<uint64_t>(16384, next_pow2(windows / (std::max<uint64_t>(mgrid, 1) * 4))));
+}
+
+// Smallest streams a chunked region-resident count uses (2^26 words, 512 MB
+// each); below that the table atomics count.
+constexpr uint64_t kResMinStreamWords = 1ull << 26;
+
 // Stream capacity of a region-resident count of `windows` windows: the
 // words the merge can reserve (merged words <= windows, plus < one flush
 // (at most `slots` words) given up per reserved chunk of `echunk`, plus each
@@ -2728,7 +2750,7 @@ static bool resident_plan(tk_ctx* c, uint64_t windows, uint64_t* cap, uint64_t* 
     return nb;
   }();
   *mgrid = (uint64_t)std::max(c->num_cu, 1) * pc;
-  *echunk = std::min<uint64_t>(65536, std::max<uint64_t>(16384, next_pow2(windows / (*mgrid * 4))));
+  *echunk = res_echunk(windows, *mgrid);
   *cap = windows + windows / (*echunk / slots - 1) + *mgrid * (*echunk + slots);
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
@@ -2767,17 +2789,42 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
                                             (unsigned long long)(1ull << t->rbits));
   }
   uint64_t cap = 0, echunk = 0, mgrid = 0, cap_max = 0;
-  if (!resident_plan(c, windows, &cap, &echunk, &mgrid, &cap_max)) return kNotResident;
-  const uint64_t chunk = re - rb;
-  (void)cap_max;
+  const bool whole = resident_plan(c, windows, &cap, &echunk, &mgrid, &cap_max);
+  // TK_RES_CHUNK_READS=n: read chunks of n reads even when one pass fits
+  // (tests), TK_RES_STREAM_WORDS=n: streams of at most n words.
+  const char* ecr = getenv("TK_RES_CHUNK_READS");
+  const char* esw = getenv("TK_RES_STREAM_WORDS");
+  const uint64_t forced_reads = ecr ? strtoull(ecr, nullptr, 10) : 0;
+  const uint64_t forced_words = esw ? strtoull(esw, nullptr, 10) : 0;
+  const uint64_t n_all = re - rb;
+  const double win_per_read = (double)windows / (double)std::max<uint64_t>(1, n_all);
+  uint64_t chunk = n_all;
+  double ratio = 2.0;  // windows per stream word, until a chunk has measured it
+  const bool chunked = !whole || forced_reads || forced_words;
+  if (chunked) {
+    // The streams of every window do not fit (or a test asks for chunks):
+    // count the reads in chunks of the locality order whose merged words
+    // should fit streams of cap words, each chunk's region pass reading and
+    // writing the table once.  A flush past the end of a chunk's stream is
+    // added to the table with atomics, so a chunk that merges worse than
+    // guessed costs time, not results.
+    if (!whole) cap = cap_max;
+    if (forced_words) cap = std::min(cap, forced_words);
+    if (!forced_words && !forced_reads && cap < kResMinStreamWords) return kNotResident;
+    chunk = forced_reads ? forced_reads : (uint64_t)(0.9 * ratio * (double)cap / std::max(win_per_read, 1.0));
+    chunk = std::min<uint64_t>(n_all, std::max<uint64_t>(chunk, kReadsPerBlock));
+    echunk = res_echunk((uint64_t)((double)chunk * win_per_read), mgrid);
+    // flushes past a stream go straight to the table, which must be clear
+    if ((rc = table_zero(c, t))) return rc;
+  }
   void *pa = nullptr, *pb = nullptr;
   if ((rc = ensure_scratch(c, kSlotResA, cap * 8, &pa))) return rc;
   if ((rc = ensure_scratch(c, kSlotResB, cap * 8, &pb))) return rc;
   uint64_t* A = (uint64_t*)pa;
   uint64_t* B = (uint64_t*)pb;
   unsigned long long* cursor = reinterpret_cast<unsigned long long*>(c->d_flags + 8);
-  for (uint64_t q = rb; q < re; q += chunk) {
-    const uint64_t qe = std::min<uint64_t>(re, q + chunk);
+  for (uint64_t q = rb, qe = rb; q < re; q = qe) {
+    qe = std::min<uint64_t>(re, q + chunk);
     const uint64_t keys_before = t->n_keys;
     TableView v = table_view(t);
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
@@ -2801,6 +2848,15 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     if (t->zero_pending && nnew)  // the plan's stream holds every word the merge can emit
       return fail(TK_ERR_STATE, "count: merged words missed the stream of an uncleared table");
     t->n_keys += nnew;
+    if (chunked && !forced_reads && qe < re) {
+      // the next chunk from this one's windows per stream word (reserved
+      // words, so an underestimate of the merge)
+      const double wq = (double)(qe - q) * win_per_read;
+      ratio = std::max(1.0, std::min(64.0, wq / (double)std::max<uint64_t>(1, std::min(n, cap))));
+      chunk = (uint64_t)(0.9 * ratio * (double)cap / std::max(win_per_read, 1.0));
+      chunk = std::min<uint64_t>(re - qe, std::max<uint64_t>(chunk, kReadsPerBlock));
+      echunk = res_echunk((uint64_t)((double)chunk * win_per_read), mgrid);
+    }
     n = std::min(n, cap);
     if (!n) {
       if ((rc = table_zero(c, t))) return rc;
@@ -3409,6 +3465,18 @@ int select_check(tk_ctx* c, const tk_read_stat* dl, const tk_read_stat* dr, uint
     return fail(TK_ERR_ARG, "max_cv=%d below the bound 1.2*sqrt(%u)+1 where the CV test is exact",
                 max_cv, max_nwin);
   if (n > 0xFFFFFFFFull) return fail(TK_ERR_ARG, "too many pairs");
+  return TK_OK;
+}
+
+// d_out[i] = d_stats[d_idx[i]] for i < n, on the context's stream (the
+// resident normalization service's general jobs).
+int gather_stats(tk_ctx* c, const tk_read_stat* d_stats, uint64_t n_reads, const uint64_t* d_idx, uint64_t n,
+                 tk_read_stat* d_out) {
+  if (!c || (n && (!d_stats || !d_idx || !d_out))) return fail(TK_ERR_ARG, "NULL argument");
+  if (!n) return TK_OK;
+  if (set_device(c->device)) return TK_ERR_HIP;
+  const uint64_t grid = std::min<uint64_t>(ceil_div(n, 256), 65536);
+  TK_LAUNCH(c, "gather_stats", grid, 256, 0, k_gather_stats, d_stats, n_reads, d_idx, n, d_out);
   return TK_OK;
 }
 }  // namespace tk

@@ -38,6 +38,9 @@ void cache_end();
 // Device bytes this thread's region-resident counts may use for their word
 // streams (tk_lib.hip); a count whose streams do not fit uses table atomics.
 void resident_limit(size_t bytes);
+// d_out[i] = d_stats[d_idx[i]] on the context's stream (tk_lib.hip).
+int gather_stats(tk_ctx* c, const tk_read_stat* d_stats, uint64_t n_reads, const uint64_t* d_idx, uint64_t n,
+                 tk_read_stat* d_out);
 }  // namespace tk
 
 namespace {
@@ -50,10 +53,14 @@ double secs(Clock::time_point a, Clock::time_point b) {
 
 struct Job {
   uint64_t id = 0;
-  const char* seq = nullptr;        // both.fa layout: pairs' left reads, then right reads
+  const char* seq = nullptr;
   const uint64_t* offsets = nullptr;
+  uint64_t n_reads = 0;
+  int k = 25, canonical = 1, max_cov = 0, min_cov = 1, max_cv = 0;
+  // tk_server_submit: pair i = reads i and n_pairs + i (both.fa layout)
   uint64_t n_pairs = 0;
   uint8_t* selected = nullptr;      // host, n_pairs
+  tk_server_job* general = nullptr; // tk_server_submit_job: the caller's units
   Clock::time_point t_submit;
   tk_job_result res{};
   bool done = false;
@@ -105,12 +112,53 @@ void release(tk_server* s, uint64_t bytes, bool job_done) {
   s->cv_mem.notify_all();
 }
 
+// One selection on device stats: units gathered into merged-row order (when
+// given as indices), then nbkc_normalize.pl's draws.
+int run_selection(tk_ctx* ctx, const Job& j, const tk_read_stat* d_stats, uint32_t max_nwin,
+                  const uint64_t* left, const uint64_t* right, uint64_t n, uint8_t* selected, uint64_t* nsel) {
+  *nsel = 0;
+  if (!n) return TK_OK;
+  int rc = TK_OK;
+  void *d_idx = nullptr, *d_rows = nullptr, *d_sel = nullptr;
+  do {
+    const tk_read_stat* dl = d_stats;
+    const tk_read_stat* dr = d_stats + j.n_pairs;
+    if (left) {
+      if ((rc = tk_buffer_alloc(ctx, n * 8, &d_idx))) break;
+      if ((rc = tk_buffer_alloc(ctx, (right ? 2 : 1) * n * sizeof(tk_read_stat), &d_rows))) break;
+      tk_read_stat* rows = (tk_read_stat*)d_rows;
+      if ((rc = tk_memcpy_h2d(ctx, d_idx, left, n * 8))) break;
+      if ((rc = tk::gather_stats(ctx, d_stats, j.n_reads, (const uint64_t*)d_idx, n, rows))) break;
+      dl = dr = rows;
+      if (right) {
+        // the copy waits for the gather above on the context's stream
+        if ((rc = tk_memcpy_h2d(ctx, d_idx, right, n * 8))) break;
+        if ((rc = tk::gather_stats(ctx, d_stats, j.n_reads, (const uint64_t*)d_idx, n, rows + n))) break;
+        dr = rows + n;
+      }
+    }
+    if ((rc = tk_buffer_alloc(ctx, n + 64, &d_sel))) break;
+    if ((rc = tk_select_pairs_d(ctx, dl, dr, n, j.max_cov, j.min_cov, j.max_cv, max_nwin, (uint8_t*)d_sel, nsel)))
+      break;
+    if (selected && (rc = tk_memcpy_d2h(ctx, selected, d_sel, n))) break;
+  } while (0);
+  tk_buffer_free(ctx, d_sel);
+  tk_buffer_free(ctx, d_rows);
+  tk_buffer_free(ctx, d_idx);
+  return rc;
+}
+
 int run_job(tk_server* s, tk_ctx* ctx, Job& j) {
   const tk_server_opts& o = s->opts;
-  const uint64_t n_reads = 2 * j.n_pairs;
+  const uint64_t n_reads = j.n_reads;
   const uint64_t n_bases = j.offsets[n_reads];
-  // stage 1: reads (ASCII staging + packed codes + stats output)
-  const uint64_t read_bytes = n_bases + n_bases * 3 / 8 + n_reads * (16 + sizeof(tk_read_stat)) + j.n_pairs;
+  uint64_t unit_bytes = j.n_pairs;
+  if (j.general)
+    for (int q = 0; q < j.general->n_selections; q++)
+      unit_bytes += j.general->sel[q].n_units * (8 + 2 * sizeof(tk_read_stat) + 1) + 64;
+  // stage 1: reads (ASCII staging + packed codes + stats output) and the
+  // selections' rows
+  const uint64_t read_bytes = n_bases + n_bases * 3 / 8 + n_reads * (16 + sizeof(tk_read_stat)) + unit_bytes;
   const auto t0 = Clock::now();
   reserve(s, read_bytes, false);
   const auto t1 = Clock::now();
@@ -119,36 +167,42 @@ int run_job(tk_server* s, tk_ctx* ctx, Job& j) {
   tk_reads* reads = nullptr;
   tk_table* table = nullptr;
   void* d_stats = nullptr;
-  void* d_sel = nullptr;
   uint64_t table_bytes = 0;
   do {
     if ((rc = tk_reads_from_host(ctx, j.seq, j.offsets, n_reads, &reads))) break;
     double est = 0;
-    if ((rc = tk_estimate_distinct(ctx, reads, o.k, 1, &est))) break;
+    if ((rc = tk_estimate_distinct(ctx, reads, j.k, j.canonical, &est))) break;
     j.res.distinct_estimate = est;
     const uint64_t expected = (uint64_t)(est * 1.05) + 4096;
     // stage 2: the table (8-byte slots at load <= max_load; the layout
     // rounds up by at most 2x) plus its side table
     table_bytes = (uint64_t)(2.0 * (double)expected / o.max_load * 8.0) + (1ull << 20);
     reserve(s, table_bytes, true);
-    if ((rc = tk_table_create(ctx, o.k, 1, expected, o.max_load, &table))) break;
+    if ((rc = tk_table_create(ctx, j.k, j.canonical, expected, o.max_load, &table))) break;
     if ((rc = tk_count(ctx, table, reads))) break;
-    if ((rc = tk_buffer_alloc(ctx, n_reads * sizeof(tk_read_stat), &d_stats))) break;
-    if ((rc = tk_buffer_alloc(ctx, j.n_pairs + 64, &d_sel))) break;
+    if ((rc = tk_buffer_alloc(ctx, n_reads * sizeof(tk_read_stat) + 64, &d_stats))) break;
     tk_stats_opts so{(uint32_t)o.min_count, 1, TK_NWIN_AS_MISSING, TK_MEDIAN_UPPER};
     if ((rc = tk_read_cov_stats_d(ctx, table, reads, &so, (tk_read_stat*)d_stats))) break;
+    // the table is no longer needed: its blocks go back before the selection
+    tk_table_free(table);
+    table = nullptr;
     uint32_t max_len = 0;
     tk_reads_info(reads, nullptr, nullptr, &max_len);
-    const uint32_t max_nwin = max_len >= (uint32_t)o.k ? max_len - o.k + 1 : 0;
-    const tk_read_stat* dl = (const tk_read_stat*)d_stats;
+    const uint32_t max_nwin = max_len >= (uint32_t)j.k ? max_len - j.k + 1 : 0;
+    const tk_read_stat* ds = (const tk_read_stat*)d_stats;
     uint64_t nsel = 0;
-    if ((rc = tk_select_pairs_d(ctx, dl, dl + j.n_pairs, j.n_pairs, o.max_cov, 1, o.max_cv, max_nwin,
-                                (uint8_t*)d_sel, &nsel)))
-      break;
-    if (j.selected && (rc = tk_memcpy_d2h(ctx, j.selected, d_sel, j.n_pairs))) break;
-    j.res.n_selected = nsel;
+    if (!j.general) {
+      if ((rc = run_selection(ctx, j, ds, max_nwin, nullptr, nullptr, j.n_pairs, j.selected, &nsel))) break;
+      j.res.n_selected = nsel;
+    } else {
+      for (int q = 0; q < j.general->n_selections && !rc; q++) {
+        tk_server_selection& sel = j.general->sel[q];
+        rc = run_selection(ctx, j, ds, max_nwin, sel.left, sel.right, sel.n_units, sel.selected, &nsel);
+        sel.n_selected = nsel;
+        j.res.n_selected += nsel;
+      }
+    }
   } while (0);
-  tk_buffer_free(ctx, d_sel);
   tk_buffer_free(ctx, d_stats);
   tk_table_free(table);
   tk_reads_free(reads);
@@ -204,6 +258,20 @@ void worker(tk_server* s, int idx) {
   tk::cache_end();
 }
 
+int enqueue(tk_server* s, std::shared_ptr<Job> j, uint64_t* job_id) {
+  j->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return TK_ERR_STATE;
+    j->id = s->next_id++;
+    s->jobs[j->id] = j;
+    s->queue.push_back(j);
+  }
+  s->cv_queue.notify_one();
+  *job_id = j->id;
+  return TK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -228,19 +296,37 @@ int tk_server_submit(tk_server* s, const char* seq, const uint64_t* offsets, uin
   auto j = std::make_shared<Job>();
   j->seq = seq;
   j->offsets = offsets;
+  j->n_reads = 2 * n_pairs;
   j->n_pairs = n_pairs;
   j->selected = selected;
-  j->t_submit = Clock::now();
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) return TK_ERR_STATE;
-    j->id = s->next_id++;
-    s->jobs[j->id] = j;
-    s->queue.push_back(j);
+  j->k = s->opts.k;
+  j->canonical = 1;
+  j->max_cov = s->opts.max_cov;
+  j->min_cov = 1;
+  j->max_cv = s->opts.max_cv;
+  return enqueue(s, j, job_id);
+}
+
+int tk_server_submit_job(tk_server* s, tk_server_job* g, uint64_t* job_id) {
+  if (!s || !g || !job_id || !g->offsets || (g->n_reads && !g->seq)) return TK_ERR_ARG;
+  if (g->k < 1 || g->k > 31 || g->max_cov <= 0 || g->n_selections < 1 || g->n_selections > 2) return TK_ERR_ARG;
+  for (int q = 0; q < g->n_selections; q++) {
+    const tk_server_selection& sel = g->sel[q];
+    if (sel.n_units && !sel.left) return TK_ERR_ARG;
+    for (uint64_t u = 0; u < sel.n_units; u++)
+      if (sel.left[u] >= g->n_reads || (sel.right && sel.right[u] >= g->n_reads)) return TK_ERR_ARG;
   }
-  s->cv_queue.notify_one();
-  *job_id = j->id;
-  return TK_OK;
+  auto j = std::make_shared<Job>();
+  j->seq = g->seq;
+  j->offsets = g->offsets;
+  j->n_reads = g->n_reads;
+  j->k = g->k;
+  j->canonical = g->canonical ? 1 : 0;
+  j->max_cov = g->max_cov;
+  j->min_cov = g->min_cov;
+  j->max_cv = g->max_cv;
+  j->general = g;
+  return enqueue(s, j, job_id);
 }
 
 int tk_server_wait(tk_server* s, uint64_t job_id, tk_job_result* res) {

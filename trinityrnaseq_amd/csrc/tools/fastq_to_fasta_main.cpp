@@ -31,6 +31,7 @@
 #include <string>
 #include <vector>
 
+#include "ingest.h"
 #include "seq_io.h"
 #include "tool_util.h"
 
@@ -38,137 +39,19 @@ using namespace tools;
 
 namespace {
 
-// Perl's \s on byte strings: space, \t, \n, \v, \f, \r.
 inline bool perl_space(char c) { return c == ' ' || (c >= '\t' && c <= '\r'); }
 
-// Perl string truthiness: false for "" and "0".
-inline bool perl_true(const std::string& s) { return !(s.empty() || s == "0"); }
+using Opts = ConvertOpts;
+using Tally = ConvertTally;
 
-struct Lines {
-  const char* p;
-  uint64_t n, pos = 0;
-  Lines(const char* d, uint64_t sz) : p(d), n(sz) {}
-  // Next line with its newline (Perl's <$fh>); false at EOF.
-  bool next(std::string& out) {
-    if (pos >= n) return false;
-    const char* e = (const char*)memchr(p + pos, '\n', n - pos);
-    const uint64_t end = e ? (uint64_t)(e - p) + 1 : n;
-    out.assign(p + pos, end - pos);
-    pos = end;
-    return true;
-  }
-};
-
-void chomp(std::string& s) {
-  if (!s.empty() && s.back() == '\n') s.pop_back();
-}
-
-char rc_nuc_translator(char c) {  // tr/ACGTacgtyrkmYRKMUu/TGCAtgcarymkRYMKAa/
-  switch (c) {
-    case 'A': return 'T'; case 'C': return 'G'; case 'G': return 'C'; case 'T': return 'A';
-    case 'a': return 't'; case 'c': return 'g'; case 'g': return 'c'; case 't': return 'a';
-    case 'y': return 'r'; case 'r': return 'y'; case 'k': return 'm'; case 'm': return 'k';
-    case 'Y': return 'R'; case 'R': return 'Y'; case 'K': return 'M'; case 'M': return 'K';
-    case 'U': return 'A'; case 'u': return 'a';
-    default: return c;
-  }
-}
-
-char rc_iupac(char c) {  // tr/ACGTUMRWSYKVHDBNacgtumrwsykvhdbn/TGCAAKYWSRMBDHVNtgcaakywsrmbdhvn/
-  static const char* from = "ACGTUMRWSYKVHDBNacgtumrwsykvhdbn";
-  static const char* to = "TGCAAKYWSRMBDHVNtgcaakywsrmbdhvn";
-  const char* f = strchr(from, c);
-  return (f && c) ? to[f - from] : c;
-}
-
-struct Opts {
-  bool rev = false;
-  bool ignore_dirty = false;
-  bool seqtk = false;      // seqtk-trinity -R rule and revcomp table
-  std::string append;      // -a N (fastQ_to_fastA) / -R N (seqtk-trinity)
-};
-
-struct Tally {
-  uint64_t clean = 0, dirty = 0;
-};
-
-// Split the header (after '@') like Perl's split(/\s+/, $h): a leading empty
-// field when it starts with whitespace, trailing empty fields dropped.
-std::vector<std::string> perl_split_ws(const std::string& h) {
-  std::vector<std::string> f;
-  std::string cur;
-  for (size_t i = 0; i < h.size();) {
-    if (perl_space(h[i])) {
-      f.push_back(cur);
-      cur.clear();
-      while (i < h.size() && perl_space(h[i])) i++;
-    } else {
-      cur.push_back(h[i++]);
-    }
-  }
-  f.push_back(cur);
-  while (!f.empty() && f.back().empty()) f.pop_back();
-  return f;
-}
-
-bool ends_pair_suffix(const std::string& s) {
-  return s.size() >= 2 && s[s.size() - 2] == '/' && (s.back() == '1' || s.back() == '2');
-}
-
+// One input file; a malformed record ends the tool with status 255, as the
+// Perl script's exit does.
 void convert(const std::string& path, const Opts& o, Out& out, Tally& t) {
-  TextSource src(path);
-  Lines in(src.data(), src.size());
-  std::string line, header, seq, qh, ql;
-  while (in.next(line)) {
-    std::string l;
-    l.reserve(line.size());
-    for (char c : line)
-      if (c != '\r') l.push_back(c);
-    if (l.empty() || l[0] != '@') continue;
-    header = l;
-    const bool hs = in.next(seq), hq = in.next(qh), hl = in.next(ql);
-    chomp(header);
-    if (hs) chomp(seq);
-    if (hq) chomp(qh);
-    if (hl) chomp(ql);
-    bool ql_nonblank = false;
-    if (hl)
-      for (char c : ql)
-        if (!perl_space(c)) { ql_nonblank = true; break; }
-    const bool ok = perl_true(header) && hs && perl_true(seq) && hq && perl_true(qh) && ql_nonblank &&
-                    qh[0] == '+' && seq.size() == ql.size();
-    if (!ok) {
-      t.dirty++;
-      if (!o.ignore_dirty) {
-        out.flush();
-        fprintf(stderr, "Error, improperly formatted entry:\n\n%s\n%s\n%s\n%s\n", header.c_str(),
-                hs ? seq.c_str() : "", hq ? qh.c_str() : "", hl ? ql.c_str() : "");
-        exit(255);
-      }
-      continue;
-    }
-    std::vector<std::string> parts = perl_split_ws(header.substr(1));
-    std::string name = parts.empty() ? std::string() : parts[0];
-    if (parts.size() > 1 && !ends_pair_suffix(name) && parts[1].size() >= 2 &&
-        (parts[1][0] == '1' || parts[1][0] == '2') && parts[1][1] == ':') {
-      name += '/';
-      name += parts[1][0];
-    }
-    if (!o.append.empty() && !(o.seqtk && ends_pair_suffix(name))) {
-      name += '/';
-      name += o.append;
-    }
-    if (o.rev) {
-      std::string r(seq.rbegin(), seq.rend());
-      for (char& c : r) c = o.seqtk ? rc_iupac(c) : rc_nuc_translator(c);
-      seq.swap(r);
-    }
-    out.putc('>');
-    out.put(name);
-    out.putc('\n');
-    out.put(seq);
-    out.putc('\n');
-    t.clean++;
+  try {
+    fastq_to_fasta(path, o, out, t);
+  } catch (const DirtyRecord& d) {
+    fputs(d.what(), stderr);
+    exit(255);
   }
 }
 

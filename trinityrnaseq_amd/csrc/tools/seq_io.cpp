@@ -68,6 +68,15 @@ TextSource::~TextSource() {
 
 SeqReader::SeqReader(const std::string& path, bool keep_names)
     : src_(path), data_(src_.data()), size_(src_.size()), keep_names_(keep_names) {
+  start(path);
+}
+
+SeqReader::SeqReader(const char* data, uint64_t size, bool keep_names, const std::string& label)
+    : src_(data, size), data_(src_.data()), size_(src_.size()), keep_names_(keep_names) {
+  start(label);
+}
+
+void SeqReader::start(const std::string& path) {
   // skip leading whitespace (Fasta_reader.pm:55-59)
   while (pos_ < size_ && (data_[pos_] == '\n' || data_[pos_] == '\r' || data_[pos_] == ' ' ||
                           data_[pos_] == '\t'))
@@ -141,6 +150,16 @@ SeqSet read_all(const std::string& path, bool keep_names) {
   s.clear();
   SeqReader r(path, keep_names);
   s.seq.reserve(r.file_size());
+  while (r.next_batch(s, UINT64_MAX, UINT64_MAX)) {
+  }
+  return s;
+}
+
+SeqSet parse_all(const std::string& text, bool keep_names, const std::string& label) {
+  SeqSet s;
+  s.clear();
+  SeqReader r(text.data(), text.size(), keep_names, label);
+  s.seq.reserve(text.size());
   while (r.next_batch(s, UINT64_MAX, UINT64_MAX)) {
   }
   return s;

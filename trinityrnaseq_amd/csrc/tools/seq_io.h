@@ -37,6 +37,8 @@ struct SeqSet {
 class TextSource {
  public:
   explicit TextSource(const std::string& path);
+  // A view of bytes the caller keeps alive.
+  TextSource(const char* data, uint64_t size) : data_(data), size_(size) {}
   ~TextSource();
   TextSource(const TextSource&) = delete;
   TextSource& operator=(const TextSource&) = delete;
@@ -57,6 +59,8 @@ class TextSource {
 class SeqReader {
  public:
   explicit SeqReader(const std::string& path, bool keep_names = true);
+  // Over text in memory (kept alive by the caller); `label` names it in errors.
+  SeqReader(const char* data, uint64_t size, bool keep_names, const std::string& label);
   ~SeqReader();
   bool next_batch(SeqSet& out, uint64_t max_records, uint64_t max_bases);
   uint64_t file_size() const { return size_; }
@@ -68,9 +72,12 @@ class SeqReader {
   uint64_t pos_ = 0;
   bool fastq_ = false;
   bool keep_names_ = true;
+  void start(const std::string& label);
 };
 
 // Read every record of a file.
 SeqSet read_all(const std::string& path, bool keep_names = true);
+// Every record of FASTA/FASTQ text in memory.
+SeqSet parse_all(const std::string& text, bool keep_names, const std::string& label);
 
 }  // namespace tools

@@ -11,10 +11,20 @@
 
 namespace tools {
 
+static thread_local bool t_die_throws = false;
+
+void die_throws(bool on) { t_die_throws = on; }
+
 void die(const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
-  fputs("Error: ", stderr);
+  if (t_die_throws) {
+    char buf[1024];
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw std::runtime_error(buf);
+  }
+  if (strncmp(fmt, "Error", 5) != 0) fputs("Error: ", stderr);  // messages in the scripts' own wording print as is
   vfprintf(stderr, fmt, ap);
   fputc('\n', stderr);
   va_end(ap);
@@ -159,6 +169,11 @@ void Out::put_u64(uint64_t v) {
 }
 
 void Out::flush() {
+  if (sink_) {
+    sink_->append(buf_);
+    buf_.clear();
+    return;
+  }
   if (!buf_.empty()) {
     if (fwrite(buf_.data(), 1, buf_.size(), f_) != buf_.size()) die("write failed");
     buf_.clear();

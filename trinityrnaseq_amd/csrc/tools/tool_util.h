@@ -12,6 +12,10 @@
 namespace tools {
 
 [[noreturn]] void die(const char* fmt, ...);
+// While set on a thread, die() there throws std::runtime_error(message)
+// instead of exiting: a resident process runs many jobs, and one job's error
+// must end that job only.
+void die_throws(bool on);
 // Exit 1 with the library's message when rc != TK_OK (Pipeliner treats any
 // non-zero exit as failure: PerlLib/Pipeliner.pm:156-195).
 void check(int rc, const char* what);
@@ -45,7 +49,14 @@ uint64_t revcomp(uint64_t v, int k);
 class Out {
  public:
   explicit Out(FILE* f) : f_(f) { buf_.reserve(1 << 22); }
-  ~Out() { flush(); }
+  // Appends to *sink instead of a file.
+  explicit Out(std::string* sink) : sink_(sink) { buf_.reserve(1 << 22); }
+  ~Out() {
+    try {
+      flush();
+    } catch (...) {  // die() in a job thread: the job already failed
+    }
+  }
   void put(const char* s, size_t n) {
     buf_.append(s, n);
     if (buf_.size() >= (1u << 22)) flush();
@@ -56,7 +67,8 @@ class Out {
   void flush();
 
  private:
-  FILE* f_;
+  FILE* f_ = nullptr;
+  std::string* sink_ = nullptr;
   std::string buf_;
 };
 
