@@ -63,6 +63,13 @@ STRESS_TRAFFIC_B_PER_WIN = {"count": (880.30 + 574.82) / 37.8, "stats": (2483.85
                             "vote": (2243.57 + 129.77) / 37.8}
 
 
+# every kernel name the library times (TK_LAUNCH / timing names)
+KERNEL_NAMES = ["pack", "read_words", "hll", "sort_pairs", "minimizer", "count", "count_merge", "count_hist",
+                "count_scatter", "count_bounds", "count_regions", "count_retry", "ovf_fold", "ovf_rehash", "rehash",
+                "stats", "select", "select_flags", "scan_u32", "scan_u64", "windows", "gather_stats", "prune",
+                "ctable_build", "vote", "dump", "dump_bins", "dump_count", "histo"]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -353,16 +360,26 @@ def bench_stress(args) -> dict:
     assigned = assigned_reads()
     kern = {}
     for name in ["pack", "read_words", "hll", "sort_pairs", "count", "count_merge", "count_hist", "count_scatter",
-                 "count_regions", "stats", "prune", "ctable_build", "vote", "ovf_fold"]:
+                 "count_bounds", "count_regions", "count_retry", "stats", "prune", "ctable_build", "vote", "ovf_fold",
+                 "scan_u64"]:
         tot, n = ctx.kernel_time(name)
         if n:
             kern[name] = {"total_ms": round(tot, 3), "launches": n, "avg_ms": round(tot / n, 4)}
+    words = ctx.count_words()  # merged words of the last step's count (0: table atomics)
     ctx.timing(False)
     ms = 1000 * sum(lat) / len(lat)
     # dominant kernel by time per step, priced at its algorithmic bytes
     stage_bytes = {"count": wins * COUNT_B_PER_WIN + n_reads * COUNT_B_PER_READ,
                    "stats": wins * STATS_B_PER_WIN + n_reads * STATS_B_PER_READ,
                    "vote": wins * VOTE_B_PER_WIN + n_reads * VOTE_B_PER_READ}
+    if words:
+        # region-resident count in read chunks: each chunk's region pass reads and writes the
+        # table's lines once (the first, on a cleared table, only writes them)
+        chunks = kern.get("count_regions", {}).get("launches", args.steps) / args.steps
+        table_bytes = info["capacity"] * 8
+        stage_bytes.update({"count_merge": n_reads * READ_LEN * 0.375 + 8 * words,
+                            "count_hist": 2 * 8 * words, "count_scatter": 2 * 16 * words,
+                            "count_regions": 8 * words + table_bytes * (2 * chunks - 1)})
     roof = {}
     for name, by in stage_bytes.items():
         if name in kern:
@@ -380,7 +397,8 @@ def bench_stress(args) -> dict:
                 "traffic_source": "profiles/stress_150M_pairs_trace.md (FETCH_SIZE + WRITE_SIZE per dispatch)",
                 "kernel": dom, "algorithmic_bytes_per_launch": round(per_launch),
                 "avg_launch_ms": kern[dom]["avg_ms"],
-                "other": {n: {"gbs": round(v["gbs"], 1), "ms_per_step": round(v["ms"], 3)} for n, v in roof.items()}}
+                "other": {n: {"gbs": round(v["gbs"], 1), "ms_per_step": round(v["ms"], 3)} for n, v in roof.items()},
+                "merged_words": words}
     # the same job from pageable host memory: upload + pack inside the job (not `value`)
     host_lat = []
     for i in range(args.host_steps):
@@ -605,6 +623,8 @@ def bench_serving(args) -> dict:
     ctx = api.Context(0)
     api.thread_cache(int(args.hbm_budget_gb * 2**30))
     plain_job(ctx, samples[0][0], samples[0][1], samples[0][2], args.max_cov)  # warm-up
+    ctx.timing(True)
+    ctx.timing_reset()
     t = time.perf_counter()
     plain_lat = []
     plain_sel = []
@@ -613,6 +633,11 @@ def bench_serving(args) -> dict:
         plain_sel.append(plain_job(ctx, b, o, n, args.max_cov))
         plain_lat.append(time.perf_counter() - t1)
     plain_s = time.perf_counter() - t
+    # kernel time of the whole stream's jobs (HIP events on the context's stream): what no
+    # overlap of uploads and host work can hide
+    gpu_s = sum(ctx.kernel_time(name)[0] for name in KERNEL_NAMES) / 1000.0
+    ctx.timing(False)
+    log(f"plain serial: kernels {gpu_s:.3f}s of {plain_s:.3f}s")
     api.thread_cache(0)
     ctx.close()
     log(f"plain serial: {plain_s:.3f}s")
@@ -668,6 +693,8 @@ def bench_serving(args) -> dict:
                                f"count + coverage stats + max_cov={args.max_cov} selection per sample",
                    "slots": args.slots, "hbm_budget_gb": args.hbm_budget_gb, "total_pairs": total_pairs},
         "serial_plain_reads_per_s": 2 * total_pairs / plain_s,
+        "stream_kernel_s": round(gpu_s, 3),
+        "served_over_kernel_bound": round(gpu_s / burst_s, 3),
         "served_over_serial": plain_s / burst_s,
         "server_1slot_reads_per_s": 2 * total_pairs / slot1_s,
         "served_over_1slot": slot1_s / burst_s,

@@ -663,13 +663,17 @@ def _components(seed, n_comp, max_reads=400):
     return comps
 
 
+@pytest.mark.parametrize("agg", ["1", "0"])
 @pytest.mark.parametrize("packed", ["1", "0"])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_components_match_per_component_oracle(api, ctx, use_graph, packed, monkeypatch):
+def test_batch_components_match_per_component_oracle(api, ctx, use_graph, packed, agg, monkeypatch):
     """Packed: one-word component-table slots (the default at k = 25); else
-    the two-word slots of the plain component table."""
-    say("batch_components graph", use_graph, "packed", packed)
+    the two-word slots of the plain component table.  agg: the count merges
+    each block's group of one component's reads in LDS (default at k <= 25);
+    else one table atomic per window."""
+    say("batch_components graph", use_graph, "packed", packed, "agg", agg)
     monkeypatch.setenv("TK_BATCH_CT_PACKED", packed)
+    monkeypatch.setenv("TK_BATCH_AGG", agg)
     comps = _components(11, 40)
     all_reads = [r for rs, _ in comps for r in rs]
     all_contigs = [c for _, cs in comps for c in cs]
@@ -704,6 +708,28 @@ def test_batch_components_match_per_component_oracle(api, ctx, use_graph, packed
                 wv["comp"] = 0xFFFFFFFF
                 wv["nwin"] = [max(0, len(r) - 24) for r in rs]
             assert np.array_equal(vo[a:z], wv.astype(vo.dtype)), (rep, c)
+
+
+@pytest.mark.parametrize("agg", ["1", "0"])
+def test_batch_count_runs_past_the_slot_field(api, ctx, agg, monkeypatch):
+    """Components whose reads repeat one k-mer thousands of times (counts past
+    the 13-bit slot field go to the side table; with the block merge, each
+    group's run is cut at the merge's occurrence bound) next to ordinary
+    ones: every component's stats equal the oracle's."""
+    say("batch_count_runs agg", agg)
+    monkeypatch.setenv("TK_BATCH_AGG", agg)
+    comps = _components(13, 6)
+    comps.insert(2, (["A" * 150] * 2000, ["A" * 200]))
+    comps.insert(4, (["ACG" * 50] * 700 + ["T" * 90] * 130, ["ACG" * 60]))
+    all_reads = [r for rs, _ in comps for r in rs]
+    all_contigs = [c for _, cs in comps for c in cs]
+    ro = np.cumsum([0] + [len(rs) for rs, _ in comps]).astype(np.uint64)
+    co = np.cumsum([0] + [len(cs) for _, cs in comps]).astype(np.uint64)
+    reads = api.Reads.from_list(ctx, all_reads)
+    contigs = api.Reads.from_list(ctx, all_contigs)
+    b = api.Batch(ctx, reads, ro, contigs, co, use_graph=True)
+    _check_batch(api, ctx, comps, b, ro)
+    b.free()
 
 
 def _check_batch(api, ctx, comps, b, ro):

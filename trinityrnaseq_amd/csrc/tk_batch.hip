@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "tk_count_agg.h"
 #include "tk_internal.h"
 #include "tk_perread.h"
 
@@ -110,6 +111,73 @@ __global__ __launch_bounds__(256) void k_batch_count(ReadsView rv, const uint32_
     coop_add64(t, key, valid, tid & 63, &flags[0], lost);
     if (lost) atomicOr(&flags[0], 1u);
   }
+}
+
+// The count with the block merge of k_count_agg (k <= 25): each block takes
+// a group of at most 64 consecutive reads of ONE component, merges their
+// windows in its LDS table and adds each distinct key once, with its
+// occurrences, to that component's table.  A component of many reads at high
+// coverage repeats most k-mers within a group, so the table sees a fraction
+// of the per-window atomics of k_batch_count.
+struct BatchGroup {
+  uint64_t r0;    // first read
+  uint32_t nr;    // reads (<= kReadsPerBlock)
+  uint32_t comp;  // component
+};
+
+struct BatchGroups {
+  const BatchGroup* g;
+  uint64_t n;
+  const CompLayout* lay;
+  TableView base;
+  __device__ __forceinline__ uint64_t count() const { return n; }
+  __device__ __forceinline__ void get(uint64_t i, uint64_t& r0, int& nr, TableView& t) const {
+    const BatchGroup x = g[i];
+    r0 = x.r0;
+    nr = (int)x.nr;
+    t = comp_view(base, lay[x.comp]);
+  }
+};
+
+constexpr int kBatchAggSlots = 4096;
+
+template <bool KHI, int WPR>
+__global__ __launch_bounds__(256, 1) void k_batch_count_agg(ReadsView rv, const BatchGroup* __restrict__ groups,
+                                                            uint64_t n_groups, const CompLayout* __restrict__ lay,
+                                                            int k, int canonical, TableView base,
+                                                            uint32_t* __restrict__ flags) {
+  count_agg<kBatchAggSlots, false, KHI, WPR>(rv, BatchGroups{groups, n_groups, lay, base}, k, canonical, base, flags,
+                                             nullptr, 0, 0, nullptr);
+}
+
+// Locality key of each read within its component: the component in the top
+// bits, then the read's minimizer key (MinKey), so that sorting by it keeps
+// every component's reads together and puts reads of one locus side by side
+// (the merge then sees their shared windows within one group).
+__global__ __launch_bounds__(256) void k_batch_order_keys(ReadsView rv, const uint32_t* __restrict__ read_comp,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rv.n_reads;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t L = rv.len[r];
+    const uint64_t wb = rv.word_off[r];
+    MinKey mk;
+    for (uint32_t w = 0; 32 * w < L; w++) {
+      const uint64_t code = rv.codes[wb + w];
+      const uint32_t nb = L - 32 * w < 32u ? L - 32 * w : 32u;
+      for (uint32_t j = 0; j < nb; j++) mk.push((uint32_t)(code >> (62 - 2 * j)) & 3u, 32 * w + j);
+    }
+    keys[r] = ((uint64_t)read_comp[r] << kOrderKeyBits) | mk.key(L);
+    idx[r] = (uint32_t)r;
+  }
+}
+
+// Whether the batched count merges per block (TK_BATCH_AGG=0: one atomic per
+// window, k_batch_count), read when the batch is created.  Needs k <= 25 (a
+// merged count fits the slot field); without the read order (more than 2^24
+// components or 2^31 reads) the batch counts per window.
+static bool batch_agg(int k) {
+  const char* e = getenv("TK_BATCH_AGG");
+  return !(e && e[0] == '0') && 63 - 2 * k >= 13;
 }
 
 // Finalise every component's side table (see k_ovf_fold in tk_lib.hip).
@@ -235,6 +303,9 @@ struct tk_batch {
   uint32_t* d_read_comp = nullptr;
   uint32_t* d_contig_comp = nullptr;
   uint32_t* d_contig_local = nullptr;
+  BatchGroup* d_groups = nullptr;  // block-merge groups of the count
+  uint64_t n_groups = 0;
+  uint32_t* d_order = nullptr;     // reads of each component in locality order (merged count)
   uint64_t* d_slots = nullptr;
   uint64_t* d_ovf = nullptr;
   uint64_t* d_ct = nullptr;
@@ -339,8 +410,18 @@ static int batch_enqueue(tk_ctx* c, tk_batch* b, const tk_stats_opts* o, tk_read
   const TableView base = batch_base_view(b);
   const ReadsView rv = rview(b->reads);
   if (rv.n_reads) {
-    hipLaunchKernelGGL(k_batch_count, dim3((rv.n_reads + kBatchReads - 1) / kBatchReads), dim3(256), 0,
-                       c->stream, rv, b->d_read_comp, b->d_lay, b->k, b->canonical, base, c->d_flags);
+    const unsigned ag = (unsigned)((b->n_groups + 7) / 8 * 8);  // xcd_block maps grids of 8k blocks
+    ReadsView ro = rv;
+    ro.order = b->d_order;
+    if (b->d_order && b->k >= 16)
+      hipLaunchKernelGGL((k_batch_count_agg<true, 2>), dim3(ag), dim3(256), kBatchAggSlots * 8, c->stream, ro,
+                         b->d_groups, b->n_groups, b->d_lay, b->k, b->canonical, base, c->d_flags);
+    else if (b->d_order)
+      hipLaunchKernelGGL((k_batch_count_agg<false, 1>), dim3(ag), dim3(256), kBatchAggSlots * 8, c->stream, ro,
+                         b->d_groups, b->n_groups, b->d_lay, b->k, b->canonical, base, c->d_flags);
+    else
+      hipLaunchKernelGGL(k_batch_count, dim3((rv.n_reads + kBatchReads - 1) / kBatchReads), dim3(256), 0,
+                         c->stream, rv, b->d_read_comp, b->d_lay, b->k, b->canonical, base, c->d_flags);
     hipLaunchKernelGGL(k_batch_fold, dim3(grid(b->n_ovf_pairs, 256)), dim3(256), 0, c->stream, base, b->d_lay,
                        b->n_comp, b->n_ovf_pairs, c->d_flags);
   }
@@ -486,7 +567,18 @@ int tk_batch_create(tk_ctx* c, int k, int canonical, const tk_reads* reads, cons
     delete b;
     return fail(TK_ERR_ARG, "reads longer than %u bases are not supported", 16128 + k - 1);
   }
-  rc = dev_alloc((void**)&b->d_read_comp, std::max<uint64_t>(reads->n_reads, 1) * 4);
+  // block-merge groups: each component's reads in runs of kReadsPerBlock
+  std::vector<BatchGroup> groups;
+  for (uint64_t i = 0; i < n_comp; i++)
+    for (uint64_t r = read_comp_off[i]; r < read_comp_off[i + 1]; r += kReadsPerBlock)
+      groups.push_back({r, (uint32_t)std::min<uint64_t>(kReadsPerBlock, read_comp_off[i + 1] - r), (uint32_t)i});
+  b->n_groups = groups.size();
+  rc = dev_alloc((void**)&b->d_groups, std::max<uint64_t>(b->n_groups, 1) * sizeof(BatchGroup));
+  if (!rc && b->n_groups &&
+      (e = hipMemcpyAsync(b->d_groups, groups.data(), b->n_groups * sizeof(BatchGroup), hipMemcpyHostToDevice,
+                          c->stream)) != hipSuccess)
+    rc = hip_fail(e, "upload count groups");
+  if (!rc) rc = dev_alloc((void**)&b->d_read_comp, std::max<uint64_t>(reads->n_reads, 1) * 4);
   if (!rc) rc = dev_alloc((void**)&b->d_contig_comp, std::max<uint64_t>(contigs->n_reads, 1) * 4);
   if (!rc) rc = dev_alloc((void**)&b->d_contig_local, std::max<uint64_t>(contigs->n_reads, 1) * 4);
   if (!rc && reads->n_reads)
@@ -510,6 +602,26 @@ int tk_batch_create(tk_ctx* c, int k, int canonical, const tk_reads* reads, cons
     for (uint64_t i = 0; i < n_comp; i++)
       max_contigs = std::max<uint64_t>(max_contigs, contig_comp_off[i + 1] - contig_comp_off[i]);
     b->ct_packed = !(ev && ev[0] == '0') && k <= 25 && max_contigs <= (1ull << (63 - 2 * k));
+  }
+  // the merged count's read order: by component, then by locality key
+  // (component ids fit above the 40 key bits; one sort call takes < 2^31 items)
+  if (!rc && batch_agg(k) && b->n_groups && reads->n_reads <= 0x7FFFFFFFull && n_comp <= (1ull << 24)) {
+    uint64_t* keys = nullptr;
+    rc = dev_alloc((void**)&keys, reads->n_reads * 8);
+    if (!rc) rc = dev_alloc((void**)&b->d_order, reads->n_reads * 4);
+    if (!rc) {
+      const uint64_t grid = std::min<uint64_t>((reads->n_reads + 255) / 256, (uint64_t)std::max(c->num_cu, 1) * 16);
+      hipLaunchKernelGGL(k_batch_order_keys, dim3(grid), dim3(256), 0, c->stream, rview(reads), b->d_read_comp,
+                         keys, b->d_order);
+      if ((e = hipGetLastError()) != hipSuccess) rc = hip_fail(e, "k_batch_order_keys");
+    }
+    int cbits = 0;
+    while ((1ull << cbits) < n_comp) cbits++;
+    if (!rc) rc = sort_pairs_u64_u32(c, keys, b->d_order, reads->n_reads, kOrderKeyBits + cbits);
+    if (keys) {
+      (void)hipStreamSynchronize(c->stream);
+      dev_free(keys, reads->n_reads * 8);
+    }
   }
   if (!rc) rc = batch_layout(c, b);
   if (rc) { tk_batch_free(b); return rc; }
@@ -607,6 +719,8 @@ int tk_batch_free(tk_batch* b) {
   dev_free(b->d_read_comp, std::max<uint64_t>(b->reads->n_reads, 1) * 4);
   dev_free(b->d_contig_comp, std::max<uint64_t>(b->contigs->n_reads, 1) * 4);
   dev_free(b->d_contig_local, std::max<uint64_t>(b->contigs->n_reads, 1) * 4);
+  dev_free(b->d_groups, std::max<uint64_t>(b->n_groups, 1) * sizeof(BatchGroup));
+  dev_free(b->d_order, b->reads->n_reads * 4);
   delete b;
   return TK_OK;
 }

@@ -745,4 +745,40 @@ __device__ __forceinline__ uint32_t coop4_add(const TableView& t, uint64_t key, 
   return claimed;
 }
 
+// Locality sort key of a read, fed its bases in order: bits 39..8 hold the
+// smallest mixed hash over its canonical 16-mers (a minimizer; an N base
+// reads as A, the key only orders reads), bits 7..0 the read's start relative
+// to that 16-mer on the 16-mer's canonical strand.  Reads that share the
+// minimizer overlap one locus of the transcriptome and sort by where they
+// start along it, so neighbours in the order share most k-mers: their table
+// lines are then reused from L2, and their windows merge before counting.
+// Reads shorter than 16 bases sort last.
+struct MinKey {
+  uint32_t f = 0, r = 0, best = 0xFFFFFFFFu, bpos = 0;
+  bool bfwd = true;
+  __device__ __forceinline__ void push(uint32_t b, uint32_t pos) {
+    f = (f << 2) | b;
+    r = (r >> 2) | ((3u - b) << 30);
+    if (pos >= 15) {
+      const bool fw = f <= r;
+      uint32_t x = fw ? f : r;
+      x *= 0x9E3779B1u;
+      x ^= x >> 16;
+      x *= 0x85EBCA6Bu;
+      x ^= x >> 13;
+      if (x < best) {
+        best = x;
+        bpos = pos - 15;
+        bfwd = fw;
+      }
+    }
+  }
+  __device__ __forceinline__ uint64_t key(uint32_t L) const {
+    uint32_t rel = bfwd ? bpos : L - 16 - bpos;
+    rel = rel > 255u ? 255u : rel;
+    return ((uint64_t)best << 8) | (255u - rel);
+  }
+};
+constexpr int kOrderKeyBits = 40;
+
 }  // namespace tk
