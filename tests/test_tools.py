@@ -541,6 +541,48 @@ def test_insilico_normalization_plan_matches_reference_merge(tmp_path):
     assert got[1] == "".join(x + "\n" for x in sorted(accs["right"], key=str.encode))
 
 
+def test_insilico_normalization_service_protocol(tmp_path):
+    """The service's socket protocol without device work: concurrent
+    --plan_only jobs come back with the plan the one-process run prints,
+    option errors come back as the job's failure, --shutdown ends the
+    service and removes its socket."""
+    import threading
+    import time
+    (tmp_path / "l.fa").write_text("".join(f">r{i}/1\nACGTACGTAC\n" for i in (3, 1, 2)))
+    (tmp_path / "r.fa").write_text("".join(f">r{i}/2\nACGTACGTAC\n" for i in (2, 3, 9)))
+    job = ["--seqType", "fa", "--JM", "1G", "--max_cov", "30", "--left", "l.fa", "--right", "r.fa",
+           "--pairs_together", "--plan_only"]
+    want = run([IRN, *job], cwd=tmp_path).stdout
+    assert want == "# selection 0\nr2/1\tr2/2\nr3/1\tr3/2\n"
+    sock = tmp_path / "s.sock"
+    srv = subprocess.Popen([str(IRN), "--serve", str(sock), "--slots", "1", "--hbm_budget_gb", "1"],
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    try:
+        t0 = time.time()
+        while not sock.exists():
+            assert srv.poll() is None, srv.communicate()[1]
+            assert time.time() - t0 < 60
+            time.sleep(0.1)
+        got = []
+        th = [threading.Thread(target=lambda: got.append(run([IRN, "--server", sock, *job], cwd=tmp_path,
+                                                             timeout=60))) for _ in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        assert len(got) == 3 and all(r.returncode == 0 and want in r.stderr for r in got)
+        bad = run([IRN, "--server", sock, "--seqType", "fx", "--JM", "1G", "--max_cov", "30", "--single", "l.fa"],
+                  cwd=tmp_path, check=False, timeout=60)
+        assert bad.returncode == 1 and "set --seqType" in bad.stderr
+        run([IRN, "--server", sock, "--shutdown"], timeout=60)
+        assert srv.wait(timeout=60) == 0
+        assert not sock.exists()
+    finally:
+        if srv.poll() is None:
+            srv.kill()
+            srv.wait()
+
+
 @pytest.mark.parametrize("args,msg", [
     (["--seqType", "fx", "--JM", "1G", "--max_cov", "30", "--single", "x.fa"], "set --seqType to 'fq' or 'fa'"),
     (["--seqType", "fa", "--JM", "1", "--max_cov", "30", "--single", "x.fa"], "must set --JM"),

@@ -55,6 +55,7 @@
 #include <string.h>
 #include <sys/socket.h>
 #include <sys/stat.h>
+#include <sys/time.h>
 #include <sys/un.h>
 #include <unistd.h>
 
@@ -642,10 +643,12 @@ tk_server* make_server(int slots, double budget_gb) {
 // request:  "TKNR1\n", u32 n, n x (u32 len, bytes): cwd, NO_SYMLINK ("0"/"1"), args...
 // response: i32 status, u32 len, log bytes
 
+// (send with MSG_NOSIGNAL: a peer that went away is an error here, not a
+// SIGPIPE that would end the service)
 bool write_all(int fd, const void* p, size_t n) {
   const char* c = (const char*)p;
   while (n) {
-    const ssize_t w = write(fd, c, n);
+    const ssize_t w = send(fd, c, n, MSG_NOSIGNAL);
     if (w < 0 && errno == EINTR) continue;
     if (w <= 0) return false;
     c += w;
@@ -753,6 +756,10 @@ struct Service {
 
 void serve_one(Service* sv, int fd) {
   die_throws(true);
+  // a client that connects and sends nothing must not hold the service
+  // (shutdown waits for every job thread): reads give up after 60 s
+  timeval tv{60, 0};
+  setsockopt(fd, SOL_SOCKET, SO_RCVTIMEO, &tv, sizeof tv);
   std::vector<std::string> req;
   int32_t status = 0;
   std::string log;
