@@ -663,17 +663,20 @@ def _components(seed, n_comp, max_reads=400):
     return comps
 
 
-@pytest.mark.parametrize("agg", ["1", "0"])
+@pytest.mark.parametrize("agg,ct_lds", [("1", "1"), ("0", "0"), ("1", "0")])
 @pytest.mark.parametrize("packed", ["1", "0"])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_components_match_per_component_oracle(api, ctx, use_graph, packed, agg, monkeypatch):
+def test_batch_components_match_per_component_oracle(api, ctx, use_graph, packed, agg, ct_lds, monkeypatch):
     """Packed: one-word component-table slots (the default at k = 25); else
     the two-word slots of the plain component table.  agg: the count merges
     each block's group of one component's reads in LDS (default at k <= 25);
-    else one table atomic per window."""
-    say("batch_components graph", use_graph, "packed", packed, "agg", agg)
+    else one table atomic per window.  ct_lds: small components' tables are
+    built in LDS and written out whole (default); else every contig window
+    takes a memory-side atomic."""
+    say("batch_components graph", use_graph, "packed", packed, "agg", agg, "ct_lds", ct_lds)
     monkeypatch.setenv("TK_BATCH_CT_PACKED", packed)
     monkeypatch.setenv("TK_BATCH_AGG", agg)
+    monkeypatch.setenv("TK_BATCH_CT_LDS", ct_lds)
     comps = _components(11, 40)
     all_reads = [r for rs, _ in comps for r in rs]
     all_contigs = [c for _, cs in comps for c in cs]

@@ -32,8 +32,14 @@ struct CompLayout {
   uint64_t ct_boff;  // first component-table bucket
   uint64_t ct_mask;  // component-table buckets - 1
   uint32_t rbits;
-  uint32_t pad;
+  uint32_t flags;    // kCompCtInLds: the component table is built in LDS (k_batch_ctable_lds)
 };
+
+constexpr uint32_t kCompCtInLds = 1;
+// Component tables of at most kCtLdsBuckets 64-B buckets (64 KB) are built
+// by one workgroup in LDS and written out whole (32 KB when the 64 KB of
+// dynamic LDS cannot be granted).
+constexpr uint64_t kCtLdsBuckets = 1024;
 
 __device__ __forceinline__ TableView comp_view(const TableView& base, const CompLayout& L) {
   TableView v = base;
@@ -242,6 +248,7 @@ __global__ void k_batch_ctable(ReadsView cv, const uint32_t* __restrict__ contig
   for (uint64_t r = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); r < cv.n_reads;
        r += (uint64_t)gridDim.x * wpb) {
     const CompLayout L = lay[contig_comp[r]];
+    if (L.flags & kCompCtInLds) continue;  // built by k_batch_ctable_lds
     const uint32_t local = contig_local[r];
     const uint32_t len = cv.len[r];
     const uint32_t nwin = len >= (uint32_t)k ? len - k + 1 : 0;
@@ -257,6 +264,49 @@ __global__ void k_batch_ctable(ReadsView cv, const uint32_t* __restrict__ contig
         ctable_add(ct, L.ct_boff, L.ct_mask, occ, canon_key(fwd, k, canonical), local, dummy, &flags[0]);
     }
   }
+}
+
+// The component tables of small components (kCompCtInLds), one workgroup
+// each: the table is built in LDS with the same insert protocol and probe
+// sequence as in global memory (every contig k-mer's word claimed with a
+// CAS, the smallest contig index kept), then written out whole, so each
+// contig window costs an LDS atomic instead of a memory-side one.  Waves
+// take the component's contigs in turn, lanes their windows.
+template <bool kPacked>
+__global__ __launch_bounds__(256) void k_batch_ctable_lds(ReadsView cv, const uint32_t* __restrict__ comps,
+                                                          const uint64_t* __restrict__ contig_off,
+                                                          const CompLayout* __restrict__ lay, int k, int canonical,
+                                                          uint64_t* __restrict__ ct, uint32_t* __restrict__ flags) {
+  extern __shared__ uint64_t s_ct[];
+  const uint32_t c = comps[blockIdx.x];
+  const CompLayout L = lay[c];
+  const uint64_t words = (L.ct_mask + 1) << 3;
+  for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) s_ct[i] = 0;
+  __syncthreads();
+  const uint64_t occ = 1ull << (2 * k);
+  const int lane = threadIdx.x & 63;
+  const uint64_t wpb = blockDim.x >> 6;
+  uint32_t dummy = 0;
+  const uint64_t c0 = contig_off[c], c1 = contig_off[c + 1];
+  for (uint64_t r = c0 + (threadIdx.x >> 6); r < c1; r += wpb) {
+    const uint32_t local = (uint32_t)(r - c0);
+    const uint32_t len = cv.len[r];
+    const uint32_t nwin = len >= (uint32_t)k ? len - k + 1 : 0;
+    const uint64_t wb = cv.word_off[r];
+    for (uint32_t p = lane; p < nwin; p += 64) {
+      uint64_t fwd;
+      bool valid;
+      window_at(cv, wb, p, k, fwd, valid);
+      if (!valid) continue;
+      if (kPacked)
+        pctable_add(s_ct, 0, L.ct_mask, k, canon_key(fwd, k, canonical), local, &flags[0]);
+      else
+        ctable_add(s_ct, 0, L.ct_mask, occ, canon_key(fwd, k, canonical), local, dummy, &flags[0]);
+    }
+  }
+  __syncthreads();
+  uint64_t* g = ct + (L.ct_boff << 3);
+  for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) g[i] = s_ct[i];
 }
 
 template <bool kPacked>
@@ -306,6 +356,9 @@ struct tk_batch {
   BatchGroup* d_groups = nullptr;  // block-merge groups of the count
   uint64_t n_groups = 0;
   uint32_t* d_order = nullptr;     // reads of each component in locality order (merged count)
+  uint64_t* d_contig_off = nullptr;  // contigs of each component (n_comp + 1)
+  uint32_t* d_ct_lds = nullptr;      // components whose table is built in LDS
+  uint64_t n_ct_lds = 0, ct_lds_bytes = 0;
   uint64_t* d_slots = nullptr;
   uint64_t* d_ovf = nullptr;
   uint64_t* d_ct = nullptr;
@@ -329,8 +382,26 @@ static void batch_release_device(tk_batch* b) {
   dev_free(b->d_ovf, b->n_ovf_pairs * 16);
   dev_free(b->d_ct, b->n_ct_words * 8);
   dev_free(b->d_lay, b->n_comp * sizeof(CompLayout));
+  dev_free(b->d_ct_lds, std::max<uint64_t>(b->n_ct_lds, 1) * 4);
   b->d_slots = b->d_ovf = b->d_ct = nullptr;
   b->d_lay = nullptr;
+  b->d_ct_lds = nullptr;
+  b->n_ct_lds = b->ct_lds_bytes = 0;
+}
+
+// Whether k_batch_ctable_lds may take 64 KB of dynamic LDS (the attribute is
+// set once per process; a function-local static is initialised once even
+// when several threads get here together).
+static bool ct_lds_64k() {
+  static const bool ok = [] {
+    const bool r = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_ctable_lds<true>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 65536) == hipSuccess &&
+                   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_ctable_lds<false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 65536) == hipSuccess;
+    (void)hipGetLastError();
+    return r;
+  }();
+  return ok;
 }
 
 // Lay the per-component tables out for the current window_load.
@@ -358,18 +429,37 @@ static int batch_layout(tk_ctx* c, tk_batch* b) {
     L.ct_boff = coff;
     L.ct_mask = cb - 1;
     coff += cb;
-    L.pad = 0;
+    L.flags = cb <= kCtLdsBuckets ? kCompCtInLds : 0u;
   }
   b->n_slots_words = boff * 8;
   b->n_ovf_pairs = ooff;
   b->n_ct_words = coff * 8;
+  // components whose table is built in LDS (TK_BATCH_CT_LDS=0: none); one
+  // launch takes them all, with the LDS of the largest
+  std::vector<uint32_t> small;
+  {
+    const char* ev = getenv("TK_BATCH_CT_LDS");
+    const bool on = !(ev && ev[0] == '0');
+    const uint64_t max_buckets = ct_lds_64k() ? kCtLdsBuckets : kCtLdsBuckets / 2;
+    for (uint64_t i = 0; i < b->n_comp; i++) {
+      if (!on || lay[i].ct_mask + 1 > max_buckets) lay[i].flags &= ~kCompCtInLds;
+      if ((lay[i].flags & kCompCtInLds) && b->contig_off[i + 1] > b->contig_off[i]) {
+        small.push_back((uint32_t)i);
+        b->ct_lds_bytes = std::max<uint64_t>(b->ct_lds_bytes, (lay[i].ct_mask + 1) * 64);
+      }
+    }
+  }
+  b->n_ct_lds = small.size();
   int rc = dev_alloc((void**)&b->d_slots, b->n_slots_words * 8);
   if (!rc) rc = dev_alloc((void**)&b->d_ovf, b->n_ovf_pairs * 16);
   if (!rc) rc = dev_alloc((void**)&b->d_ct, b->n_ct_words * 8);
   if (!rc) rc = dev_alloc((void**)&b->d_lay, b->n_comp * sizeof(CompLayout));
+  if (!rc) rc = dev_alloc((void**)&b->d_ct_lds, std::max<uint64_t>(b->n_ct_lds, 1) * 4);
   if (rc) return rc;
   TK_HIP(hipMemcpyAsync(b->d_lay, lay.data(), b->n_comp * sizeof(CompLayout), hipMemcpyHostToDevice,
                         c->stream));
+  if (b->n_ct_lds)
+    TK_HIP(hipMemcpyAsync(b->d_ct_lds, small.data(), b->n_ct_lds * 4, hipMemcpyHostToDevice, c->stream));
   TK_HIP(hipStreamSynchronize(c->stream));
   b->bytes = b->n_slots_words * 8 + b->n_ovf_pairs * 16 + b->n_ct_words * 8;
   return TK_OK;
@@ -443,6 +533,13 @@ static int batch_enqueue(tk_ctx* c, tk_batch* b, const tk_stats_opts* o, tk_read
     hipLaunchKernelGGL(k_fill_zero, dim3(grid(b->n_ct_words, 256)), dim3(256), 0, c->stream, b->d_ct, b->n_ct_words);
     const ReadsView cv = rview(b->contigs);
     const size_t vlds = (size_t)slab * 4 * wpb + (size_t)wpb * 256;
+    if (b->n_ct_lds && b->ct_packed)
+      hipLaunchKernelGGL(k_batch_ctable_lds<true>, dim3((unsigned)b->n_ct_lds), dim3(256), b->ct_lds_bytes, c->stream,
+                         cv, b->d_ct_lds, b->d_contig_off, b->d_lay, b->k, b->canonical, b->d_ct, c->d_flags);
+    else if (b->n_ct_lds)
+      hipLaunchKernelGGL(k_batch_ctable_lds<false>, dim3((unsigned)b->n_ct_lds), dim3(256), b->ct_lds_bytes,
+                         c->stream, cv, b->d_ct_lds, b->d_contig_off, b->d_lay, b->k, b->canonical, b->d_ct,
+                         c->d_flags);
     if (b->ct_packed) {
       if (cv.n_reads)
         hipLaunchKernelGGL(k_batch_ctable<true>, dim3(grid(cv.n_reads, 4)), dim3(256), 0, c->stream, cv,
@@ -578,6 +675,10 @@ int tk_batch_create(tk_ctx* c, int k, int canonical, const tk_reads* reads, cons
       (e = hipMemcpyAsync(b->d_groups, groups.data(), b->n_groups * sizeof(BatchGroup), hipMemcpyHostToDevice,
                           c->stream)) != hipSuccess)
     rc = hip_fail(e, "upload count groups");
+  if (!rc) rc = dev_alloc((void**)&b->d_contig_off, (n_comp + 1) * 8);
+  if (!rc && (e = hipMemcpyAsync(b->d_contig_off, contig_comp_off, (n_comp + 1) * 8, hipMemcpyHostToDevice,
+                                 c->stream)) != hipSuccess)
+    rc = hip_fail(e, "upload contig offsets");
   if (!rc) rc = dev_alloc((void**)&b->d_read_comp, std::max<uint64_t>(reads->n_reads, 1) * 4);
   if (!rc) rc = dev_alloc((void**)&b->d_contig_comp, std::max<uint64_t>(contigs->n_reads, 1) * 4);
   if (!rc) rc = dev_alloc((void**)&b->d_contig_local, std::max<uint64_t>(contigs->n_reads, 1) * 4);
@@ -721,6 +822,7 @@ int tk_batch_free(tk_batch* b) {
   dev_free(b->d_contig_local, std::max<uint64_t>(b->contigs->n_reads, 1) * 4);
   dev_free(b->d_groups, std::max<uint64_t>(b->n_groups, 1) * sizeof(BatchGroup));
   dev_free(b->d_order, b->reads->n_reads * 4);
+  dev_free(b->d_contig_off, (b->n_comp + 1) * 8);
   delete b;
   return TK_OK;
 }
