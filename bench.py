@@ -574,6 +574,163 @@ def bench_inchworm(args) -> dict:
                              "kind": "port", "sample": "the same dictionary, oracle/tk_oracle_iworm.c (serial)"}}
 
 
+def _kmer_keys(codes: np.ndarray, kk: int, canonical: bool = True) -> tuple[np.ndarray, np.ndarray]:
+    """(lo, hi) words of the kk-mers in the rows of `codes` (n x kk, values 0-3),
+    as tk_kset reads them: the 2kk-bit value's low 64 bits and the bits above;
+    canonical: the smaller of the k-mer and its reverse complement."""
+    def words(c):
+        lo = np.zeros(len(c), np.uint64)
+        hi = np.zeros(len(c), np.uint64)
+        for i in range(kk):
+            sh = 2 * (kk - 1 - i)
+            v = c[:, i].astype(np.uint64)
+            if sh >= 64:
+                hi |= v << np.uint64(sh - 64)
+            else:
+                lo |= v << np.uint64(sh)
+                if sh > 62:
+                    hi |= v >> np.uint64(64 - sh)
+        return lo, hi
+    lo, hi = words(codes)
+    if canonical:
+        rlo, rhi = words(3 - codes[:, ::-1])
+        use_rc = (rhi < hi) | ((rhi == hi) & (rlo < lo))
+        lo, hi = np.where(use_rc, rlo, lo), np.where(use_rc, rhi, hi)
+    return lo, hi
+
+
+def bench_kset(args) -> dict:
+    """Counts of a fixed k-mer set (tk_kset), the GPU part of GraphFromFasta
+    (Trinity:2180-2196) and QuantifyGraph (Trinity:2348-2356):
+    welds: 2 M distinct canonical 48-mers drawn from the 50k-transcript set
+    (candidate welds, each spanning two contigs) counted over every 48-mer
+    window of the first --kset-pairs pairs of the 20M-pair stream, reads
+    resident in HBM; nodes: QuantifyGraph --batch over the first 2,000
+    components of the components workload, every distinct 24-mer of each
+    component's contigs tagged by the component, counted over the
+    component's reads.  Each checked against the oracle on a read sample,
+    whose single-thread time is the CPU baseline."""
+    from trinityrnaseq_amd import api, synth
+    from oracle import oracle as O  # the checker and the CPU baseline only
+    ctx = api.Context(0)
+    rng = np.random.default_rng(4801)
+    t = time.perf_counter()
+    tx = synth.large_transcriptome(1002)
+    n_pairs = args.kset_pairs
+    host = np.empty(2 * n_pairs * READ_LEN, np.uint8)
+    synth.pe_reads(tx, n_pairs, READ_LEN, seed=1002, out=host)
+    off = np.arange(2 * n_pairs + 1, dtype=np.uint64) * np.uint64(READ_LEN)
+    lut = np.full(256, 0, np.uint8)
+    for ch, v in zip(b"ACGT", range(4)):
+        lut[ch] = v
+    picks = []
+    while sum(len(p) for p in picks) < 2_200_000:
+        j = int(rng.integers(0, tx.n))
+        sq = tx.seq(j).encode()
+        if len(sq) < 48:
+            continue
+        starts = rng.integers(0, len(sq) - 47, size=64)
+        arr = np.frombuffer(sq, np.uint8)
+        picks.append(lut[arr[starts[:, None] + np.arange(48)]])
+    codes = np.concatenate(picks)
+    lo, hi = _kmer_keys(codes, 48)
+    pairs = np.unique(np.stack([hi, lo], axis=1), axis=0)[:2_000_000]
+    keys = np.ascontiguousarray(np.stack([pairs[:, 1], pairs[:, 0]], axis=1))
+    log(f"kset: {len(keys)} weld 48-mers, {2 * n_pairs} reads in {time.perf_counter() - t:.1f}s")
+    reads = api.Reads(ctx, host, off)
+    ks = api.KSet(ctx, keys)
+    windows = 2 * n_pairs * (READ_LEN - 48 + 1)
+    ks.count(reads, 48, True)  # warm-up
+    ctx.timing(True)
+    ctx.timing_reset()
+    lat = []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        counts = ks.count(reads, 48, True)
+        lat.append(time.perf_counter() - t1)
+    kms, kn = ctx.kernel_time("kset_count")
+    ctx.timing(False)
+    step_s = sum(lat) / len(lat)
+    log(f"kset welds: {step_s * 1000:.1f} ms per count, kernel {kms / kn:.1f} ms, {int((counts > 0).sum())} supported")
+    # parity on a sample and the CPU baseline (single-thread oracle)
+    n_s = 200_000
+    s_seq = host[:n_s * READ_LEN].tobytes()
+    s_off = off[:n_s + 1]
+    s_reads = api.Reads(ctx, host[:n_s * READ_LEN], s_off)
+    got = ks.count(s_reads, 48, True)
+    t1 = time.perf_counter()
+    want = O.kset_count(keys, s_seq, s_off, 48, True)
+    cpu_s = time.perf_counter() - t1
+    if not np.array_equal(got, want):
+        raise RuntimeError("weld counts differ from the oracle's on the sample")
+    s_reads.free()
+    ks.free()
+    reads.free()
+    algo = windows * 64.375 + 2 * n_pairs * 12.0  # bucket line per window; length + offset per read
+    kernel_s = kms / kn / 1000.0
+    # QuantifyGraph --batch: tagged (component, 24-mer) node keys of 2,000 components
+    r_, ro_, rco, cseq, coff, cco = make_components(10_000)
+    nc = 2000
+    nr = int(rco[nc])
+    q_reads = api.Reads(ctx, r_[:int(ro_[nr])], ro_[:nr + 1])
+    tags = np.repeat(np.arange(nc, dtype=np.uint32), np.diff(rco[:nc + 1]).astype(np.int64))
+    ca = lut[np.frombuffer(cseq, np.uint8)[:int(coff[int(cco[nc])])]]
+    n_ct = int(cco[nc])
+    clen = np.diff(coff[:n_ct + 1]).astype(np.int64)
+    nwin = np.maximum(clen - 23, 0)
+    # window starts of every contig of the first nc components, and their component
+    first = np.repeat(coff[:n_ct].astype(np.int64) - np.cumsum(nwin) + nwin, nwin)
+    starts = first + np.arange(int(nwin.sum()), dtype=np.int64)
+    comp_of = np.repeat(np.repeat(np.arange(nc, dtype=np.uint64), np.diff(cco[:nc + 1]).astype(np.int64)), nwin)
+    lo24, _ = _kmer_keys(np.lib.stride_tricks.sliding_window_view(ca, 24)[starts], 24)
+    nk = np.unique(np.stack([lo24, comp_of], axis=1), axis=0)
+    qs = api.KSet(ctx, nk)
+    qs.count(q_reads, 24, True, tags)
+    ctx.timing(True)
+    ctx.timing_reset()
+    t1 = time.perf_counter()
+    qs.count(q_reads, 24, True, tags)
+    q_s = time.perf_counter() - t1
+    q_kms, _ = ctx.kernel_time("kset_count")
+    ctx.timing(False)
+    q_windows = nr * (READ_LEN - 24 + 1)
+    # the oracle on the first 200 components
+    n_q = int(rco[200])
+    qwant = O.kset_count(nk, r_[:int(ro_[n_q])].tobytes(), ro_[:n_q + 1], 24, True, tags[:n_q])
+    qs2 = api.KSet(ctx, nk)
+    q_reads2 = api.Reads(ctx, r_[:int(ro_[n_q])], ro_[:n_q + 1])
+    if not np.array_equal(qs2.count(q_reads2, 24, True, tags[:n_q]), qwant):
+        raise RuntimeError("node weights differ from the oracle's on the first 200 components")
+    for o in (qs, qs2, q_reads, q_reads2):
+        o.free()
+    log(f"kset nodes: {len(nk)} tagged 24-mers, {q_windows} windows in {q_s * 1000:.1f} ms "
+        f"(count kernel {q_kms:.1f} ms)")
+    ctx.close()
+    return {
+        "metric": METRIC + " [fixed k-mer set: weld 48-mers]", "value": windows / step_s, "unit": "k-mers/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(step_s * 1000, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u128",
+        "data": "synthetic (seeded transcriptome + reads generated in-process)",
+        "config": {"workload": f"GraphFromFasta weld support: {len(keys)} canonical 48-mers over {2 * n_pairs} "
+                               f"2x150 reads ({windows} windows), reads resident in HBM",
+                   "welds": int(len(keys)), "reads": 2 * n_pairs},
+        "latency_s": percentiles(lat),
+        "roofline": {"bound": "hbm", "kernel": "kset_count", "achieved": round(algo / kernel_s / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(algo / kernel_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(kms / kn, 3),
+                     "algorithmic_bytes_per_launch": round(algo)},
+        "supported_welds": int((counts > 0).sum()),
+        "cpu_baseline": {"value": n_s * (READ_LEN - 48 + 1) / cpu_s, "unit": "k-mers/s", "cores": 1,
+                         "kind": "port", "sample": f"first {n_s} reads, the same 2 M keys (or_kset_count, "
+                                                   f"single-threaded C)"},
+        "nodes": {"workload": f"QuantifyGraph --batch: {nc} components, {len(nk)} (component, 24-mer) node keys, "
+                              f"{nr} reads", "ms": round(q_s * 1000, 3), "kernel_ms": round(q_kms, 3),
+                  "k-mers/s_kernel": q_windows / (q_kms / 1000.0),
+                  "note": "ms includes gathering and downloading one u64 count per node key",
+                  "checked": "first 200 components equal to the oracle's"},
+    }
+
+
 def plain_job(ctx, seq: np.ndarray, off: np.ndarray, n: int, max_cov: int):
     """One sample through the plain per-job API: upload + pack, size, count,
     stats, select, selected flags back to the host, free everything."""
@@ -910,7 +1067,7 @@ def main() -> int:
                          "the count table after the timed steps")
     ap.add_argument("--copy-ref-gb", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GB (HBM counter reference)")
-    ap.add_argument("--workload", choices=["stress", "large_pe", "components", "serving", "inchworm", "cli"],
+    ap.add_argument("--workload", choices=["stress", "large_pe", "components", "serving", "inchworm", "cli", "kset"],
                     default="stress")
     ap.add_argument("--host-steps", type=int, default=2,
                     help="stress: jobs run from pageable host memory after the timed steps")
@@ -926,6 +1083,7 @@ def main() -> int:
                     help="serving: hold the samples in page-locked host memory")
     ap.add_argument("--hbm-budget-gb", type=float, default=96.0)
     ap.add_argument("--cli-pairs", type=int, default=2_000_000, help="cli: pairs of the one-sample comparison")
+    ap.add_argument("--kset-pairs", type=int, default=10_000_000, help="kset: pairs whose 48-mer windows are counted")
     ap.add_argument("--cli-samples", type=int, default=8, help="cli: samples of the stream")
     ap.add_argument("--cli-stream-pairs", type=int, default=1_000_000, help="cli: pairs per stream sample")
     ap.add_argument("--cli-clients", type=int, default=4, help="cli: concurrent clients of the service")
@@ -948,6 +1106,9 @@ def main() -> int:
         return 0
     if args.workload == "cli":
         print(json.dumps(bench_cli(args)), flush=True)
+        return 0
+    if args.workload == "kset":
+        print(json.dumps(bench_kset(args)), flush=True)
         return 0
 
     from trinityrnaseq_amd import api, synth
