@@ -561,6 +561,42 @@ def bench_inchworm(args) -> dict:
         log(f"inchworm {name}: {out[name]}")
         table.free()
         reads.free()
+    if args.iworm_pairs:
+        # Trinity's phase-1 Inchworm on the dictionary of a whole large sample (every k-mer of
+        # count >= 1 with entropy >= 1.0: Trinity:117,1050,2625), too large for the serial
+        # oracle: checked through invariants on the device instead (no k-mer in two places,
+        # every contig k-mer in the dictionary)
+        n = args.iworm_pairs
+        buf = np.empty(2 * n * READ_LEN, np.uint8)
+        synth.pe_reads(synth.large_transcriptome(1002), n, READ_LEN, seed=1002, out=buf)
+        reads = api.Reads(ctx, buf, np.arange(2 * n + 1, dtype=np.uint64) * np.uint64(READ_LEN))
+        del buf
+        est = api.estimate_distinct(ctx, reads, K, True)
+        table = api.Table(ctx, K, True, int(est * 1.05) + 4096, 0.5)
+        table.count(reads)
+        reads.free()
+        kept = table.prune(1, keep1)
+        log(f"inchworm full: {kept} k-mers in the dictionary; contigs next")
+        t = time.perf_counter()
+        contigs, sums, seeds, info = api.inchworm(ctx, table, 2, keep15)
+        g = time.perf_counter() - t
+        nk = int(sum(len(c) - K + 1 for c in contigs))
+        log(f"inchworm full: {len(contigs)} contigs, {nk} contig k-mers in {g:.2f}s ({info['rounds']} rounds)")
+        cr = api.Reads.from_list(ctx, contigs)
+        ct = api.Table(ctx, K, True, nk + 4096, 0.5)
+        ct.count(cr)
+        ck, cc = ct.dump()
+        once = len(ck) == nk and int(cc.max(initial=0)) <= 1
+        inside = bool(np.all(table.lookup(ck) >= 1))
+        if not (once and inside):
+            raise RuntimeError("inchworm full: a k-mer sits in two places or outside the dictionary")
+        out[f"large_pe_{n}_pairs"] = {"dictionary_kmers": int(kept), "contigs": len(contigs),
+                                       "contigs_ge_200nt": int(sum(len(c) >= 200 for c in contigs)),
+                                       "contig_kmers": nk, "gpu_s": round(g, 3), "gpu_kmers_per_s": nk / g,
+                                       "rounds": info["rounds"], "walks": info["walks"],
+                                       "checked": "every contig k-mer once, and in the dictionary"}
+        for o in (ct, cr, table):
+            o.free()
     ctx.close()
     first = out["large_pe_500k_pairs"]
     return {"metric": METRIC + " [inchworm contigs]", "value": first["gpu_kmers_per_s"], "unit": "contig k-mers/s",
@@ -1084,6 +1120,8 @@ def main() -> int:
     ap.add_argument("--hbm-budget-gb", type=float, default=96.0)
     ap.add_argument("--cli-pairs", type=int, default=2_000_000, help="cli: pairs of the one-sample comparison")
     ap.add_argument("--kset-pairs", type=int, default=10_000_000, help="kset: pairs whose 48-mer windows are counted")
+    ap.add_argument("--iworm-pairs", type=int, default=0,
+                    help="inchworm: also assemble the whole dictionary of this many pairs (0: skip)")
     ap.add_argument("--cli-samples", type=int, default=8, help="cli: samples of the stream")
     ap.add_argument("--cli-stream-pairs", type=int, default=1_000_000, help="cli: pairs per stream sample")
     ap.add_argument("--cli-clients", type=int, default=4, help="cli: concurrent clients of the service")
