@@ -169,7 +169,11 @@ __global__ void k_iw_renumber(uint32_t* __restrict__ claim, uint64_t n) {
 
 // One workgroup: the first `want` seeds from position `cursor` on whose slot
 // is unclaimed.  out[0] = number picked; out64[1] = the position after the
-// last one examined.
+// last one examined.  Each thread examines four consecutive positions per
+// pass (their seed and claim loads in flight together), ranked by a block
+// scan of the threads' counts, so a pass covers 4,096 positions.
+constexpr uint32_t kPickPer = 4;
+
 __global__ __launch_bounds__(1024) void k_iw_pick(const uint32_t* __restrict__ seeds, uint64_t n_seeds,
                                                   uint64_t cursor, const uint32_t* __restrict__ claim,
                                                   uint32_t want, uint32_t* __restrict__ picked_slot,
@@ -178,18 +182,34 @@ __global__ __launch_bounds__(1024) void k_iw_pick(const uint32_t* __restrict__ s
   __shared__ uint32_t s_wave[16];
   __shared__ uint32_t s_total;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t span = (uint64_t)kPickPer * blockDim.x;
   uint32_t got = 0;
   uint64_t base = cursor;
   uint64_t end = cursor;
   while (got < want && base < n_seeds) {
-    const uint64_t pos = base + threadIdx.x;
-    const bool fr = pos < n_seeds && claim[seeds[pos]] == kFree;
-    const uint64_t m = __ballot(fr);
-    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    const uint64_t p0 = base + (uint64_t)kPickPer * threadIdx.x;
+    uint32_t sl[kPickPer];
+    bool fr[kPickPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPickPer; j++) sl[j] = p0 + j < n_seeds ? seeds[p0 + j] : 0u;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPickPer; j++) {
+      fr[j] = p0 + j < n_seeds && claim[sl[j]] == kFree;
+      cnt += fr[j] ? 1u : 0u;
+    }
+    // inclusive scan of the counts within the wave, then across the waves
+    uint32_t x = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
       uint32_t acc = 0;
-      for (int w = 0; w < 16; w++) {
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
         const uint32_t c = s_wave[w];
         s_wave[w] = acc;
         acc += c;
@@ -197,21 +217,25 @@ __global__ __launch_bounds__(1024) void k_iw_pick(const uint32_t* __restrict__ s
       s_total = acc;
     }
     __syncthreads();
-    const uint32_t rank = got + s_wave[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
-    if (fr && rank < want) {
-      picked_slot[rank] = seeds[pos];
-      picked_pos[rank] = pos;
+    uint32_t rank = got + s_wave[wave] + x - cnt;
+#pragma unroll
+    for (uint32_t j = 0; j < kPickPer; j++) {
+      if (!fr[j]) continue;
+      if (rank < want) {
+        picked_slot[rank] = sl[j];
+        picked_pos[rank] = p0 + j;
+        if (rank == want - 1) out[1] = p0 + j + 1;  // the last picked seed ends the scan
+      }
+      rank++;
     }
     const uint32_t total = s_total;
     if (got + total >= want) {
-      // the last picked seed ends the scan
-      if (fr && rank == want - 1) out[1] = pos + 1;
       got = want;
     } else {
       got += total;
-      end = std::min<uint64_t>(base + 1024, n_seeds);
+      end = std::min<uint64_t>(base + span, n_seeds);
     }
-    base += 1024;
+    base += span;
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -477,11 +501,31 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
     return r2;
   };
   if ((rc = alloc_paths())) return rc;
-  std::vector<uint32_t> h_stat(Bmax), h_lost(Bmax), h_lens(2 * Bmax), h_depn(Bmax), h_deps((size_t)Bmax * kDeps),
-      h_owner(Bmax), h_rel(Bmax);
+  // the per-round host copies live in page-locked memory, so each is one DMA
+  // (copies to pageable memory are staged and wait on the host)
+  struct Pinned {
+    void* p = nullptr;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+  } pin;
+  // u32: stat, lost, lens (2), depn, owner, rel, walks, deps (kDeps) per walk;
+  // u64: sums, pos, woff per walk, woff's end, out (2)
+  const size_t u32s = (size_t)Bmax * (8 + kDeps), u64s = (size_t)Bmax * 3 + 1 + 2;
+  TK_HIP(hipHostMalloc(&pin.p, u64s * 8 + u32s * 4, hipHostMallocDefault));
+  uint64_t* h_sums = static_cast<uint64_t*>(pin.p);
+  uint64_t* h_pos = h_sums + Bmax;
+  uint64_t* h_woff = h_pos + Bmax;  // Bmax + 1
+  unsigned long long* h_out = reinterpret_cast<unsigned long long*>(h_woff + Bmax + 1);  // 2
+  uint32_t* h_stat = reinterpret_cast<uint32_t*>(h_out + 2);
+  uint32_t* h_lost = h_stat + Bmax;
+  uint32_t* h_lens = h_lost + Bmax;  // 2 * Bmax
+  uint32_t* h_depn = h_lens + 2 * (size_t)Bmax;
+  uint32_t* h_owner = h_depn + Bmax;
+  uint32_t* h_rel = h_owner + Bmax;
+  uint32_t* h_walks = h_rel + Bmax;
+  uint32_t* h_deps = h_walks + Bmax;  // Bmax * kDeps
   std::vector<uint8_t> late(Bmax);
-  std::vector<uint64_t> h_sums(Bmax), h_pos(Bmax), h_woff(Bmax + 1);
-  std::vector<uint32_t> h_walks(Bmax);
   uint64_t cursor = 0;
   uint32_t S = 0;
   std::string chunk;
@@ -491,7 +535,7 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
                 dclaim.as<uint32_t>(), n_slots);
       S = 1;
     }
-    unsigned long long h_out[2] = {0, 0};
+    h_out[0] = h_out[1] = 0;
     IW_LAUNCH(c, "iworm_pick", 1, 1024, k_iw_pick, dseeds.as<uint32_t>(), n_cand, cursor, dclaim.as<uint32_t>(), B,
               dpick.as<uint32_t>(), dpos.as<uint64_t>(), dout.as<unsigned long long>());
     TK_HIP(hipMemcpyAsync(h_out, dout.p, 16, hipMemcpyDeviceToHost, c->stream));
@@ -516,14 +560,14 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
               ddepn.as<uint32_t>());
     IW_LAUNCH(c, "iworm_owner", cdiv(n, 256), 256, k_iw_owner, dpick.as<uint32_t>(), n, dclaim.as<uint32_t>(),
               downer.as<uint32_t>());
-    TK_HIP(hipMemcpyAsync(h_depn.data(), ddepn.p, n * 4, hipMemcpyDeviceToHost, c->stream));
-    TK_HIP(hipMemcpyAsync(h_deps.data(), ddeps.p, (size_t)n * kDeps * 4, hipMemcpyDeviceToHost, c->stream));
-    TK_HIP(hipMemcpyAsync(h_owner.data(), downer.p, n * 4, hipMemcpyDeviceToHost, c->stream));
-    TK_HIP(hipMemcpyAsync(h_stat.data(), dstat.p, n * 4, hipMemcpyDeviceToHost, c->stream));
-    TK_HIP(hipMemcpyAsync(h_lost.data(), dlost.p, n * 4, hipMemcpyDeviceToHost, c->stream));
-    TK_HIP(hipMemcpyAsync(h_lens.data(), dlens.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
-    TK_HIP(hipMemcpyAsync(h_sums.data(), dsums.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
-    TK_HIP(hipMemcpyAsync(h_pos.data(), dpos.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_depn, ddepn.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_deps, ddeps.p, (size_t)n * kDeps * 4, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_owner, downer.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_stat, dstat.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_lost, dlost.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_lens, dlens.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_sums, dsums.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    TK_HIP(hipMemcpyAsync(h_pos, dpos.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     // Committed prefix, in rank order.  A walk whose seed an earlier walk of
     // the round took ("late skip") is the serial skip of that seed when the
@@ -564,7 +608,7 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
       any_rel = any_rel || h_rel[i];
     }
     if (any_rel) {
-      TK_HIP(hipMemcpyAsync(drel.p, h_rel.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+      TK_HIP(hipMemcpyAsync(drel.p, h_rel, n * 4, hipMemcpyHostToDevice, c->stream));
       IW_LAUNCH(c, "iworm_release", std::min<uint64_t>(cdiv((uint64_t)n * 2, 256), 4096), 256, k_iw_release, n, S,
                 lmax, dpick.as<uint32_t>(), drel.as<uint32_t>(), dlens.as<uint32_t>(), dpslot.as<uint32_t>(),
                 dclaim.as<uint32_t>());
@@ -597,8 +641,8 @@ int inchworm_run(tk_ctx* c, tk_table* t, uint32_t min_seed_cov, const uint8_t* s
     }
     if (nw) {
       if (dchars.bytes < total && (rc = dchars.alloc(std::max<uint64_t>(total, 1 << 20)))) return rc;
-      TK_HIP(hipMemcpyAsync(dwalks.p, h_walks.data(), (size_t)nw * 4, hipMemcpyHostToDevice, c->stream));
-      TK_HIP(hipMemcpyAsync(dwoff.p, h_woff.data(), (size_t)nw * 8, hipMemcpyHostToDevice, c->stream));
+      TK_HIP(hipMemcpyAsync(dwalks.p, h_walks, (size_t)nw * 4, hipMemcpyHostToDevice, c->stream));
+      TK_HIP(hipMemcpyAsync(dwoff.p, h_woff, (size_t)nw * 8, hipMemcpyHostToDevice, c->stream));
       IW_LAUNCH(c, "iworm_emit", nw, 256, k_iw_emit, dwalks.as<uint32_t>(), dwoff.as<uint64_t>(),
                 dpick.as<uint32_t>(), v, k, lmax, dlens.as<uint32_t>(), dpbase.as<uint8_t>(), dchars.as<char>());
       chunk.resize(total);
