@@ -311,6 +311,38 @@ def test_locality_order_changes_nothing(api, ctx, small_se, monkeypatch):
         r.free()
 
 
+@pytest.mark.parametrize("grab", ["0", "1", "4", "37"])
+def test_per_read_passes_ticket_modes(api, ctx, small_se, grab, monkeypatch):
+    """The scoring and vote passes with each XCD's waves taking runs of
+    `grab` reads from a per-XCD counter (0: a fixed stride per wave), in
+    locality and input order, short and long reads, against the oracle."""
+    say("ticket", grab)
+    monkeypatch.setenv("TK_READ_TICKET", grab)
+    seq0, off0 = small_se
+    long_ = random_reads(91, 30, 200, 3000, n_rate=0.002)
+    for ordered, extra in ((True, edge_reads()), (False, long_)):
+        monkeypatch.setenv("TK_ORDER_MIN_READS", "0" if ordered else "100000000")
+        eseq, eoff = concat(extra)
+        seq = seq0 + eseq
+        off = np.concatenate([off0, off0[-1] + eoff[1:]])
+        r, t, _, _ = _count(api, ctx, (seq, off))
+        ok, oc = O.count(seq, off, 25, True)
+        got = api.read_cov_stats(ctx, t, r, 2, 1, 0, 0)
+        want = O.read_stats(seq, off, ok, oc, 25, True, 2, 1, 0, 0)
+        assert np.array_equal(got, want.astype(got.dtype)), ordered
+        # vote against components made of the first reads
+        n_c = 200
+        comps = np.arange(n_c, dtype=np.uint32) // 3
+        cseq = seq[: int(off[n_c])]
+        ct = api.CTable(ctx, api.Reads(ctx, cseq, off[: n_c + 1]), comps, 25, True)
+        keys, cv = O.r2t_build(cseq, off[: n_c + 1], comps, 25, True)
+        gv = ct.vote(r)
+        wv = O.r2t_vote(seq, off, keys, cv, 25, True)
+        assert np.array_equal(gv, wv.astype(gv.dtype)), ordered
+        t.free()
+        r.free()
+
+
 @pytest.mark.parametrize("cache", ["2048", "4096"])
 def test_read_stats_cached_policies(api, ctx, small_se, cache, monkeypatch):
     """The ordered scoring pass with the block count cache (reads of at most

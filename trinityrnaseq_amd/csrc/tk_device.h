@@ -74,6 +74,41 @@ __device__ __forceinline__ uint64_t xcd_block(uint64_t b, uint64_t grid) {
   return (b & 7) * (grid >> 3) + (b >> 3);
 }
 
+// Per-XCD position counters of the per-read table passes: eight u32, one per
+// 256 B, cleared before each launch.
+constexpr uint32_t kTicketStride = 64;
+constexpr uint32_t kTicketWords = 8 * kTicketStride;
+
+// Positions [0, n) of one XCD's share of a per-read pass, for one wave.
+// kTicket: the XCD's waves take runs of `grab` consecutive positions from
+// that XCD's counter *ctr (one returning atomic per run; the next run is
+// taken while the current one is processed), so the positions in flight on
+// the XCD stay one narrow front of the order however far single waves drift
+// apart.  Otherwise the wave takes first, first + stride, ... (a fixed
+// stride, along which waves drift apart by thousands of reads on large sets).
+// Calls f(position) for each; every wave leaves once its next position is at
+// or past n.
+template <bool kTicket, class F>
+__device__ __forceinline__ void xcd_positions(uint32_t* ctr, uint64_t n, uint32_t grab, uint64_t first,
+                                              uint64_t stride, int lane, F&& f) {
+  if (!kTicket) {
+    for (uint64_t j = first; j < n; j += stride) f(j);
+    return;
+  }
+  // positions fit 32 bits: a share is at most 2^31 / 8 reads
+  const uint32_t n32 = (uint32_t)n;
+  uint32_t t = 0;
+  if (lane == 0) t = atomicAdd(ctr, grab);
+  uint32_t b = __shfl(t, 0);
+  while (b < n32) {
+    t = 0;
+    if (lane == 0) t = atomicAdd(ctr, grab);
+    const uint32_t e = n32 - b > grab ? b + grab : n32;
+    for (uint32_t j = b; j < e; j++) f(j);
+    b = __shfl(t, 0);
+  }
+}
+
 // Forward k-mer starting at base `pos` of the read whose first word is wbase.
 __device__ __forceinline__ void window_at(const ReadsView& rv, uint64_t wbase,
                                           uint32_t pos, int k, uint64_t& fwd,

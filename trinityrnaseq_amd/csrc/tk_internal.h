@@ -57,6 +57,7 @@ struct tk_ctx {
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, uint64_t>> times;
   uint32_t* d_flags = nullptr;     // 64 words of small device counters / error bits
+  uint32_t* d_tickets = nullptr;   // per-XCD position counters of the per-read passes (after d_flags)
   uint32_t* h_flags = nullptr;     // pinned mirror
   tk::Scratch scratch[8];          // reusable device scratch buffers (4-7: region-resident count)
   int num_cu = 0;

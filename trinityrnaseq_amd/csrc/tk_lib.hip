@@ -1475,9 +1475,10 @@ __global__ void k_lookup(TableView t, const uint64_t* __restrict__ keys, uint64_
 // =====================================================================
 // Per-read coverage statistics: one wavefront per read
 // =====================================================================
-template <bool kShort>
+template <bool kShort, bool kTicket>
 __device__ __forceinline__ void stats_pass(ReadsView rv, int k, int canonical, TableView t, StatsParams sp,
-                                           tk_read_stat* __restrict__ out, uint32_t slab) {
+                                           tk_read_stat* __restrict__ out, uint32_t slab, uint32_t* tickets,
+                                           uint32_t grab) {
   extern __shared__ uint32_t s_dyn[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -1496,25 +1497,28 @@ __device__ __forceinline__ void stats_pass(ReadsView rv, int k, int canonical, T
   const uint64_t x = blockIdx.x & 7;
   const uint64_t lo = x * per_xcd;
   const uint64_t hi = lo + per_xcd < rv.n_reads ? lo + per_xcd : rv.n_reads;
-  for (uint64_t i = lo + (uint64_t)(blockIdx.x >> 3) * wpb + wv; i < hi;
-       i += (uint64_t)(gridDim.x >> 3) * wpb) {
-    const uint64_t r = read_at(rv, i);
-    stats_read<kShort>(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
-  }
+  if (lo >= hi) return;
+  xcd_positions<kTicket>(kTicket ? tickets + x * kTicketStride : nullptr, hi - lo, grab, (blockIdx.x >> 3) * wpb + wv,
+                         (uint64_t)(gridDim.x >> 3) * wpb, lane, [&](uint64_t j) {
+                           const uint64_t r = read_at(rv, lo + j);
+                           stats_read<kShort>(rv, r, k, canonical, t, sp, vals, tmp, lane, out + r);
+                         });
 }
 
 // The scoring pass for any read lengths, and for read sets whose reads all
 // have at most 128 windows (no long-read path compiled: 77 instead of 86
 // VGPRs, 6 waves per SIMD instead of 5).
+template <bool kTicket>
 __global__ __launch_bounds__(256) void k_stats(ReadsView rv, int k, int canonical, TableView t,
                                                StatsParams sp, tk_read_stat* __restrict__ out,
-                                               uint32_t slab) {
-  stats_pass<false>(rv, k, canonical, t, sp, out, slab);
+                                               uint32_t slab, uint32_t* tickets, uint32_t grab) {
+  stats_pass<false, kTicket>(rv, k, canonical, t, sp, out, slab, tickets, grab);
 }
+template <bool kTicket>
 __global__ __launch_bounds__(256) void k_stats_short(ReadsView rv, int k, int canonical, TableView t,
                                                      StatsParams sp, tk_read_stat* __restrict__ out,
-                                                     uint32_t slab) {
-  stats_pass<true>(rv, k, canonical, t, sp, out, slab);
+                                                     uint32_t slab, uint32_t* tickets, uint32_t grab) {
+  stats_pass<true, kTicket>(rv, k, canonical, t, sp, out, slab, tickets, grab);
 }
 
 // k_stats for ordered reads of at most 128 windows each: every block takes
@@ -1697,9 +1701,11 @@ __global__ __launch_bounds__(256) void k_ctable_build(ReadsView rv, const uint32
     atomicAdd(reinterpret_cast<unsigned long long*>(&flags[2]), (unsigned long long)s_new);
 }
 
+template <bool kTicket>
 __global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical,
                                               const uint64_t* __restrict__ slots, uint64_t bmask,
-                                              tk_r2t_vote* __restrict__ out, uint32_t slab) {
+                                              tk_r2t_vote* __restrict__ out, uint32_t slab, uint32_t* tickets,
+                                              uint32_t grab) {
   extern __shared__ uint32_t s_dyn[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -1711,11 +1717,12 @@ __global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical
   const uint64_t x = blockIdx.x & 7;
   const uint64_t lo = x * per_xcd;
   const uint64_t hi = lo + per_xcd < rv.n_reads ? lo + per_xcd : rv.n_reads;
-  for (uint64_t i = lo + (uint64_t)(blockIdx.x >> 3) * wpb + wv; i < hi;
-       i += (uint64_t)(gridDim.x >> 3) * wpb) {
-    const uint64_t r = read_at(rv, i);
-    vote_read(rv, r, k, canonical, slots, 0, bmask, hits, tmp, lane, out + r);
-  }
+  if (lo >= hi) return;
+  xcd_positions<kTicket>(kTicket ? tickets + x * kTicketStride : nullptr, hi - lo, grab, (blockIdx.x >> 3) * wpb + wv,
+                         (uint64_t)(gridDim.x >> 3) * wpb, lane, [&](uint64_t j) {
+                           const uint64_t r = read_at(rv, lo + j);
+                           vote_read(rv, r, k, canonical, slots, 0, bmask, hits, tmp, lane, out + r);
+                         });
 }
 
 // Grid of the per-read table passes (k_stats, k_vote): at most TK_READ_BLOCKS_PER_CU
@@ -1727,6 +1734,19 @@ static int per_read_grid(tk_ctx* ctx, uint64_t n_reads, int wpb) {
   const uint64_t want = ceil_div(n_reads, (uint64_t)wpb);
   const uint64_t cap = per_cu ? (uint64_t)std::max(ctx->num_cu, 1) * per_cu : (uint64_t)0x7FFFFFF0;
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
+}
+
+// Context flag block: 64 words of small counters / error bits, then the
+// per-XCD ticket counters of the per-read passes.
+constexpr size_t kFlagBytes = (64 + kTicketWords) * 4;
+
+// Reads per ticket grab in the per-read table passes (TK_READ_TICKET, default
+// 4; 0 = each wave takes a fixed stride of the XCD's range instead).
+static uint32_t read_ticket_grab() {
+  const char* e = getenv("TK_READ_TICKET");
+  if (!e) return 4;
+  const unsigned long v = strtoul(e, nullptr, 10);
+  return v > 1024 ? 1024u : (uint32_t)v;
 }
 
 static int grid_for_waves(tk_ctx* ctx, uint64_t n_items, int waves_per_block) {
@@ -1807,10 +1827,11 @@ int tk_init(int device, tk_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
-  int rc = dev_alloc((void**)&c->d_flags, 64 * 4);
+  int rc = dev_alloc((void**)&c->d_flags, kFlagBytes);
   if (rc) { (void)hipStreamDestroy(c->stream); delete c; return rc; }
+  c->d_tickets = c->d_flags + 64;
   e = hipHostMalloc((void**)&c->h_flags, 64 * 4, hipHostMallocDefault);
-  if (e != hipSuccess) { dev_free(c->d_flags, 256); (void)hipStreamDestroy(c->stream); delete c; return hip_fail(e, "hipHostMalloc"); }
+  if (e != hipSuccess) { dev_free(c->d_flags, kFlagBytes); (void)hipStreamDestroy(c->stream); delete c; return hip_fail(e, "hipHostMalloc"); }
   *out = c;
   return TK_OK;
 }
@@ -1826,7 +1847,7 @@ int tk_ctx_free(tk_ctx* c) {
     if (c->stage[i]) (void)hipHostFree(c->stage[i]);
     if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
   }
-  dev_free(c->d_flags, 256);
+  dev_free(c->d_flags, kFlagBytes);
   (void)hipHostFree(c->h_flags);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -3105,12 +3126,21 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
     return TK_OK;
   }
   const size_t lds = (size_t)slab * 4 * wpb + (size_t)wpb * 8 * kLookupScratch;
-  if (max_nwin <= 128)
-    TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats_short, reads_view(r), t->k, t->canonical,
-              table_view(use), sp, d_out, slab);
+  const uint32_t grab = read_ticket_grab();
+  uint32_t* tickets = grab ? c->d_tickets : nullptr;
+  if (tickets) TK_LAUNCH(c, "tickets", 1, 256, 0, k_fill_u32, tickets, kTicketWords, 0u);
+  if (max_nwin <= 128 && tickets)
+    TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats_short<true>, reads_view(r), t->k, t->canonical,
+              table_view(use), sp, d_out, slab, tickets, grab);
+  else if (max_nwin <= 128)
+    TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats_short<false>, reads_view(r), t->k, t->canonical,
+              table_view(use), sp, d_out, slab, tickets, grab);
+  else if (tickets)
+    TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats<true>, reads_view(r), t->k, t->canonical, table_view(use),
+              sp, d_out, slab, tickets, grab);
   else
-    TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats, reads_view(r), t->k, t->canonical, table_view(use), sp,
-              d_out, slab);
+    TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats<false>, reads_view(r), t->k, t->canonical, table_view(use),
+              sp, d_out, slab, tickets, grab);
   return TK_OK;
 }
 
@@ -3338,8 +3368,16 @@ int tk_r2t_vote_reads_d(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t
   int rc = reads_order(c, r);
   if (rc) return rc;
   const int grid = (per_read_grid(c, r->n_reads, wpb) + 7) / 8 * 8;
-  TK_LAUNCH(c, "vote", grid, wpb * 64, (size_t)slab * 4 * wpb + (size_t)wpb * 256, k_vote, reads_view(r), t->k,
-            t->canonical, t->d_slots, t->nbuckets - 1, d_out, slab);
+  const uint32_t grab = read_ticket_grab();
+  uint32_t* tickets = grab ? c->d_tickets : nullptr;
+  if (tickets) TK_LAUNCH(c, "tickets", 1, 256, 0, k_fill_u32, tickets, kTicketWords, 0u);
+  const size_t lds = (size_t)slab * 4 * wpb + (size_t)wpb * 256;
+  if (tickets)
+    TK_LAUNCH(c, "vote", grid, wpb * 64, lds, k_vote<true>, reads_view(r), t->k, t->canonical, t->d_slots,
+              t->nbuckets - 1, d_out, slab, tickets, grab);
+  else
+    TK_LAUNCH(c, "vote", grid, wpb * 64, lds, k_vote<false>, reads_view(r), t->k, t->canonical, t->d_slots,
+              t->nbuckets - 1, d_out, slab, tickets, grab);
   return TK_OK;
 }
 
