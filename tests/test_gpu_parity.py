@@ -311,25 +311,30 @@ def test_locality_order_changes_nothing(api, ctx, small_se, monkeypatch):
         r.free()
 
 
-@pytest.mark.parametrize("grab", ["0", "1", "4", "37"])
-def test_per_read_passes_ticket_modes(api, ctx, small_se, grab, monkeypatch):
+@pytest.mark.parametrize("grab,pipe", [("0", "1"), ("1", "1"), ("4", "1"), ("37", "1"), ("4", "0")])
+def test_per_read_passes_ticket_modes(api, ctx, small_se, grab, pipe, monkeypatch):
     """The scoring and vote passes with each XCD's waves taking runs of
-    `grab` reads from a per-XCD counter (0: a fixed stride per wave), in
-    locality and input order, short and long reads, against the oracle."""
-    say("ticket", grab)
+    `grab` reads from a per-XCD counter (0: a fixed stride per wave), the
+    scoring of reads of at most 128 windows software-pipelined over a wave's
+    reads or one read at a time, in locality and input order, short and
+    long reads, every median / N-window policy, against the oracle."""
+    say("ticket", grab, "pipe", pipe)
     monkeypatch.setenv("TK_READ_TICKET", grab)
+    monkeypatch.setenv("TK_STATS_PIPE", pipe)
     seq0, off0 = small_se
+    short = edge_reads() + random_reads(92, 300, 0, 152, n_rate=0.01) + ["ACGTTGCA" * 19]  # 152 bases: 128 windows
     long_ = random_reads(91, 30, 200, 3000, n_rate=0.002)
-    for ordered, extra in ((True, edge_reads()), (False, long_)):
+    for ordered, extra in ((True, short), (False, long_), (False, short)):
         monkeypatch.setenv("TK_ORDER_MIN_READS", "0" if ordered else "100000000")
         eseq, eoff = concat(extra)
         seq = seq0 + eseq
         off = np.concatenate([off0, off0[-1] + eoff[1:]])
         r, t, _, _ = _count(api, ctx, (seq, off))
         ok, oc = O.count(seq, off, 25, True)
-        got = api.read_cov_stats(ctx, t, r, 2, 1, 0, 0)
-        want = O.read_stats(seq, off, ok, oc, 25, True, 2, 1, 0, 0)
-        assert np.array_equal(got, want.astype(got.dtype)), ordered
+        for median_policy, nwin_policy, min_count in ((0, 0, 2), (1, 1, 1), (2, 0, 3), (2, 1, 2)):
+            got = api.read_cov_stats(ctx, t, r, min_count, 1, nwin_policy, median_policy)
+            want = O.read_stats(seq, off, ok, oc, 25, True, min_count, 1, nwin_policy, median_policy)
+            assert np.array_equal(got, want.astype(got.dtype)), (ordered, median_policy, nwin_policy, min_count)
         # vote against components made of the first reads
         n_c = 200
         comps = np.arange(n_c, dtype=np.uint32) // 3

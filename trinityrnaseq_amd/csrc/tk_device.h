@@ -109,6 +109,35 @@ __device__ __forceinline__ void xcd_positions(uint32_t* ctr, uint64_t n, uint32_
   }
 }
 
+// The positions of xcd_positions<true>, one at a time: next() gives the
+// wave's next position of [0, n), and n once the XCD's share is used up.
+struct TicketStream {
+  uint32_t* ctr;
+  uint32_t n, grab;
+  uint32_t cb = 0, ce = 0;  // current run [cb, ce)
+  uint32_t tk = 0;          // lane 0: start of the run taken next
+  bool done = false;
+  int lane;
+  __device__ __forceinline__ TicketStream(uint32_t* c, uint32_t n_, uint32_t g, int l)
+      : ctr(c), n(n_), grab(g), lane(l) {
+    if (lane == 0) tk = atomicAdd(ctr, grab);
+  }
+  __device__ __forceinline__ uint32_t next() {
+    if (cb == ce) {
+      if (done) return n;
+      const uint32_t s = __shfl(tk, 0);
+      if (s >= n) {
+        done = true;
+        return n;
+      }
+      cb = s;
+      ce = n - s > grab ? s + grab : n;
+      if (lane == 0) tk = atomicAdd(ctr, grab);
+    }
+    return cb++;
+  }
+};
+
 // Forward k-mer starting at base `pos` of the read whose first word is wbase.
 __device__ __forceinline__ void window_at(const ReadsView& rv, uint64_t wbase,
                                           uint32_t pos, int k, uint64_t& fwd,
@@ -139,6 +168,23 @@ __device__ __forceinline__ void window_at_mask(const ReadsView& rv, uint64_t wba
   const uint32_t m = o ? ((mhi << o) | (mlo >> (32 - o))) : mhi;
   fwd = x >> (64 - 2 * k);
   nmask = m >> (32 - k);
+}
+
+// window_at over a read whose words are spread over the wave: lane j holds
+// code word j of the read (cw) and its mask word (mw), for j up to the word
+// after the read's last.  pos < 128, so words 0..4 are read.
+__device__ __forceinline__ void window_lanes(uint64_t cw, uint32_t mw, uint32_t pos, int k, uint64_t& fwd,
+                                             bool& valid) {
+  const int w = (int)(pos >> 5);
+  const uint32_t o = pos & 31u;
+  const uint64_t hi = __shfl(cw, w);
+  const uint64_t lo = __shfl(cw, w + 1);
+  const uint32_t mhi = __shfl(mw, w);
+  const uint32_t mlo = __shfl(mw, w + 1);
+  const uint64_t x = o ? ((hi << (2 * o)) | (lo >> (64 - 2 * o))) : hi;
+  const uint32_t m = o ? ((mhi << o) | (mlo >> (32 - o))) : mhi;
+  fwd = x >> (64 - 2 * k);
+  valid = (m >> (32 - k)) == 0u;
 }
 
 __device__ __forceinline__ uint64_t canon_key(uint64_t fwd, int k, int canonical) {

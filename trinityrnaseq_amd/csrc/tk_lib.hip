@@ -1521,6 +1521,71 @@ __global__ __launch_bounds__(256) void k_stats_short(ReadsView rv, int k, int ca
   stats_pass<true, kTicket>(rv, k, canonical, t, sp, out, slab, tickets, grab);
 }
 
+// k_stats_short over ticket runs, software-pipelined over each wave's reads:
+// while read j is scored, read j+1's code words (spread over lanes 0-5),
+// read j+2's length and first word and read j+3's index in the order are
+// loaded, so a read's table lookups wait on one memory round trip instead of
+// four dependent ones (order -> length / offset -> codes -> buckets).
+__global__ __launch_bounds__(256) void k_stats_pipe(ReadsView rv, int k, int canonical, TableView t,
+                                                    StatsParams sp, tk_read_stat* __restrict__ out,
+                                                    uint32_t* tickets, uint32_t grab) {
+  extern __shared__ uint64_t s_pipe[];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  uint64_t* tmp = s_pipe + wv * kLookupScratch;
+  const uint64_t per_xcd = (rv.n_reads + 7) >> 3;
+  const uint64_t x = blockIdx.x & 7;
+  const uint64_t lo = x * per_xcd;
+  const uint64_t hi = lo + per_xcd < rv.n_reads ? lo + per_xcd : rv.n_reads;
+  if (lo >= hi) return;
+  const uint32_t n = (uint32_t)(hi - lo);
+  TicketStream ts(tickets + x * kTicketStride, n, grab, lane);
+  // stage a: index known; b: length and first word known; c: words known
+  bool va = false, vb = false, vc = false;
+  uint32_t ra = 0, rb = 0, rc = 0, Lb = 0, Lc = 0, mwc = 0;
+  uint64_t wbb = 0, cwc = 0;
+  for (;;) {
+    const uint32_t p = ts.next();
+    const bool vn = p < n;
+    uint32_t rn = 0;
+    if (vn) rn = rv.order ? rv.order[lo + p] : (uint32_t)(lo + p);
+    uint32_t La = 0;
+    uint64_t wba = 0;
+    if (va) {
+      La = rv.len[ra];
+      wba = rv.word_off[ra];
+    }
+    uint64_t cwb = 0;
+    uint32_t mwb = 0;
+    if (vb && lane < 6 && (uint32_t)lane <= ((Lb + 31) >> 5)) {
+      cwb = rv.codes[wbb + lane];
+      mwb = rv.nmask[wbb + lane];
+    }
+    if (vc) {
+      const uint32_t nwin = Lc >= (uint32_t)k ? Lc - k + 1 : 0;
+      uint32_t nn, med;
+      uint64_t sum, sqlo, sqhi;
+      stats_short_core(nwin, k, canonical, t, sp, tmp, lane,
+                       [&](uint32_t pos, uint64_t& f, bool& v) { window_lanes(cwc, mwc, pos, k, f, v); }, nn, med,
+                       sum, sqlo, sqhi);
+      write_stat(lane, med, nn, sum, sqlo, sqhi, out + rc);
+      wave_sync();
+    }
+    vc = vb;
+    rc = rb;
+    Lc = Lb;
+    cwc = cwb;
+    mwc = mwb;
+    vb = va;
+    rb = ra;
+    Lb = La;
+    wbb = wba;
+    va = vn;
+    ra = rn;
+    if (!va && !vb && !vc) break;
+  }
+}
+
 // k_stats for ordered reads of at most 128 windows each: every block takes
 // kCacheChunk consecutive reads of the order (four at a time, one per wave)
 // and keeps the counts it looked up in an LDS cache of CACHE slots shared by
@@ -1739,6 +1804,13 @@ static int per_read_grid(tk_ctx* ctx, uint64_t n_reads, int wpb) {
 // Context flag block: 64 words of small counters / error bits, then the
 // per-XCD ticket counters of the per-read passes.
 constexpr size_t kFlagBytes = (64 + kTicketWords) * 4;
+
+// Scoring reads of at most 128 windows with the software-pipelined kernel
+// (TK_STATS_PIPE, default 1; 0 = k_stats_short, one read at a time).
+static bool stats_pipe() {
+  const char* e = getenv("TK_STATS_PIPE");
+  return !e || e[0] != '0';
+}
 
 // Reads per ticket grab in the per-read table passes (TK_READ_TICKET, default
 // 4; 0 = each wave takes a fixed stride of the XCD's range instead).
@@ -3129,7 +3201,11 @@ int tk_read_cov_stats_d(tk_ctx* c, tk_table* t, const tk_reads* r, const tk_stat
   const uint32_t grab = read_ticket_grab();
   uint32_t* tickets = grab ? c->d_tickets : nullptr;
   if (tickets) TK_LAUNCH(c, "tickets", 1, 256, 0, k_fill_u32, tickets, kTicketWords, 0u);
-  if (max_nwin <= 128 && tickets)
+  if (max_nwin <= 128 && tickets && stats_pipe()) {
+    const int pgrid = (per_read_grid(c, r->n_reads, 4) + 7) / 8 * 8;
+    TK_LAUNCH(c, "stats", pgrid, 256, 4 * 8 * kLookupScratch, k_stats_pipe, reads_view(r), t->k, t->canonical,
+              table_view(use), sp, d_out, tickets, grab);
+  } else if (max_nwin <= 128 && tickets)
     TK_LAUNCH(c, "stats", grid, wpb * 64, lds, k_stats_short<true>, reads_view(r), t->k, t->canonical,
               table_view(use), sp, d_out, slab, tickets, grab);
   else if (max_nwin <= 128)

@@ -85,6 +85,66 @@ __device__ __forceinline__ void wave_sum_u128(uint64_t& lo, uint64_t& hi) {
   }
 }
 
+// The statistics of a read of at most 128 windows, two per lane with their
+// values in registers: win(pos, fwd, valid) gives the forward k-mer at
+// window `pos` (< 128) and whether it is all ACGT; only windows below nwin
+// count.  Every lane calls win for both of its positions.
+template <class Win>
+__device__ __forceinline__ void stats_short_core(uint32_t nwin, int k, int canonical, const TableView& t,
+                                                 const StatsParams& sp, uint64_t* tmp, int lane, Win&& win,
+                                                 uint32_t& n, uint32_t& med, uint64_t& sum, uint64_t& sqlo,
+                                                 uint64_t& sqhi) {
+  const uint32_t pa = lane, pb = 64 + lane;
+  uint64_t fa = 0, fb = 0;
+  bool va = false, vb = false;
+  win(pa, fa, va);
+  win(pb, fb, vb);
+  va = va && pa < nwin;
+  vb = vb && pb < nwin;
+  const uint64_t keys[2] = {va ? canon_key(fa, k, canonical) : 0, vb ? canon_key(fb, k, canonical) : 0};
+  const bool want[2] = {va, vb};
+  uint64_t c[2];
+  coop4_get<2>(t, keys, want, tmp, lane, c);
+  bool ha, hb_;
+  uint32_t xa, xb;
+  window_value(sp, pa < nwin, va, c[0], ha, xa);
+  window_value(sp, pb < nwin, vb, c[1], hb_, xb);
+  n = (uint32_t)__popcll(__ballot(ha)) + (uint32_t)__popcll(__ballot(hb_));
+  sum = (uint64_t)xa + xb;
+  sqlo = sqhi = 0;
+  sq_add(xa, sqlo, sqhi);
+  sq_add(xb, sqlo, sqhi);
+  sum = wave_sum_u64(sum);
+  wave_sum_u128(sqlo, sqhi);
+  med = 0;
+  if (n) {
+    const uint32_t vmax = wave_max_u32(xa > xb ? xa : xb);
+    const int hb = 31 - __clz(vmax | 1u);
+    if (sp.median_policy == TK_MEDIAN_LOWER) {
+      med = wave_select2(ha, xa, hb_, xb, (n - 1) / 2, hb);
+    } else if (sp.median_policy == TK_MEDIAN_MEAN_FLOOR && !(n & 1)) {
+      const uint64_t a = wave_select2(ha, xa, hb_, xb, n / 2 - 1, hb);
+      const uint64_t b = wave_select2(ha, xa, hb_, xb, n / 2, hb);
+      med = (uint32_t)((a + b) / 2);
+    } else {
+      med = wave_select2(ha, xa, hb_, xb, n / 2, hb);
+    }
+  }
+}
+
+__device__ __forceinline__ void write_stat(int lane, uint32_t med, uint32_t n, uint64_t sum, uint64_t sqlo,
+                                           uint64_t sqhi, tk_read_stat* out) {
+  if (lane == 0) {
+    tk_read_stat st;
+    st.median = med;
+    st.nwin = n;
+    st.sum = sum;
+    st.sumsq_lo = sqlo;
+    st.sumsq_hi = sqhi;
+    *out = st;
+  }
+}
+
 // fastaToKmerCoverageStats for read r, one wavefront: exact integer sums and
 // the median by radix select.  Reads of up to 128 windows keep their values
 // in registers (two per lane); longer reads stage them in `vals` (LDS, >=
@@ -102,38 +162,13 @@ __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int ca
   uint32_t n = 0, vmax = 0, med = 0;
   uint64_t sum = 0, sqlo = 0, sqhi = 0;
   if (kShortOnly || nwin <= 128) {
-    const uint32_t pa = lane, pb = 64 + lane;
-    uint64_t fa = 0, fb = 0;
-    bool va = false, vb = false;
-    if (pa < nwin) window_at(rv, wb, pa, k, fa, va);
-    if (pb < nwin) window_at(rv, wb, pb, k, fb, vb);
-    const uint64_t keys[2] = {va ? canon_key(fa, k, canonical) : 0, vb ? canon_key(fb, k, canonical) : 0};
-    const bool want[2] = {va, vb};
-    uint64_t c[2];
-    coop4_get<2>(t, keys, want, tmp, lane, c);
-    bool ha, hb_;
-    uint32_t xa, xb;
-    window_value(sp, pa < nwin, va, c[0], ha, xa);
-    window_value(sp, pb < nwin, vb, c[1], hb_, xb);
-    n = (uint32_t)__popcll(__ballot(ha)) + (uint32_t)__popcll(__ballot(hb_));
-    sum = (uint64_t)xa + xb;
-    sq_add(xa, sqlo, sqhi);
-    sq_add(xb, sqlo, sqhi);
-    sum = wave_sum_u64(sum);
-    wave_sum_u128(sqlo, sqhi);
-    if (n) {
-      vmax = wave_max_u32(xa > xb ? xa : xb);
-      const int hb = 31 - __clz(vmax | 1u);
-      if (sp.median_policy == TK_MEDIAN_LOWER) {
-        med = wave_select2(ha, xa, hb_, xb, (n - 1) / 2, hb);
-      } else if (sp.median_policy == TK_MEDIAN_MEAN_FLOOR && !(n & 1)) {
-        const uint64_t a = wave_select2(ha, xa, hb_, xb, n / 2 - 1, hb);
-        const uint64_t b = wave_select2(ha, xa, hb_, xb, n / 2, hb);
-        med = (uint32_t)((a + b) / 2);
-      } else {
-        med = wave_select2(ha, xa, hb_, xb, n / 2, hb);
-      }
-    }
+    stats_short_core(nwin, k, canonical, t, sp, tmp, lane,
+                     [&](uint32_t pos, uint64_t& f, bool& v) {
+                       f = 0;
+                       v = false;
+                       if (pos < nwin) window_at(rv, wb, pos, k, f, v);
+                     },
+                     n, med, sum, sqlo, sqhi);
   } else {
     // windows are taken 128 at a time (two per lane), values appended to
     // vals in window order
@@ -179,15 +214,7 @@ __device__ inline void stats_read(const ReadsView& rv, uint64_t r, int k, int ca
       }
     }
   }
-  if (lane == 0) {
-    tk_read_stat st;
-    st.median = med;
-    st.nwin = n;
-    st.sum = sum;
-    st.sumsq_lo = sqlo;
-    st.sumsq_hi = sqhi;
-    *out = st;
-  }
+  write_stat(lane, med, n, sum, sqlo, sqhi, out);
   wave_sync();
 }
 
