@@ -59,8 +59,8 @@ TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (211.8 + 1.3) / 5.
 # profiles/stress_150M_pairs_trace.md: FETCH_SIZE + WRITE_SIZE of one job at
 # face value (their random 64-B line reads read as 64 B each, atomics as 32 B
 # of WRITE_SIZE).
-STRESS_TRAFFIC_B_PER_WIN = {"count": (880.30 + 574.82) / 37.8, "stats": (2487.01 + 9.63) / 37.8,
-                            "vote": (2241.93 + 129.45) / 37.8, "count_merge": (87.79 + 115.15) / 37.8,
+STRESS_TRAFFIC_B_PER_WIN = {"count": (880.30 + 574.82) / 37.8, "stats": (861.49 + 12.01) / 37.8,
+                            "vote": (920.67 + 116.66) / 37.8, "count_merge": (87.79 + 115.15) / 37.8,
                             "count_scatter": (130.83 + 307.42) / 37.8, "count_hist": (116.33 + 14.80) / 37.8,
                             "count_regions": (103.86 + 17.54 + 129.94 + 64.96) / 37.8}
 
@@ -69,7 +69,7 @@ STRESS_TRAFFIC_B_PER_WIN = {"count": (880.30 + 574.82) / 37.8, "stats": (2487.01
 KERNEL_NAMES = ["pack", "read_words", "hll", "sort_pairs", "minimizer", "count", "count_merge", "count_hist",
                 "count_scatter", "count_bounds", "count_regions", "count_retry", "ovf_fold", "ovf_rehash", "rehash",
                 "stats", "select", "select_flags", "scan_u32", "scan_u64", "windows", "gather_stats", "prune",
-                "ctable_build", "vote", "dump", "dump_bins", "dump_count", "histo"]
+                "ctable_build", "vote", "dump", "dump_bins", "dump_count", "histo", "tickets"]
 
 
 def log(*a):
