@@ -2567,7 +2567,10 @@ static bool resident_plan(tk_ctx* c, uint64_t windows, uint64_t* cap, uint64_t* 
     (void)hipGetLastError();
     return false;
   }
-  uint64_t held = 0;  // this context's own resident scratch counts as available
+  // this context's own resident scratch counts as available, and so do the
+  // thread's idle cached blocks (dev_alloc hands them back when a stream
+  // does not fit beside them)
+  uint64_t held = t_cache ? t_cache->cached : 0;
   for (int i = kSlotResA; i <= kSlotResMisc; i++) held += c->scratch[i].bytes;
   const uint64_t budget = t_resident_limit ? t_resident_limit : (uint64_t)(0.4 * (double)(fr + held));
   *cap_max = budget / 17;  // two 8-B streams + histograms
