@@ -54,7 +54,7 @@ VOTE_B_PER_WIN, VOTE_B_PER_READ = 64.375, 24.0
 # 5.04 G windows, counters calibrated against kernels of the same access shape), from
 # profiles/large_pe_20M_pairs_count_stats.md.  The counters cannot be read inside this
 # process, so the profiled figures are carried here for roofline.traffic.
-TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (211.8 + 1.3) / 5.04}
+TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (143.49 + 1.61) / 5.04}
 # The same for the stressed sample's kernels (37.8 G windows per step), from
 # profiles/stress_150M_pairs_trace.md: FETCH_SIZE + WRITE_SIZE of one job at
 # face value (their random 64-B line reads read as 64 B each, atomics as 32 B
