@@ -348,6 +348,27 @@ def test_per_read_passes_ticket_modes(api, ctx, small_se, grab, pipe, monkeypatc
         r.free()
 
 
+@pytest.mark.parametrize("k,canonical", [(31, True), (31, False), (13, True), (1, True), (20, False)])
+def test_pipelined_scoring_k_and_strand(api, ctx, k, canonical, monkeypatch):
+    """The software-pipelined scoring kernel (reads of at most 128 windows,
+    their code words spread over lanes 0-5) for other k and strand-specific
+    counts, reads of every length up to 127 + k bases (128 windows) with
+    Ns, in locality order, against the oracle."""
+    say("pipelined scoring k", k, "canonical", canonical)
+    monkeypatch.setenv("TK_STATS_PIPE", "1")
+    monkeypatch.setenv("TK_READ_TICKET", "4")
+    monkeypatch.setenv("TK_ORDER_MIN_READS", "0")
+    seqs = random_reads(300 + k, 3000, 0, 127 + k, n_rate=0.01) + ["ACGTTGCAT"[: (k % 9) + 1] * 40][:1]
+    seqs = [x[: 127 + k] for x in seqs] + ["G" * (127 + k), "T" * (126 + k)]
+    seq, off = concat(seqs)
+    r, t, _, _ = _count(api, ctx, (seq, off), k, canonical)
+    ok, oc = O.count(seq, off, k, canonical)
+    for median_policy, nwin_policy, min_count in ((0, 0, 1), (1, 1, 2), (2, 0, 2)):
+        got = api.read_cov_stats(ctx, t, r, min_count, 1, nwin_policy, median_policy)
+        want = O.read_stats(seq, off, ok, oc, k, canonical, min_count, 1, nwin_policy, median_policy)
+        assert np.array_equal(got, want.astype(got.dtype)), (median_policy, nwin_policy, min_count)
+
+
 @pytest.mark.parametrize("cache", ["2048", "4096"])
 def test_read_stats_cached_policies(api, ctx, small_se, cache, monkeypatch):
     """The ordered scoring pass with the block count cache (reads of at most
