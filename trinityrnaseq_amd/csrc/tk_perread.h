@@ -358,12 +358,20 @@ __device__ __forceinline__ uint32_t ctable_get(const uint64_t* slots, uint64_t b
   uint64_t b = mix64(key) & bmask;
   for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
     const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(slots + ((boff + b) << 3));
-    ulonglong2 v[4] = {l2[0], l2[1], l2[2], l2[3]};
+    // the first slot holding the key or empty decides; the slots are taken
+    // one by one into registers (an array indexed after early returns went
+    // to scratch memory)
+    uint32_t res = 0xFFFFFFFFu;
+    bool done = false;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (v[j].x == tagged) return ~(uint32_t)v[j].y;
-      if (v[j].x == 0) return 0xFFFFFFFFu;
+      const ulonglong2 v = l2[j];
+      const bool hit = !done && v.x == tagged;
+      const bool empty = !done && v.x == 0;
+      res = hit ? ~(uint32_t)v.y : res;
+      done = done || hit || empty;
     }
+    if (done) return res;
     b = (b + 1) & bmask;
   }
   return 0xFFFFFFFFu;
@@ -407,14 +415,23 @@ __device__ inline uint32_t pctable_get(const uint64_t* slots, uint64_t boff, uin
   uint64_t b = mix64(key) & bmask;
   for (uint64_t p = 0; p <= bmask && p < 4096; p++) {
     const ulonglong2* l2 = reinterpret_cast<const ulonglong2*>(slots + ((boff + b) << 3));
-    ulonglong2 v[4] = {l2[0], l2[1], l2[2], l2[3]};
+    // first slot holding the key or empty decides (slots in registers, as
+    // in ctable_get)
+    uint32_t res = 0xFFFFFFFFu;
+    bool done = false;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if ((v[j].x & km) == tagged) return pctable_comp(v[j].x, k);
-      if (v[j].x == 0) return 0xFFFFFFFFu;
-      if ((v[j].y & km) == tagged) return pctable_comp(v[j].y, k);
-      if (v[j].y == 0) return 0xFFFFFFFFu;
+      const ulonglong2 v = l2[j];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint64_t w = h ? v.y : v.x;
+        const bool hit = !done && (w & km) == tagged;
+        const bool empty = !done && w == 0;
+        res = hit ? pctable_comp(w, k) : res;
+        done = done || hit || empty;
+      }
     }
+    if (done) return res;
     b = (b + 1) & bmask;
   }
   return 0xFFFFFFFFu;
