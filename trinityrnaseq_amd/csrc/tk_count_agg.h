@@ -43,7 +43,8 @@ constexpr int kReadsPerBlock = 64;
 template <int SLOTS, bool EMIT, bool KHI, int WPR, class Groups>
 __device__ __forceinline__ void count_agg(ReadsView rv, const Groups& groups, int k, int canonical, TableView t,
                                           uint32_t* __restrict__ flags, uint64_t* __restrict__ out, uint64_t cap,
-                                          uint64_t chunk, unsigned long long* __restrict__ cursor) {
+                                          uint64_t chunk, unsigned long long* __restrict__ cursor,
+                                          int table_ok = 1) {
   extern __shared__ uint64_t s_tab[];  // SLOTS words, 0 = empty
   __shared__ uint32_t s_off[kReadsPerBlock + 1];
   __shared__ uint64_t s_wb[kReadsPerBlock];
@@ -230,7 +231,13 @@ __device__ __forceinline__ void count_agg(ReadsView rv, const Groups& groups, in
           if (tid == 0 && cur + sum <= lim) s_cur = cur + sum;
         }
         // 3. ...and adds its distinct keys to the global table, 64 at a time
-        //    (EMIT: only a flush past cap, one lane per key)
+        //    (EMIT: only a flush past cap, one lane per key; while the table
+        //    is not yet cleared, table_ok = 0, such a flush is dropped and
+        //    flags[6] set: the caller clears the table and merges again)
+        if (EMIT && !table_ok && nd) {
+          if (lane == 0) atomicOr(&flags[6], 1u);
+          nd = 0;
+        }
         for (uint32_t e0 = 0; e0 < nd; e0 += 64) {
           const bool want = e0 + lane < nd;
           const uint64_t v = want ? quarter[e0 + lane] : 0;
@@ -278,8 +285,10 @@ struct RangeGroups {
 template <int SLOTS, bool EMIT, bool KHI = true, int WPR = 1>
 __global__ __launch_bounds__(256, EMIT ? (SLOTS <= 3072 ? 6 : 5) : 1) void k_count_agg(
     ReadsView rv, uint64_t r_begin, uint64_t r_end, int k, int canonical, TableView t, uint32_t* __restrict__ flags,
-    uint64_t* __restrict__ out, uint64_t cap, uint64_t chunk, unsigned long long* __restrict__ cursor) {
-  count_agg<SLOTS, EMIT, KHI, WPR>(rv, RangeGroups{r_begin, r_end}, k, canonical, t, flags, out, cap, chunk, cursor);
+    uint64_t* __restrict__ out, uint64_t cap, uint64_t chunk, unsigned long long* __restrict__ cursor,
+    int table_ok) {
+  count_agg<SLOTS, EMIT, KHI, WPR>(rv, RangeGroups{r_begin, r_end}, k, canonical, t, flags, out, cap, chunk, cursor,
+                                   table_ok);
 }
 
 }  // namespace tk

@@ -2631,8 +2631,11 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     chunk = forced_reads ? forced_reads : (uint64_t)(0.9 * ratio * (double)cap / std::max(win_per_read, 1.0));
     chunk = std::min<uint64_t>(n_all, std::max<uint64_t>(chunk, kReadsPerBlock));
     echunk = res_echunk((uint64_t)((double)chunk * win_per_read), mgrid);
-    // flushes past a stream go straight to the table, which must be clear
-    if ((rc = table_zero(c, t))) return rc;
+    // A flush past a stream goes straight to the table, which must then be
+    // clear: while it is not (a new table), the merge drops such a flush and
+    // says so, and the chunk is merged again on the cleared table.  The first
+    // chunk's region pass writes every region, so a chunk that fits its
+    // stream costs no clearing pass.
   }
   void *pa = nullptr, *pb = nullptr;
   if ((rc = ensure_scratch(c, kSlotResA, cap * 8, &pa))) return rc;
@@ -2647,18 +2650,25 @@ static int count_resident(tk_ctx* c, tk_table* t, const tk_reads* r, const Reads
     TK_HIP(hipMemsetAsync(c->d_flags, 0, 64, c->stream));
     // persistent: as many blocks per CU as its registers and LDS allow
     const uint64_t grid = std::max<uint64_t>(8, std::min<uint64_t>((ceil_div(qe - q, kReadsPerBlock) + 7) / 8 * 8, mgrid));
+    const int table_ok = t->zero_pending ? 0 : 1;
     if (t->k >= 16 && merge_wpr() == 2)
       TK_LAUNCH(c, "count_merge", grid, 256, kResSlots * 8, (k_count_agg<kResSlots, true, true, 2>), rview, q, qe,
-                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor, table_ok);
     else if (t->k >= 16)
       TK_LAUNCH(c, "count_merge", grid, 256, kResSlots * 8, (k_count_agg<kResSlots, true, true>), rview, q, qe,
-                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor, table_ok);
     else
       TK_LAUNCH(c, "count_merge", grid, 256, kResSlots * 8, (k_count_agg<kResSlots, true, false>), rview, q, qe,
-                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor);
+                t->k, t->canonical, v, c->d_flags, A, cap, echunk, cursor, table_ok);
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 64, hipMemcpyDeviceToHost, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
     if (c->h_flags[0]) return fail(TK_ERR_FULL, "count: table probe limit reached (flags=%u)", c->h_flags[0]);
+    if (c->h_flags[6]) {  // a flush past the stream was dropped: clear the table, merge this chunk again
+      if (!t->zero_pending) return fail(TK_ERR_STATE, "count: merge dropped words on a cleared table");
+      if ((rc = table_zero(c, t))) return rc;
+      qe = q;
+      continue;
+    }
     uint64_t nnew = 0, n = 0;
     memcpy(&nnew, &c->h_flags[2], 8);
     memcpy(&n, &c->h_flags[8], 8);
@@ -2837,10 +2847,10 @@ int tk_count(tk_ctx* c, tk_table* t, const tk_reads* r) {
     const bool agg = rview.order && 63 - 2 * t->k >= 13;
     if (agg && t->k >= 16)
       TK_LAUNCH(c, "count", grid, 256, kAggSlots * 8, (k_count_agg<kAggSlots, false, true, 2>), rview, rb,
-                rb + nreads, t->k, t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
+                rb + nreads, t->k, t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr, 1);
     else if (agg)
       TK_LAUNCH(c, "count", grid, 256, kAggSlots * 8, (k_count_agg<kAggSlots, false, false>), rview, rb,
-                rb + nreads, t->k, t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr);
+                rb + nreads, t->k, t->canonical, v, c->d_flags, nullptr, 0, 0, nullptr, 1);
     else
       TK_LAUNCH(c, "count", grid, 256, 0, k_count, rview, rb, rb + nreads, t->k, t->canonical, v, c->d_flags);
     TK_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, 16, hipMemcpyDeviceToHost, c->stream));
