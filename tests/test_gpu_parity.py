@@ -176,7 +176,10 @@ def test_count_resident_in_chunks(api, ctx, mode, monkeypatch):
     its words onto the regions the earlier chunks wrote.  "reads": chunks of
     2,500 reads, every word in the streams; "words": streams of 200,000
     words, shorter than a chunk's words, so flushes past their end are added
-    with table atomics.  A second count adds onto the non-empty table."""
+    with table atomics; on the new table, not yet cleared, the first chunk's
+    overflowing flushes are dropped and the chunk is merged again after the
+    table is cleared (one merge launch more than region passes).  A second
+    count adds onto the non-empty table."""
     say("count_resident_in_chunks", mode)
     monkeypatch.setenv("TK_PARTITION", "0")
     monkeypatch.setenv("TK_ORDER", "1")
@@ -188,7 +191,15 @@ def test_count_resident_in_chunks(api, ctx, mode, monkeypatch):
     else:
         monkeypatch.setenv("TK_RES_STREAM_WORDS", "200000")
     seqs = random_reads(17, 20000, 100, 150, n_rate=0.001) + repetitive_reads(25, 3000)
+    ctx.timing(True)
+    ctx.timing_reset()
     r, t, seq, off = _count(api, ctx, seqs, 25, True)
+    ctx.sync()
+    merges = ctx.kernel_time("count_merge")[1]
+    regions = ctx.kernel_time("count_regions")[1]
+    ctx.timing(False)
+    assert regions > 1
+    assert merges == regions + (1 if mode == "words" else 0), (merges, regions)
     ok, oc = O.count(seq, off, 25, True)
     assert ctx.count_words() > 0
     gk, gc = t.dump()
