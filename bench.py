@@ -60,7 +60,7 @@ TRAFFIC_B_PER_WIN = {"count": (121.5 + 83.4) / 5.04, "stats": (143.49 + 1.61) / 
 # face value (their random 64-B line reads read as 64 B each, atomics as 32 B
 # of WRITE_SIZE).
 STRESS_TRAFFIC_B_PER_WIN = {"count": (880.30 + 574.82) / 37.8, "stats": (861.49 + 12.01) / 37.8,
-                            "vote": (920.67 + 116.66) / 37.8, "count_merge": (87.79 + 115.15) / 37.8,
+                            "vote": (790.17 + 14.40) / 37.8, "count_merge": (87.79 + 115.15) / 37.8,
                             "count_scatter": (130.83 + 307.42) / 37.8, "count_hist": (116.33 + 14.80) / 37.8,
                             "count_regions": (103.86 + 17.54 + 129.94 + 64.96) / 37.8}
 
