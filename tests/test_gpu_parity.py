@@ -914,6 +914,44 @@ def test_server_general_job_selections(api, ctx):
         srv.close()
 
 
+def test_server_job_out_of_device_memory(api, ctx, monkeypatch):
+    """A job whose device allocations fail (forced from the host: the
+    library's bytes in use may not pass TK_DEVICE_ALLOC_LIMIT) reports
+    TK_ERR_OOM through wait() and gives its admission back; a small job
+    submitted beside it, and the large job submitted again once the limit is
+    lifted, equal the same jobs run alone."""
+    say("server_oom")
+    from trinityrnaseq_amd import synth
+    samples = []
+    for j, n in enumerate([4000, 150000]):
+        tx = synth.large_transcriptome(3100 + j, 800)
+        buf = np.empty(2 * n * 150, np.uint8)
+        synth.pe_reads(tx, n, 150, seed=3100 + j, out=buf)
+        off = np.arange(2 * n + 1, dtype=np.uint64) * np.uint64(150)
+        samples.append((buf, off, n))
+    alone = [_plain_job(api, ctx, b, o, n) for b, o, n in samples]
+    import gc
+    gc.collect()  # the plain jobs' device blocks are freed before the limit is set
+    srv = api.Server(slots=2, hbm_budget_bytes=8 << 30)
+    try:
+        # room for the small job (about 16 MB) but not for the large one's
+        # 45 MB of reads, whatever the other holds when it asks
+        monkeypatch.setenv("TK_DEVICE_ALLOC_LIMIT", str(api.device_bytes_in_use() + (40 << 20)))
+        big = srv.submit(*samples[1])
+        small = srv.submit(*samples[0])
+        with pytest.raises(api.TkError) as err:
+            srv.wait(big)
+        assert err.value.code == -3  # TK_ERR_OOM
+        res, sel = srv.wait(small)
+        assert res.n_selected == alone[0][1] and np.array_equal(sel, alone[0][0])
+        monkeypatch.delenv("TK_DEVICE_ALLOC_LIMIT")
+        again = srv.submit(*samples[1])
+        res, sel = srv.wait(again)
+        assert res.n_selected == alone[1][1] and np.array_equal(sel, alone[1][0])
+    finally:
+        srv.close()
+
+
 def test_server_results_equal_alone_and_in_the_mix(api, ctx):
     """Jobs admitted concurrently (tight budget forces queueing) give exactly
     the selections of the same jobs run alone."""

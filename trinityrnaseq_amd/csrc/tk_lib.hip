@@ -78,9 +78,22 @@ void cache_end() {
   delete c;
 }
 
+// TK_DEVICE_ALLOC_LIMIT=n: an allocation that would take the library's
+// device bytes in use past n fails with TK_ERR_OOM, as an exhausted device
+// would (the out-of-memory paths are tested with it).
+static uint64_t alloc_limit() {
+  const char* e = getenv("TK_DEVICE_ALLOC_LIMIT");
+  return e ? strtoull(e, nullptr, 10) : 0;
+}
+
 int dev_alloc(void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;
+  if (const uint64_t lim = alloc_limit()) {
+    if (g_dev_bytes.load() + bytes > lim)
+      return fail(TK_ERR_OOM, "device allocation of %zu bytes past the limit of %llu bytes (TK_DEVICE_ALLOC_LIMIT)",
+                  bytes, (unsigned long long)lim);
+  }
   size_t cap = bytes;
   if (BlockCache* c = t_cache) {
     auto it = c->free_blocks.lower_bound(bytes);
