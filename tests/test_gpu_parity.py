@@ -332,6 +332,7 @@ def test_per_read_passes_ticket_modes(api, ctx, small_se, grab, pipe, monkeypatc
     say("ticket", grab, "pipe", pipe)
     monkeypatch.setenv("TK_READ_TICKET", grab)
     monkeypatch.setenv("TK_STATS_PIPE", pipe)
+    monkeypatch.setenv("TK_VOTE_PIPE", pipe)
     seq0, off0 = small_se
     short = edge_reads() + random_reads(92, 300, 0, 152, n_rate=0.01) + ["ACGTTGCA" * 19]  # 152 bases: 128 windows
     long_ = random_reads(91, 30, 200, 3000, n_rate=0.002)

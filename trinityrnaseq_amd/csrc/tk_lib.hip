@@ -1803,6 +1803,110 @@ __global__ __launch_bounds__(256) void k_vote(ReadsView rv, int k, int canonical
                          });
 }
 
+// k_vote over ticket runs for reads of at most 128 windows (unpacked
+// component table), software-pipelined over each wave's reads as
+// k_stats_pipe: read j+1's code words, read j+2's length and first word and
+// read j+3's index load while read j votes (64 windows per lookup round).
+__global__ __launch_bounds__(256) void k_vote_pipe(ReadsView rv, int k, int canonical,
+                                                   const uint64_t* __restrict__ slots, uint64_t bmask,
+                                                   tk_r2t_vote* __restrict__ out, uint32_t* tickets, uint32_t grab) {
+  __shared__ uint32_t s_vote[4][192];  // per wave: 128 hits, 64 lookup scratch
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  uint32_t* hits = s_vote[wv];
+  uint32_t* tmp = s_vote[wv] + 128;
+  const uint64_t occ = 1ull << (2 * k);
+  const uint64_t per_xcd = (rv.n_reads + 7) >> 3;
+  const uint64_t x = blockIdx.x & 7;
+  const uint64_t lo = x * per_xcd;
+  const uint64_t hi = lo + per_xcd < rv.n_reads ? lo + per_xcd : rv.n_reads;
+  if (lo >= hi) return;
+  const uint32_t n = (uint32_t)(hi - lo);
+  TicketStream ts(tickets + x * kTicketStride, n, grab, lane);
+  bool va = false, vb = false, vc = false;
+  uint32_t ra = 0, rb = 0, rc = 0, Lb = 0, Lc = 0, mwc = 0;
+  uint64_t wbb = 0, cwc = 0;
+  for (;;) {
+    const uint32_t p = ts.next();
+    const bool vn = p < n;
+    uint32_t rn = 0;
+    if (vn) rn = rv.order ? rv.order[lo + p] : (uint32_t)(lo + p);
+    uint32_t La = 0;
+    uint64_t wba = 0;
+    if (va) {
+      La = rv.len[ra];
+      wba = rv.word_off[ra];
+    }
+    uint64_t cwb = 0;
+    uint32_t mwb = 0;
+    if (vb && lane < 6 && (uint32_t)lane <= ((Lb + 31) >> 5)) {
+      cwb = rv.codes[wbb + lane];
+      mwb = rv.nmask[wbb + lane];
+    }
+    if (vc) {
+      const uint32_t nwin = Lc >= (uint32_t)k ? Lc - k + 1 : 0;
+      uint32_t nh = 0;
+      for (uint32_t base = 0; base < nwin; base += 64) {
+        const uint32_t pos = base + lane;
+        uint64_t f = 0;
+        bool valid = false;
+        window_lanes(cwc, mwc, pos, k, f, valid);
+        valid = valid && pos < nwin;
+        const uint64_t key = valid ? canon_key(f, k, canonical) : 0;
+        const uint32_t cmp = coop_cget64(slots, 0, bmask, occ, key, valid, tmp, lane);
+        const bool has = cmp != 0xFFFFFFFFu;
+        const uint64_t m = __ballot(has);
+        if (has) hits[nh + __popcll(m & ((1ull << lane) - 1))] = cmp;
+        nh += (uint32_t)__popcll(m);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      uint32_t best = 0xFFFFFFFFu, bestn = 0;
+      uint64_t prev = 0;
+      bool first = true;
+      while (nh) {
+        uint32_t cand = 0xFFFFFFFFu;
+        for (uint32_t i = lane; i < nh; i += 64) {
+          const uint32_t v = hits[i];
+          if ((first || v > (uint32_t)prev) && v < cand) cand = v;
+        }
+        cand = wave_min_u32(cand);
+        if (cand == 0xFFFFFFFFu) break;
+        uint32_t cnt = 0;
+        for (uint32_t i = lane; i < nh; i += 64) cnt += hits[i] == cand;
+        cnt = wave_sum_u32(cnt);
+        if (cnt > bestn) {
+          bestn = cnt;
+          best = cand;
+        }
+        prev = cand;
+        first = false;
+      }
+      if (lane == 0) {
+        tk_r2t_vote v;
+        v.comp = best;
+        v.hits = bestn;
+        v.nwin = nwin;
+        out[rc] = v;
+      }
+      wave_sync();
+    }
+    vc = vb;
+    rc = rb;
+    Lc = Lb;
+    cwc = cwb;
+    mwc = mwb;
+    vb = va;
+    rb = ra;
+    Lb = La;
+    wbb = wba;
+    va = vn;
+    ra = rn;
+    if (!va && !vb && !vc) break;
+  }
+}
+
 // Grid of the per-read table passes (k_stats, k_vote): at most TK_READ_BLOCKS_PER_CU
 // (default 128, the fastest of 4..128 and unbounded; 0 = one block per wpb
 // reads) blocks per CU.
@@ -1822,6 +1926,13 @@ constexpr size_t kFlagBytes = (64 + kTicketWords) * 4;
 // (TK_STATS_PIPE, default 1; 0 = k_stats_short, one read at a time).
 static bool stats_pipe() {
   const char* e = getenv("TK_STATS_PIPE");
+  return !e || e[0] != '0';
+}
+
+// Voting reads of at most 128 windows with the software-pipelined kernel
+// (TK_VOTE_PIPE, default 1; 0 = k_vote, one read at a time).
+static bool vote_pipe() {
+  const char* e = getenv("TK_VOTE_PIPE");
   return !e || e[0] != '0';
 }
 
@@ -3474,7 +3585,11 @@ int tk_r2t_vote_reads_d(tk_ctx* c, const tk_ctable* t, const tk_reads* r, tk_r2t
   uint32_t* tickets = grab ? c->d_tickets : nullptr;
   if (tickets) TK_LAUNCH(c, "tickets", 1, 256, 0, k_fill_u32, tickets, kTicketWords, 0u);
   const size_t lds = (size_t)slab * 4 * wpb + (size_t)wpb * 256;
-  if (tickets)
+  if (tickets && max_nwin <= 128 && vote_pipe()) {
+    const int pgrid = (per_read_grid(c, r->n_reads, 4) + 7) / 8 * 8;
+    TK_LAUNCH(c, "vote", pgrid, 256, 0, k_vote_pipe, reads_view(r), t->k, t->canonical, t->d_slots,
+              t->nbuckets - 1, d_out, tickets, grab);
+  } else if (tickets)
     TK_LAUNCH(c, "vote", grid, wpb * 64, lds, k_vote<true>, reads_view(r), t->k, t->canonical, t->d_slots,
               t->nbuckets - 1, d_out, slab, tickets, grab);
   else
